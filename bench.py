@@ -1,0 +1,201 @@
+"""Benchmark: Boruvka MST + union-find CC on RMAT n=2^24, m=2^28 on one MI355X.
+
+One step = one minimum-spanning-forest solve (forest ids, exact weight,
+min-id labels) plus one union-find connected-components solve (labels +
+spanning forest) over the same device-resident graph.  Inputs are generated
+on the device before the timed region (RMAT a,b,c,d = .57,.19,.19,.05, seed 2,
+weights k * 2^-24 in (0,1]).
+
+Prints one JSON line.  `value` = m / (time per step) in edges/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "phem-graph_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (scale, log2 m, seed)
+    "rmat_n2^24_m2^28": (24, 28, 2),
+    "rmat_n2^20_m2^24": (20, 24, 2),
+}
+
+
+def mst_kernel_bytes(m_log, n):
+    """Algorithmic bytes per solve of each MST kernel from the round log.
+    sweep0 reads (u, v, w) once; compaction round r reads 12 B (r = 0) or
+    16 B (r > 0) per alive edge and writes 16 B per survivor; labels read the
+    parent array and write labels."""
+    m0 = m_log[0] if m_log else 0
+    nxt = list(m_log[1:]) + [0]
+    b = {"k_sweep0": 12 * m0,
+         "k_compact<R0>": 12 * m0 + 16 * nxt[0],
+         "k_compact": sum(16 * m_log[r] + 16 * nxt[r] for r in range(1, len(m_log))),
+         "k_labels": 8 * n}
+    return b
+
+
+def b_mst(m, n, m_log, c_log):
+    """Algorithmic bytes of a whole MST solve: 32 S_m - 20 m + 16 S_c + 12 n."""
+    return 32 * sum(m_log) - 20 * m + 16 * sum(c_log) + 12 * n
+
+
+def b_cc(m, n):
+    return 8 * m + 12 * n
+
+
+def cpu_baseline(u_np, v_np, w_np, n, threads):
+    """The CPU restatement (OpenMP PBBS filter-Kruskal + HAST union-find) on
+    a prefix of the same edge stream; returns edges/s of MST + CC."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t0 = time.perf_counter()
+    O.par_mst(n, u_np, v_np, w_np, 1, threads=threads)
+    t1 = time.perf_counter()
+    O.par_cc(n, u_np, v_np, threads=threads)
+    t2 = time.perf_counter()
+    return len(u_np) / (t2 - t0), t1 - t0, t2 - t1
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat_n2^24_m2^28", choices=list(WORKLOADS))
+    ap.add_argument("--cpu-sample-log2", type=int, default=25,
+                    help="edges of the CPU baseline sample (prefix of the stream)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "multi-GPU sharding is described in DESIGN.md and not run", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import phem_native as P
+
+    P.require_device()
+    scale, lm, seed = WORKLOADS[args.workload]
+    n, m = 1 << scale, 1 << lm
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    print(f"# generating {args.workload} on device", file=sys.stderr, flush=True)
+    du, dv, dw = P.gen_rmat(scale, m, seed, positive=True, device=dev)
+    de = P.pack_edges(du, dv)
+    solver = P.MstSolver(n, m, device=dev)
+    cc_par = torch.empty(n, dtype=torch.int32, device=dev)
+    cc_lab = torch.empty(n, dtype=torch.int32, device=dev)
+    cc_forest = torch.empty(n, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        ids, lab, st = solver.solve(du, dv, dw, P.W_F32)
+        P._check(P.lib().phem_cc_solve(n, P._p(de), m, P._p(cc_par), P._p(cc_forest),
+                                       P._p(cc_lab), P._stream()), "phem_cc_solve")
+        return st
+
+    for i in range(args.warmup):
+        st = step()
+        print(f"# warmup {i}: rounds {st['rounds']} forest {st['mst_count']}",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+
+    # separate MST / CC phase times on their own (events on the library's stream)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    st = solver.solve(du, dv, dw, P.W_F32)[2]
+    ev[1].record(stream)
+    P._check(P.lib().phem_cc_solve(n, P._p(de), m, P._p(cc_par), P._p(cc_forest),
+                                   P._p(cc_lab), P._stream()), "phem_cc_solve")
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    mst_ms = ev[0].elapsed_time(ev[1])
+    cc_ms = ev[1].elapsed_time(ev[2])
+
+    P.prof_reset()
+    P.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    P.prof_enable(False)
+    prof = P.prof_read()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = m / (ms_per_step * 1e-3)
+
+    # roofline of the dominant kernel (largest share of the timed region)
+    kb = mst_kernel_bytes(st["m_log"], n)
+    kb["k_cc_union"] = 8 * m
+    kb["k_cc_labels"] = 8 * n
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    dname, (dlaunch, dms) = dom
+    per_launch_ms = dms / dlaunch
+    launches_per_step = dlaunch / args.steps
+    if dname in kb:
+        bytes_per_launch = kb[dname] / launches_per_step
+        achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+    else:
+        bytes_per_launch, achieved = None, None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(args.workload, {}).get(dname)
+        except (OSError, ValueError):
+            traffic = None
+    roof = {"bound": "hbm", "kernel": dname,
+            "achieved": None if achieved is None else round(achieved, 1),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": None if achieved is None else round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": round(per_launch_ms, 4)}
+
+    bm, bc = b_mst(m, n, st["m_log"], st["c_log"]), b_cc(m, n)
+    out = {
+        "metric": "MST/CC edges/s at RMAT n=2^24 m=2^28; batched vs plain; p50/p99 latency",
+        "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": args.workload, "n": n, "m": m,
+                   "rmat_abcd": [0.57, 0.19, 0.19, 0.05], "weights": "k*2^-24 in (0,1]",
+                   "step": "MST solve + union-find CC solve"},
+        "mst": {"ms": round(mst_ms, 4), "edges_per_s": m / (mst_ms * 1e-3),
+                "rounds": st["rounds"], "forest": st["mst_count"],
+                "B_bytes": bm, "roofline_frac": round(bm / (mst_ms * 1e-3) / 8e12, 4),
+                "m_log": st["m_log"], "c_log": st["c_log"]},
+        "cc": {"ms": round(cc_ms, 4), "edges_per_s": m / (cc_ms * 1e-3), "B_bytes": bc,
+               "roofline_frac": round(bc / (cc_ms * 1e-3) / 8e12, 4)},
+        "kernels_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in
+                                sorted(prof.items(), key=lambda kv: -kv[1][1])},
+        "roofline": roof,
+    }
+
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        k = min(m, 1 << args.cpu_sample_log2)
+        u_np = du[:k].cpu().numpy().view(np.uint32)
+        v_np = dv[:k].cpu().numpy().view(np.uint32)
+        w_np = dw[:k].cpu().numpy().view(np.uint32)
+        print(f"# cpu baseline on {k} edges, {threads} threads", file=sys.stderr, flush=True)
+        eps, t_mst, t_cc = cpu_baseline(u_np, v_np, w_np, n, threads)
+        out["cpu_baseline"] = {"value": eps, "unit": "edges/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"first {k} edges of the same RMAT stream on n={n}: "
+                                         f"OpenMP PBBS filter-Kruskal MST {t_mst:.2f} s + "
+                                         f"HAST union-find CC {t_cc:.2f} s"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

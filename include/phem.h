@@ -1,0 +1,164 @@
+/*
+ * phem.h -- C ABI of the MI355X (gfx950) implementation of PHEM's graph hot
+ * path: Boruvka minimum spanning forest and union-find connected components /
+ * spanning forest, single-graph, streaming and batched.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Pointers named d_* are device (HBM)
+ *    pointers; h_* are host pointers.  `stream` is a hipStream_t (NULL = the
+ *    null stream).
+ *  - Every entry point returns PHEM_OK (0) or a negative PHEM_E* code; none
+ *    throws or exits.  phem_last_error() gives a message for the last failure
+ *    on the calling thread.
+ *  - An MST edge i is (u[i], v[i], w[i]), its id is i.  Edges with u == v or
+ *    with an endpoint >= n are not part of the graph and are ignored.
+ *    Duplicate edges are kept.
+ *  - Weights are 32-bit patterns compared through an order key:
+ *    PHEM_W_U32 = unsigned integers (the reference's `typedef uint32_t
+ *    weight`, MST/src/constants.h:9); PHEM_W_F32 = IEEE float32 in the
+ *    IEEE total order.  Edges are ordered by (weight, id): this makes the
+ *    minimum spanning forest unique and equal to the reference's
+ *    (MST/src/utils.cu:801-802; the GPU path's (weight, position) order with
+ *    stable compaction, MST/src/kernels.cu:24-70,641-646, is the same order).
+ *  - Component labels are the minimum vertex id of each component (the
+ *    reference hooks larger root -> smaller root,
+ *    CC/ExternalSpanningTree.cu:52-53, CC/host_spanning_tree.cu:31-38).
+ *  - Edge ids are 32-bit: m <= PHEM_MAX_EDGES.  n <= PHEM_MAX_VERTICES.
+ */
+#ifndef PHEM_H
+#define PHEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHEM_OK 0
+#define PHEM_EINVAL (-1)    /* bad argument */
+#define PHEM_ENOMEM (-2)    /* workspace too small / allocation failed */
+#define PHEM_EHIP (-3)      /* HIP runtime error */
+#define PHEM_ENOCONV (-4)   /* round cap reached */
+#define PHEM_ENODEV (-5)    /* no usable gfx950 device */
+#define PHEM_EBUSY (-6)     /* admission refused (serving) */
+#define PHEM_ENOTFOUND (-7) /* unknown request / handle */
+
+#define PHEM_W_U32 0
+#define PHEM_W_F32 1
+
+#define PHEM_MAX_EDGES 0xFFFFFFFEull
+#define PHEM_MAX_VERTICES 0xFFFFFFFEu
+#define PHEM_NO_EDGE 0xFFFFFFFFFFFFFFFFull /* empty spanning-forest slot */
+#define PHEM_MAX_ROUNDS 64
+
+/* Library / device information. */
+const char *phem_version(void);
+const char *phem_last_error(void);
+/* 0 if a gfx950 device is visible and the kernels can launch on it. */
+int phem_device_check(int *device_count);
+
+/* Per-kernel timing with HIP events recorded around every launch on the
+ * launch's own stream (no counterpart in the reference, which times whole
+ * phases with cudaEvents, CC/ExternalSpanningTree.cu:147-151).  Off by
+ * default.  phem_prof_count() waits for the recorded events and returns the
+ * number of distinct kernels seen since the last reset. */
+int phem_prof_enable(int on);
+int phem_prof_reset(void);
+int phem_prof_count(void);
+int phem_prof_get(int i, char *name, int name_cap, uint64_t *launches,
+                  double *total_ms);
+
+/* ---------------------------------------------------------------------------
+ * Boruvka minimum spanning forest (one graph)
+ *
+ * Replaces the plain GPU path boruvka_for_edge_array / device_boruvka
+ * (MST/src/new_boruvka.cu:40-179, MST/src/kernels.cu:507-689).
+ * ------------------------------------------------------------------------- */
+
+typedef struct phem_mst_stats {
+  uint32_t rounds;        /* Boruvka rounds executed */
+  uint32_t mst_count;     /* edges in the forest */
+  uint64_t weight_fx;     /* exact total weight: U32 sum of w; F32 sum of w*2^24 */
+  int32_t weight_exact;   /* F32: 0 if a weight is negative or off the 2^-24 grid */
+  int32_t reserved;
+  uint64_t sum_m;         /* S_m = sum over rounds of alive edges */
+  uint64_t sum_c;         /* S_c = sum over rounds of active components */
+  uint64_t m_log[PHEM_MAX_ROUNDS]; /* alive edges at the start of round r */
+  uint64_t c_log[PHEM_MAX_ROUNDS]; /* components with >= 1 alive edge in round r */
+} phem_mst_stats;
+
+/* Bytes of device workspace phem_mst_solve needs for (n, m). */
+int phem_mst_workspace_bytes(uint32_t n, uint64_t m, uint64_t *bytes);
+
+/* Minimum spanning forest of (d_u, d_v, d_w).  Writes the forest's edge ids in
+ * ascending order to d_mst_ids (capacity >= n), and min-id labels to d_labels
+ * (n entries; may be NULL).  d_ws: device workspace of >= the queried bytes.
+ * Blocks the calling thread until the result is in HBM (the round loop reads
+ * one counter per round); stats (host, may be NULL) receives the round log. */
+int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
+                   const uint32_t *d_v, const uint32_t *d_w, int wmode,
+                   void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
+                   uint32_t *d_labels, phem_mst_stats *stats, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Union-find connected components / spanning forest
+ *
+ * Edges are packed uint64 (u << 32) | v (CC/graph.cu:74-77).  As in the
+ * reference kernel (CC/ExternalSpanningTree.cu:77-86) only records with u < v
+ * are processed; records with v >= n are ignored.  The forest holds, for each
+ * vertex that was hooked as a root, the packed edge that hooked it
+ * (T1edges, CC/ExternalSpanningTree.cu:52-55), else PHEM_NO_EDGE; which edge is
+ * chosen depends on the schedule, as in the reference.
+ * ------------------------------------------------------------------------- */
+
+/* parent[i] = i, forest[i] = PHEM_NO_EDGE (init_parent_label_T1edges,
+ * CC/ExternalSpanningTree.cu:108-116).  d_forest may be NULL. */
+int phem_cc_init(uint32_t n, uint32_t *d_parent, uint64_t *d_forest,
+                 void *stream);
+/* Union all m edges into the parent forest (union_find_gpu_COO,
+ * CC/ExternalSpanningTree.cu:77-86).  Asynchronous on `stream`.  The parent
+ * array stays valid between calls, so batches can be applied one after the
+ * other (CC/ExternalSpanningTree.cu:170-204). */
+int phem_cc_union(uint32_t n, const uint64_t *d_edges, uint64_t m,
+                  uint32_t *d_parent, uint64_t *d_forest, void *stream);
+/* labels[i] = root of i = minimum vertex id of i's component (`cc`,
+ * CC/GPUSpanningTree.cu:122-128).  Asynchronous. */
+int phem_cc_labels(uint32_t n, const uint32_t *d_parent, uint32_t *d_labels,
+                   void *stream);
+/* init + union + labels.  Asynchronous. */
+int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
+                  uint32_t *d_parent, uint64_t *d_forest, uint32_t *d_labels,
+                  void *stream);
+/* Merge the device forest d_forest (n slots) with n extra candidate edges
+ * (the CPU-side forest) into a second forest d_forest2 and recompute parent
+ * and labels (gpu_spanning_tree, CC/GPUSpanningTree.cu:74-96,152-158,169-198).
+ * Asynchronous. */
+int phem_cc_merge_forests(uint32_t n, const uint64_t *d_forest,
+                          const uint64_t *d_extra, uint32_t *d_parent,
+                          uint64_t *d_forest2, uint32_t *d_labels,
+                          void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Synthetic inputs on the device (counter-based splitmix64 streams; the same
+ * bytes as the host generator on every host).  Edges are stored with u < v.
+ * ------------------------------------------------------------------------- */
+
+/* m uniform edges over [0, n); ids id0 .. id0+m-1 index the streams.
+ * positive_w: weights in (0,1] instead of [0,1). */
+int phem_gen_uniform(uint32_t n, uint64_t m, uint64_t seed, uint64_t id0,
+                     int positive_w, uint32_t *d_u, uint32_t *d_v,
+                     uint32_t *d_w, void *stream);
+/* m RMAT edges over 2^scale vertices with cumulative 32-bit thresholds
+ * t1 = a*2^32, t2 = (a+b)*2^32, t3 = (a+b+c)*2^32. */
+int phem_gen_rmat(uint32_t scale, uint64_t m, uint64_t seed, uint32_t t1,
+                  uint32_t t2, uint32_t t3, int positive_w, uint32_t *d_u,
+                  uint32_t *d_v, uint32_t *d_w, void *stream);
+/* Pack (u, v) into (u << 32) | v records. */
+int phem_pack_edges(const uint32_t *d_u, const uint32_t *d_v, uint64_t m,
+                    uint64_t *d_edges, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHEM_H */

@@ -1,0 +1,152 @@
+// phem_cc.hip -- union-find connected components / spanning forest for gfx950.
+//
+// Semantics follow the reference HAST GPU path: records are packed
+// (u << 32) | v and only u < v is processed (union_find_gpu_COO,
+// CC/ExternalSpanningTree.cu:77-86); the larger root is hooked under the
+// smaller by compare-and-swap and the hooking edge is stored at the hooked
+// root (union_async, CC/ExternalSpanningTree.cu:46-59); labels are roots, i.e.
+// minimum vertex ids (cc, CC/GPUSpanningTree.cu:122-128).  Finds use path
+// halving instead of the reference's two-pass compression
+// (custom_compress, CC/ExternalSpanningTree.cu:27-43): both only ever move a
+// pointer to an ancestor.
+#include "phem_dev.h"
+#include "phem_internal.h"
+
+namespace phem {
+
+constexpr int CBLK = 256;
+constexpr int CEPT = 8;
+constexpr int CTILE = CBLK * CEPT;
+
+__global__ __launch_bounds__(CBLK) void k_cc_init(uint32_t n, uint32_t *par,
+                                                  ull *forest) {
+  for (uint64_t i = (uint64_t)blockIdx.x * CBLK + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * CBLK) {
+    par[i] = (uint32_t)i;
+    if (forest) forest[i] = KEY_NONE;
+  }
+}
+
+__device__ __forceinline__ void cc_edge(ull e, uint32_t n, uint32_t *par,
+                                        ull *forest) {
+  const uint32_t a = (uint32_t)(e >> 32), b = (uint32_t)e;
+  if (a < b && b < n) {
+    uint32_t h = uf_union(par, a, b);
+    if (forest && h != U32_NONE) forest[h] = e;
+  }
+}
+
+__global__ __launch_bounds__(CBLK) void k_cc_union(const ull *__restrict__ edges,
+                                                   uint64_t m, uint32_t n,
+                                                   uint32_t *par, ull *forest) {
+  for (uint64_t tile = blockIdx.x; tile * CTILE < m; tile += gridDim.x) {
+    const uint64_t base = tile * CTILE + threadIdx.x;
+    ull e[CEPT];
+#pragma unroll
+    for (int j = 0; j < CEPT; j++) {
+      uint64_t i = base + (uint64_t)j * CBLK;
+      e[j] = i < m ? __builtin_nontemporal_load(&edges[i]) : KEY_NONE;
+    }
+#pragma unroll
+    for (int j = 0; j < CEPT; j++) cc_edge(e[j], n, par, forest);
+  }
+}
+
+__global__ __launch_bounds__(CBLK) void k_cc_labels(uint32_t n,
+                                                    const uint32_t *__restrict__ par,
+                                                    uint32_t *__restrict__ labels) {
+  for (uint64_t i = (uint64_t)blockIdx.x * CBLK + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * CBLK)
+    labels[i] = uf_root(par, (uint32_t)i);
+}
+
+// Union over the 2n candidates forest[0..n) ++ extra[0..n)
+// (union_find_gpu, CC/GPUSpanningTree.cu:74-96).
+__global__ __launch_bounds__(CBLK) void k_cc_merge(uint32_t n,
+                                                   const ull *__restrict__ f1,
+                                                   const ull *__restrict__ extra,
+                                                   uint32_t *par, ull *f2) {
+  const uint64_t tot = 2ull * n;
+  for (uint64_t i = (uint64_t)blockIdx.x * CBLK + threadIdx.x; i < tot;
+       i += (uint64_t)gridDim.x * CBLK) {
+    ull e = i < n ? f1[i] : extra[i - n];
+    if (e != KEY_NONE) cc_edge(e, n, par, f2);
+  }
+}
+
+}  // namespace phem
+
+using namespace phem;
+
+extern "C" int phem_cc_init(uint32_t n, uint32_t *d_parent, uint64_t *d_forest,
+                            void *stream) {
+  if (n > PHEM_MAX_VERTICES) return set_err(PHEM_EINVAL, "n=%u too large", n);
+  if (n && !d_parent) return set_err(PHEM_EINVAL, "d_parent is NULL");
+  if (!n) return PHEM_OK;
+  {
+    PHEM_PROF("k_cc_init", (hipStream_t)stream);
+    hipLaunchKernelGGL(k_cc_init, dim3(grid_for(n, CBLK)), dim3(CBLK), 0,
+                       (hipStream_t)stream, n, d_parent, (ull *)d_forest);
+  }
+  PHEM_LAUNCHED("k_cc_init");
+  return PHEM_OK;
+}
+
+extern "C" int phem_cc_union(uint32_t n, const uint64_t *d_edges, uint64_t m,
+                             uint32_t *d_parent, uint64_t *d_forest,
+                             void *stream) {
+  if (n > PHEM_MAX_VERTICES) return set_err(PHEM_EINVAL, "n=%u too large", n);
+  if (m && (!d_edges || !d_parent))
+    return set_err(PHEM_EINVAL, "edges or parent is NULL");
+  if (!m || !n) return PHEM_OK;
+  {
+    PHEM_PROF("k_cc_union", (hipStream_t)stream);
+    hipLaunchKernelGGL(k_cc_union, dim3(grid_for(m, CTILE)), dim3(CBLK), 0,
+                       (hipStream_t)stream, (const ull *)d_edges, m, n, d_parent,
+                       (ull *)d_forest);
+  }
+  PHEM_LAUNCHED("k_cc_union");
+  return PHEM_OK;
+}
+
+extern "C" int phem_cc_labels(uint32_t n, const uint32_t *d_parent,
+                              uint32_t *d_labels, void *stream) {
+  if (n && (!d_parent || !d_labels))
+    return set_err(PHEM_EINVAL, "parent or labels is NULL");
+  if (!n) return PHEM_OK;
+  {
+    PHEM_PROF("k_cc_labels", (hipStream_t)stream);
+    hipLaunchKernelGGL(k_cc_labels, dim3(grid_for(n, CBLK)), dim3(CBLK), 0,
+                       (hipStream_t)stream, n, d_parent, d_labels);
+  }
+  PHEM_LAUNCHED("k_cc_labels");
+  return PHEM_OK;
+}
+
+extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
+                             uint32_t *d_parent, uint64_t *d_forest,
+                             uint32_t *d_labels, void *stream) {
+  int rc = phem_cc_init(n, d_parent, d_forest, stream);
+  if (rc) return rc;
+  rc = phem_cc_union(n, d_edges, m, d_parent, d_forest, stream);
+  if (rc) return rc;
+  return d_labels ? phem_cc_labels(n, d_parent, d_labels, stream) : PHEM_OK;
+}
+
+extern "C" int phem_cc_merge_forests(uint32_t n, const uint64_t *d_forest,
+                                     const uint64_t *d_extra,
+                                     uint32_t *d_parent, uint64_t *d_forest2,
+                                     uint32_t *d_labels, void *stream) {
+  if (n && (!d_forest || !d_extra || !d_parent))
+    return set_err(PHEM_EINVAL, "forest, extra or parent is NULL");
+  int rc = phem_cc_init(n, d_parent, d_forest2, stream);
+  if (rc || !n) return rc;
+  {
+    PHEM_PROF("k_cc_merge", (hipStream_t)stream);
+    hipLaunchKernelGGL(k_cc_merge, dim3(grid_for(2ull * n, CBLK)), dim3(CBLK), 0,
+                       (hipStream_t)stream, n, (const ull *)d_forest,
+                       (const ull *)d_extra, d_parent, (ull *)d_forest2);
+  }
+  PHEM_LAUNCHED("k_cc_merge");
+  return d_labels ? phem_cc_labels(n, d_parent, d_labels, stream) : PHEM_OK;
+}

@@ -1,0 +1,170 @@
+// phem_dev.h -- device helpers shared by the MST, CC and generator kernels
+// (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace phem {
+
+constexpr uint64_t KEY_NONE = 0xFFFFFFFFFFFFFFFFull;
+constexpr uint32_t U32_NONE = 0xFFFFFFFFu;
+constexpr int WAVE = 64;
+
+typedef unsigned long long ull;
+
+// Order key of a 32-bit weight pattern: identity for unsigned weights, the
+// IEEE-754 total order for float32 (negatives reversed below positives).
+__device__ __forceinline__ uint32_t wkey(uint32_t w, int mode) {
+  return mode ? ((w & 0x80000000u) ? ~w : (w | 0x80000000u)) : w;
+}
+__device__ __forceinline__ uint32_t wkey_inv(uint32_t k, int mode) {
+  return mode ? ((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k) : k;
+}
+
+// ---- wave64 helpers --------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Number of set lanes below this lane in a 64-bit ballot.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wave-aggregated append: every lane with `pred` gets a distinct slot of
+// `*counter`; one atomic per wave.  Returns the slot (valid where pred).
+__device__ __forceinline__ uint64_t wave_append(bool pred, ull *counter) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return 0;
+  uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+  ull base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (ull)__popcll(mask));
+  base = __shfl(base, leader);
+  return base + lanes_below(mask);
+}
+
+// Block-aggregated append of IPT items per thread: one global atomic per
+// block call.  Every thread of the block must call it (uniform control flow).
+// slot[j] receives the slot of item j where pred[j] holds.
+template <int BLOCK, int IPT>
+__device__ __forceinline__ void block_append(const bool (&pred)[IPT],
+                                             uint64_t (&slot)[IPT],
+                                             ull *counter) {
+  constexpr int NW = BLOCK / WAVE;
+  __shared__ uint32_t s_c[IPT * NW];
+  __shared__ ull s_base;
+  const uint32_t lane = lane_id(), wv = threadIdx.x / WAVE;
+  uint64_t mask[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {
+    mask[j] = __ballot(pred[j]);
+    if (lane == 0) s_c[j * NW + wv] = (uint32_t)__popcll(mask[j]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int k = 0; k < IPT * NW; k++) {
+      uint32_t c = s_c[k];
+      s_c[k] = t;
+      t += c;
+    }
+    s_base = t ? atomicAdd(counter, (ull)t) : 0ull;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; j++)
+    slot[j] = s_base + s_c[j * NW + wv] + lanes_below(mask[j]);
+  __syncthreads();
+}
+
+// ---- minimum-key store ----------------------------------------------------
+
+// best[c] = min(best[c], key).  The plain read filters out the atomics that
+// cannot lower the value (best only decreases, so a stale read is an upper
+// bound and only costs an unnecessary atomic).
+__device__ __forceinline__ void min_store(ull *best, uint32_t c, uint64_t key) {
+  if (key < best[c]) atomicMin(&best[c], (ull)key);
+}
+
+// ---- union-find over a parent array where every pointer goes to a smaller
+// id (roots are the minimum vertex of their tree) ---------------------------
+//
+// Reads may observe an older value of an entry.  Every value an entry ever
+// holds is an ancestor of the entry, so an older value still lies on a path to
+// the true root; only the compare-and-swap on a root is authoritative.
+
+__device__ __forceinline__ uint32_t par_load(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void par_store(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// find with path halving
+__device__ __forceinline__ uint32_t uf_find(uint32_t *par, uint32_t x) {
+  uint32_t p = par_load(&par[x]);
+  while (p != x) {
+    uint32_t gp = par_load(&par[p]);
+    if (gp == p) return p;
+    par_store(&par[x], gp);
+    x = gp;
+    p = par_load(&par[x]);
+  }
+  return x;
+}
+
+// find without writes (for read-only walks)
+__device__ __forceinline__ uint32_t uf_root(const uint32_t *par, uint32_t x) {
+  uint32_t p = par[x];
+  while (p != x) {
+    x = p;
+    p = par[x];
+  }
+  return x;
+}
+
+// Union of the trees of a and b: the larger root is hooked under the smaller
+// by compare-and-swap.  Returns the hooked root, or U32_NONE if a and b were
+// already in one tree.
+__device__ __forceinline__ uint32_t uf_union(uint32_t *par, uint32_t a,
+                                             uint32_t b) {
+  a = uf_find(par, a);
+  b = uf_find(par, b);
+  while (a != b) {
+    uint32_t hi = a > b ? a : b;
+    uint32_t lo = a ^ b ^ hi;
+    uint32_t old = atomicCAS(&par[hi], hi, lo);
+    if (old == hi) return hi;
+    // hi was hooked meanwhile; `old` is its current parent
+    uint32_t r = uf_find(par, old);
+    if (hi == a) a = r; else b = r;
+  }
+  return U32_NONE;
+}
+
+// ---- counter-based random streams (splitmix64; Steele, Lea, Flood 2014) ----
+
+constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  uint64_t z = x + GOLDEN;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ __forceinline__ uint64_t stream_base(uint64_t seed,
+                                                         uint64_t stream) {
+  return mix64(seed * GOLDEN + stream);
+}
+__host__ __device__ __forceinline__ uint32_t mulhi_n(uint64_t r, uint32_t n) {
+  return (uint32_t)(((r >> 32) * (uint64_t)n) >> 32);
+}
+// weight k * 2^-24 with k in [0, 2^24) (or [1, 2^24] if positive)
+__host__ __device__ __forceinline__ uint32_t weight_bits(uint64_t r,
+                                                         int positive) {
+  uint32_t k = (uint32_t)(r >> 40) + (positive ? 1u : 0u);
+  float f = (float)k * 5.9604644775390625e-08f;  // 2^-24
+  return __builtin_bit_cast(uint32_t, f);
+}
+
+}  // namespace phem

@@ -1,0 +1,58 @@
+// phem_internal.h -- host-side helpers shared by the library's translation
+// units (error reporting, launch sizing).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "phem.h"
+
+namespace phem {
+
+// Records a formatted message for phem_last_error() and returns `code`.
+int set_err(int code, const char *fmt, ...);
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+inline uint64_t align_up(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
+
+// Grid size for `work` items at `per_block` items per block; kernels loop
+// (grid-stride) over anything beyond the cap.
+inline unsigned grid_for(uint64_t work, uint64_t per_block,
+                         uint64_t cap = 1u << 20) {
+  uint64_t g = ceil_div(work ? work : 1, per_block);
+  return (unsigned)(g < cap ? g : cap);
+}
+
+// Records HIP events around the launches in its scope when profiling is on.
+class ProfScope {
+ public:
+  ProfScope(const char *name, hipStream_t s);
+  ~ProfScope();
+  ProfScope(const ProfScope &) = delete;
+  ProfScope &operator=(const ProfScope &) = delete;
+
+ private:
+  const char *name_;
+  hipStream_t s_;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+  bool on_ = false;
+};
+
+}  // namespace phem
+
+#define PHEM_PROF(name, s) phem::ProfScope phem_prof_scope_(name, s)
+
+#define PHEM_HIP(x)                                                     \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess)                                                  \
+      return phem::set_err(PHEM_EHIP, "%s failed: %s (%s:%d)", #x,         \
+                           hipGetErrorString(e_), __FILE__, __LINE__);     \
+  } while (0)
+
+#define PHEM_LAUNCHED(name)                                                \
+  do {                                                                     \
+    hipError_t e_ = hipGetLastError();                                     \
+    if (e_ != hipSuccess)                                                  \
+      return phem::set_err(PHEM_EHIP, "launch of %s failed: %s", name,     \
+                           hipGetErrorString(e_));                         \
+  } while (0)

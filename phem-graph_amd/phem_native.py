@@ -1,0 +1,227 @@
+"""Python host side of the PHEM MI355X library: ctypes bindings of libphem.so.
+
+The graph work runs in the HIP kernels of libphem.so; torch only provides
+device memory and the current HIP stream.  There is no CPU fallback: every
+entry point raises PhemError when the library or a gfx950 device is missing.
+"""
+import ctypes
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphem.so")
+
+W_U32 = 0
+W_F32 = 1
+NO_EDGE = 0xFFFFFFFFFFFFFFFF
+MAX_ROUNDS = 64
+# RMAT (a, b, c, d) = (0.57, 0.19, 0.19, 0.05) as cumulative 32-bit thresholds
+RMAT_T = (2448131359, 3264175145, 4080218931)
+
+
+class PhemError(RuntimeError):
+    pass
+
+
+class MstStats(ctypes.Structure):
+    _fields_ = [
+        ("rounds", ctypes.c_uint32),
+        ("mst_count", ctypes.c_uint32),
+        ("weight_fx", ctypes.c_uint64),
+        ("weight_exact", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("sum_m", ctypes.c_uint64),
+        ("sum_c", ctypes.c_uint64),
+        ("m_log", ctypes.c_uint64 * MAX_ROUNDS),
+        ("c_log", ctypes.c_uint64 * MAX_ROUNDS),
+    ]
+
+    def as_dict(self):
+        r = self.rounds
+        return dict(rounds=r, mst_count=self.mst_count, weight_fx=self.weight_fx,
+                    weight_exact=bool(self.weight_exact), sum_m=self.sum_m,
+                    sum_c=self.sum_c, m_log=list(self.m_log[:r]),
+                    c_log=list(self.c_log[:r]))
+
+
+_lib = None
+
+
+def lib():
+    """Load libphem.so (built in-tree by `make -C phem-graph_amd`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PhemError(f"libphem.so not built: {LIB_PATH}")
+    L = ctypes.CDLL(LIB_PATH)
+    c = ctypes
+    vp, u32, u64, i32 = c.c_void_p, c.c_uint32, c.c_uint64, c.c_int
+    sig = {
+        "phem_version": (c.c_char_p, []),
+        "phem_last_error": (c.c_char_p, []),
+        "phem_device_check": (i32, [c.POINTER(c.c_int)]),
+        "phem_mst_workspace_bytes": (i32, [u32, u64, c.POINTER(c.c_uint64)]),
+        "phem_mst_solve": (i32, [u32, u64, vp, vp, vp, i32, vp, u64, vp, vp,
+                                 c.POINTER(MstStats), vp]),
+        "phem_cc_init": (i32, [u32, vp, vp, vp]),
+        "phem_cc_union": (i32, [u32, vp, u64, vp, vp, vp]),
+        "phem_cc_labels": (i32, [u32, vp, vp, vp]),
+        "phem_cc_solve": (i32, [u32, vp, u64, vp, vp, vp, vp]),
+        "phem_cc_merge_forests": (i32, [u32, vp, vp, vp, vp, vp, vp]),
+        "phem_gen_uniform": (i32, [u32, u64, u64, u64, i32, vp, vp, vp, vp]),
+        "phem_gen_rmat": (i32, [u32, u64, u64, u32, u32, u32, i32, vp, vp, vp, vp]),
+        "phem_pack_edges": (i32, [vp, vp, u64, vp, vp]),
+        "phem_prof_enable": (i32, [i32]),
+        "phem_prof_reset": (i32, []),
+        "phem_prof_count": (i32, []),
+        "phem_prof_get": (i32, [i32, c.c_char_p, i32, c.POINTER(c.c_uint64),
+                                c.POINTER(c.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().phem_last_error().decode(errors="replace")
+        raise PhemError(f"{what} returned {rc}: {msg}")
+
+
+def require_device():
+    """Raise unless a gfx950 device is visible to the library."""
+    cnt = ctypes.c_int(0)
+    _check(lib().phem_device_check(ctypes.byref(cnt)), "phem_device_check")
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def as_u32(t):
+    """Reinterpret a 32-bit tensor as uint32 (no copy)."""
+    return t.view(torch.uint32) if t.dtype != torch.uint32 else t
+
+
+def mst_workspace_bytes(n, m):
+    b = ctypes.c_uint64(0)
+    _check(lib().phem_mst_workspace_bytes(n, m, ctypes.byref(b)), "phem_mst_workspace_bytes")
+    return b.value
+
+
+class MstSolver:
+    """Reusable workspace + outputs for phem_mst_solve on one device.
+
+    Inputs are uint32 / int32 device tensors u, v, w (w holds float32 or
+    uint32 bit patterns).  Returns (ids, labels, stats): the forest's edge ids
+    ascending (int32 view of uint32), min-id labels, and the round log."""
+
+    def __init__(self, n, m, device="cuda"):
+        self.n, self.m, self.device = n, m, torch.device(device)
+        self.ws_bytes = mst_workspace_bytes(n, m)
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.device)
+        self.ids = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
+        self.labels = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
+
+    def solve(self, u, v, w, wmode=W_F32, labels=True, stream=None):
+        m = u.numel()
+        if m > self.m:
+            raise PhemError(f"m={m} exceeds solver capacity {self.m}")
+        st = MstStats()
+        _check(lib().phem_mst_solve(self.n, m, _p(u), _p(v), _p(w), wmode, _p(self.ws),
+                                    self.ws_bytes, _p(self.ids),
+                                    _p(self.labels) if labels else None,
+                                    ctypes.byref(st), _stream(stream)), "phem_mst_solve")
+        s = st.as_dict()
+        return self.ids[: s["mst_count"]], (self.labels[: self.n] if labels else None), s
+
+
+def mst_solve(n, u, v, w, wmode=W_F32, labels=True):
+    return MstSolver(n, u.numel(), device=u.device).solve(u, v, w, wmode, labels)
+
+
+def cc_solve(n, edges, forest=True, stream=None):
+    """Union-find CC over packed (u << 32 | v) int64 edges.
+    Returns (labels, parent, forest or None)."""
+    dev = edges.device
+    par = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    lab = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    f = torch.empty(max(n, 1), dtype=torch.int64, device=dev) if forest else None
+    _check(lib().phem_cc_solve(n, _p(edges), edges.numel(), _p(par), _p(f), _p(lab),
+                               _stream(stream)), "phem_cc_solve")
+    return lab[:n], par[:n], (f[:n] if forest else None)
+
+
+def cc_init(n, parent, forest=None, stream=None):
+    _check(lib().phem_cc_init(n, _p(parent), _p(forest), _stream(stream)), "phem_cc_init")
+
+
+def cc_union(n, edges, parent, forest=None, stream=None):
+    _check(lib().phem_cc_union(n, _p(edges), edges.numel(), _p(parent), _p(forest),
+                               _stream(stream)), "phem_cc_union")
+
+
+def cc_labels(n, parent, labels, stream=None):
+    _check(lib().phem_cc_labels(n, _p(parent), _p(labels), _stream(stream)), "phem_cc_labels")
+
+
+def cc_merge_forests(n, forest, extra, parent, forest2, labels, stream=None):
+    _check(lib().phem_cc_merge_forests(n, _p(forest), _p(extra), _p(parent), _p(forest2),
+                                       _p(labels), _stream(stream)), "phem_cc_merge_forests")
+
+
+def gen_uniform(n, m, seed, positive=False, id0=0, device="cuda", out=None):
+    """m uniform edges on n vertices: int32 device tensors (u, v, w-bits)."""
+    if out is None:
+        out = tuple(torch.empty(m, dtype=torch.int32, device=device) for _ in range(3))
+    u, v, w = out
+    _check(lib().phem_gen_uniform(n, m, seed, id0, int(positive), _p(u), _p(v), _p(w),
+                                  _stream()), "phem_gen_uniform")
+    return u, v, w
+
+
+def gen_rmat(scale, m, seed, t=RMAT_T, positive=False, device="cuda", out=None):
+    if out is None:
+        out = tuple(torch.empty(m, dtype=torch.int32, device=device) for _ in range(3))
+    u, v, w = out
+    _check(lib().phem_gen_rmat(scale, m, seed, t[0], t[1], t[2], int(positive), _p(u),
+                               _p(v), _p(w), _stream()), "phem_gen_rmat")
+    return u, v, w
+
+
+def pack_edges(u, v, out=None):
+    e = out if out is not None else torch.empty(u.numel(), dtype=torch.int64, device=u.device)
+    _check(lib().phem_pack_edges(_p(u), _p(v), u.numel(), _p(e), _stream()), "phem_pack_edges")
+    return e
+
+
+def prof_enable(on=True):
+    lib().phem_prof_enable(int(on))
+
+
+def prof_reset():
+    lib().phem_prof_reset()
+
+
+def prof_read():
+    """{kernel name: (launches, total ms)} since the last reset (waits for the
+    recorded events)."""
+    L = lib()
+    out = {}
+    for i in range(L.phem_prof_count()):
+        name = ctypes.create_string_buffer(64)
+        n = ctypes.c_uint64(0)
+        ms = ctypes.c_double(0)
+        _check(L.phem_prof_get(i, name, 64, ctypes.byref(n), ctypes.byref(ms)), "phem_prof_get")
+        out[name.value.decode()] = (n.value, ms.value)
+    return out

@@ -1,0 +1,43 @@
+"""The C ABI library loads and exports every function include/phem.h declares
+(CPU: no kernel is launched)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "phem-graph_amd", "libphem.so")
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(phem_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libphem.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    names = declared_functions()
+    assert len(names) >= 12
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libphem.so not built")
+def test_workspace_query_without_device():
+    lib = ctypes.CDLL(LIB)
+    b = ctypes.c_uint64(0)
+    assert lib.phem_mst_workspace_bytes(ctypes.c_uint32(1 << 24), ctypes.c_uint64(1 << 28),
+                                        ctypes.byref(b)) == 0
+    # two 16-byte edge buffers dominate
+    assert 2 * 16 * (1 << 28) <= b.value < 2 * 16 * (1 << 28) + (1 << 30)
+    assert lib.phem_mst_workspace_bytes(ctypes.c_uint32(10), ctypes.c_uint64(1 << 40),
+                                        ctypes.byref(b)) == -1
+    lib.phem_last_error.restype = ctypes.c_char_p
+    assert b"beyond limits" in lib.phem_last_error()

@@ -1,0 +1,118 @@
+"""GPU parity of the Boruvka MST path (phem_mst_solve through the C ABI)
+against the CPU oracle: bit-exact forest id sets, min-id labels, exact
+weights and the per-round (m_r, c_r) log."""
+import numpy as np
+import pytest
+
+import cases
+import oracle as O
+import synth as S
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def P():
+    import phem_native
+    phem_native.require_device()
+    return phem_native
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def run_mst(P, n, u, v, w, wmode):
+    ids, lab, st = P.mst_solve(n, dev(u), dev(v), dev(w), wmode)
+    torch.cuda.synchronize()
+    return host(ids), host(lab), st
+
+
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_mst_matches_oracle(P, name):
+    n, u, v, w, wmode = cases.make(name)
+    print(f"case {name}: n={n} m={len(u)}", flush=True)
+    ids, lab, st = run_mst(P, n, u, v, w, wmode)
+    k_ids, k_lab = O.kruskal(n, u, v, w, wmode)
+    assert np.array_equal(ids, k_ids)
+    assert np.array_equal(lab, k_lab)
+    b = O.boruvka(n, u, v, w, wmode)
+    assert st["rounds"] == b["rounds"]
+    assert st["m_log"] == b["log_m"].tolist()
+    assert st["c_log"] == b["log_c"].tolist()
+    ws, exact = O.weight_sum(k_ids, w, wmode)
+    assert st["weight_exact"] == exact
+    if exact:
+        assert st["weight_fx"] == ws
+
+
+def test_mst_repeat_is_deterministic(P):
+    n, u, v, w, wmode = cases.make("rmat_small")
+    solver = P.MstSolver(n, len(u))
+    du, dv, dw = dev(u), dev(v), dev(w)
+    first = None
+    for _ in range(3):
+        ids, lab, st = solver.solve(du, dv, dw, wmode)
+        got = (host(ids).copy(), host(lab).copy(), st["weight_fx"])
+        if first is None:
+            first = got
+        assert np.array_equal(got[0], first[0]) and np.array_equal(got[1], first[1])
+        assert got[2] == first[2]
+
+
+def test_device_generators_match_numpy(P):
+    n, m = 1 << 14, 1 << 17
+    du, dv, dw = P.gen_uniform(n, m, 42)
+    u, v, w = S.uniform(n, m, 42)
+    assert np.array_equal(host(du), u) and np.array_equal(host(dv), v)
+    assert np.array_equal(host(dw), w)
+    du, dv, dw = P.gen_uniform(n, 1000, 42, id0=5000, positive=True)
+    u, v, w = S.uniform(n, 1000, 42, id0=5000, positive=True)
+    assert np.array_equal(host(du), u) and np.array_equal(host(dw), w)
+    du, dv, dw = P.gen_rmat(14, m, 7, positive=True)
+    u, v, w = S.rmat(14, m, 7, positive=True)
+    assert np.array_equal(host(du), u) and np.array_equal(host(dv), v)
+    assert np.array_equal(host(dw), w)
+
+
+def test_mst_rejects_bad_arguments(P):
+    u = dev(np.array([0, 1], np.uint32))
+    with pytest.raises(P.PhemError):
+        P.mst_solve(4, u, u, u, wmode=7)
+    solver = P.MstSolver(4, 2)
+    big = dev(np.zeros(5, np.uint32))
+    with pytest.raises(P.PhemError):
+        solver.solve(big, big, big)
+
+
+@pytest.mark.slow
+def test_mst_rmat_full_size_vs_parallel_oracle(P):
+    """RMAT n=2^24, m=2^28 (a,b,c,d = .57,.19,.19,.05), weights in (0,1]: the
+    device forest equals the OpenMP PBBS restatement's, labels equal the
+    device union-find's, |F| = n - c."""
+    scale, m = 24, 1 << 28
+    n = 1 << scale
+    print("generating rmat 2^24 / 2^28", flush=True)
+    du, dv, dw = P.gen_rmat(scale, m, 2, positive=True)
+    ids, lab, st = P.mst_solve(n, du, dv, dw, P.W_F32)
+    torch.cuda.synchronize()
+    d_ids = host(ids).copy()
+    d_lab = host(lab).copy()
+    print(f"device forest {len(d_ids)} edges, rounds {st['rounds']}", flush=True)
+    e = P.pack_edges(du, dv)
+    clab, _, _ = P.cc_solve(n, e, forest=False)
+    assert np.array_equal(host(clab), d_lab)
+    comps = int((d_lab == np.arange(n, dtype=np.uint32)).sum())
+    assert len(d_ids) == n - comps
+    u, v, w = host(du).copy(), host(dv).copy(), host(dw).copy()
+    del e, du, dv, dw
+    p_ids = O.par_mst(n, u, v, w, 1)
+    assert np.array_equal(d_ids, p_ids)
+    ws, exact = O.weight_sum(p_ids, w, 1)
+    assert exact and st["weight_fx"] == ws
