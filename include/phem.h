@@ -101,6 +101,14 @@ int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
                    void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
                    uint32_t *d_labels, phem_mst_stats *stats, void *stream);
 
+/* Same as phem_mst_solve over the reference's array-of-records edge layout:
+ * d_edges holds m records {u, v, w} of three 32-bit words
+ * (struct edge, MST/src/constants.h:15-19).  Same workspace size. */
+int phem_mst_solve_aos(uint32_t n, uint64_t m, const uint32_t *d_edges,
+                       int wmode, void *d_ws, uint64_t ws_bytes,
+                       uint32_t *d_mst_ids, uint32_t *d_labels,
+                       phem_mst_stats *stats, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Union-find connected components / spanning forest
  *

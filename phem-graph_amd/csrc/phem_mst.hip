@@ -56,6 +56,36 @@ __host__ __device__ constexpr int cA(int r) { return C_ROUND + 4 * r + 1; }
 __host__ __device__ constexpr int cL(int r) { return C_ROUND + 4 * r + 2; }
 
 // ---------------------------------------------------------------------------
+// Input edge sources: structure-of-arrays (u[], v[], w[]) or the reference's
+// array of `edge {u, v, w}` records (MST/src/constants.h:15-19).
+
+struct SrcSoA {
+  const uint32_t *u, *v, *w;
+  __device__ __forceinline__ uint32_t U(uint64_t i) const { return u[i]; }
+  __device__ __forceinline__ uint32_t V(uint64_t i) const { return v[i]; }
+  __device__ __forceinline__ uint32_t W(uint64_t i) const { return w[i]; }
+  __device__ __forceinline__ void stream_load(uint64_t i, uint32_t &a,
+                                              uint32_t &b, uint32_t &c) const {
+    a = __builtin_nontemporal_load(&u[i]);
+    b = __builtin_nontemporal_load(&v[i]);
+    c = __builtin_nontemporal_load(&w[i]);
+  }
+};
+
+struct SrcAoS {
+  const uint32_t *e;  // 3 words per edge
+  __device__ __forceinline__ uint32_t U(uint64_t i) const { return e[3 * i]; }
+  __device__ __forceinline__ uint32_t V(uint64_t i) const { return e[3 * i + 1]; }
+  __device__ __forceinline__ uint32_t W(uint64_t i) const { return e[3 * i + 2]; }
+  __device__ __forceinline__ void stream_load(uint64_t i, uint32_t &a,
+                                              uint32_t &b, uint32_t &c) const {
+    a = __builtin_nontemporal_load(&e[3 * i]);
+    b = __builtin_nontemporal_load(&e[3 * i + 1]);
+    c = __builtin_nontemporal_load(&e[3 * i + 2]);
+  }
+};
+
+// ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(BLK) void k_mst_init(uint32_t n, uint32_t *par,
                                                   ull *best, uint64_t words,
@@ -75,10 +105,8 @@ __global__ __launch_bounds__(BLK) void k_mst_init(uint32_t n, uint32_t *par,
 }
 
 // Round-0 sweep over the input edge arrays.
-__global__ __launch_bounds__(BLK) void k_sweep0(const uint32_t *__restrict__ u,
-                                                const uint32_t *__restrict__ v,
-                                                const uint32_t *__restrict__ w,
-                                                uint64_t m, uint32_t n,
+template <class Src>
+__global__ __launch_bounds__(BLK) void k_sweep0(Src src, uint64_t m, uint32_t n,
                                                 int wmode, ull *best) {
   for (uint64_t tile = blockIdx.x; tile * TILE < m; tile += gridDim.x) {
     const uint64_t base = tile * TILE + threadIdx.x;
@@ -87,9 +115,7 @@ __global__ __launch_bounds__(BLK) void k_sweep0(const uint32_t *__restrict__ u,
     for (int j = 0; j < EPT; j++) {
       uint64_t i = base + (uint64_t)j * BLK;
       if (i < m) {
-        a[j] = __builtin_nontemporal_load(&u[i]);
-        b[j] = __builtin_nontemporal_load(&v[i]);
-        ww[j] = __builtin_nontemporal_load(&w[i]);
+        src.stream_load(i, a[j], b[j], ww[j]);
       } else {
         a[j] = b[j] = U32_NONE;
         ww[j] = 0;
@@ -108,10 +134,11 @@ __global__ __launch_bounds__(BLK) void k_sweep0(const uint32_t *__restrict__ u,
 }
 
 // Hook: components of `list` (or 0..n_ident-1 when list is NULL).
+template <class Src>
 __global__ __launch_bounds__(BLK) void k_hook(
     const uint32_t *__restrict__ list, const ull *list_count, uint32_t n_ident,
-    ull *best, const uint32_t *__restrict__ eu, const uint32_t *__restrict__ ev,
-    uint32_t *par, uint32_t *bitmap, uint32_t *act, ull *act_count) {
+    ull *best, Src src, uint32_t *par, uint32_t *bitmap, uint32_t *act,
+    ull *act_count) {
   const uint64_t cnt = list ? *list_count : (uint64_t)n_ident;
   for (uint64_t base = (uint64_t)blockIdx.x * (BLK * IPT); base < cnt;
        base += (uint64_t)gridDim.x * (BLK * IPT)) {
@@ -131,7 +158,7 @@ __global__ __launch_bounds__(BLK) void k_hook(
           best[c[j]] = KEY_NONE;  // reset for the next round's sweep
           uint32_t e = (uint32_t)key;
           atomicOr(&bitmap[e >> 5], 1u << (e & 31));
-          uf_union(par, eu[e], ev[e]);
+          uf_union(par, src.U(e), src.V(e));
         }
       }
     }
@@ -175,11 +202,10 @@ __global__ __launch_bounds__(BLK) void k_compress(const uint32_t *__restrict__ a
 // Relabel + drop intra-component edges + compact + next-round sweep.
 // R0: the input is the original (u, v, w) arrays, keys are formed from w and
 // the index, and invalid edges are dropped.
-template <bool R0>
+template <bool R0, class Src>
 __global__ __launch_bounds__(BLK) void k_compact(
-    const uint32_t *__restrict__ ia, const uint32_t *__restrict__ ib,
-    const ull *__restrict__ ik, const uint32_t *__restrict__ iw,
-    const ull *in_count, uint64_t m0, uint32_t n, int wmode,
+    Src src, const uint32_t *__restrict__ ia, const uint32_t *__restrict__ ib,
+    const ull *__restrict__ ik, const ull *in_count, uint64_t m0, uint32_t n, int wmode,
     const uint32_t *__restrict__ par, uint32_t *__restrict__ oa,
     uint32_t *__restrict__ ob, ull *__restrict__ ok, ull *out_count,
     ull *valid_count, ull *best) {
@@ -207,13 +233,14 @@ __global__ __launch_bounds__(BLK) void k_compact(
       a[j] = b[j] = 0;
       key[j] = KEY_NONE;
       if (i < mr) {
-        a[j] = __builtin_nontemporal_load(&ia[i]);
-        b[j] = __builtin_nontemporal_load(&ib[i]);
         if (R0) {
-          uint32_t wv = __builtin_nontemporal_load(&iw[i]);
+          uint32_t wv;
+          src.stream_load(i, a[j], b[j], wv);
           key[j] = ((ull)wkey(wv, wmode) << 32) | i;
           val[j] = a[j] < n && b[j] < n && a[j] != b[j];
         } else {
+          a[j] = __builtin_nontemporal_load(&ia[i]);
+          b[j] = __builtin_nontemporal_load(&ib[i]);
           key[j] = __builtin_nontemporal_load(&ik[i]);
           val[j] = true;
         }
@@ -343,12 +370,12 @@ __device__ __forceinline__ bool weight_fx(uint32_t bits, int wmode, ull *out) {
 }
 
 // Bitmap -> ascending ids; exact weight sum of the forest.
+template <class Src>
 __global__ __launch_bounds__(BLK) void k_bm_emit(const uint32_t *__restrict__ bm,
                                                  uint64_t words,
                                                  const ull *__restrict__ blk,
                                                  uint32_t *__restrict__ ids,
-                                                 const uint32_t *__restrict__ w,
-                                                 int wmode, ull *wsum,
+                                                 Src src, int wmode, ull *wsum,
                                                  ull *inexact) {
   const uint64_t w0 = (uint64_t)blockIdx.x * WPB + (uint64_t)threadIdx.x * WPT;
   uint32_t word[WPT];
@@ -370,7 +397,7 @@ __global__ __launch_bounds__(BLK) void k_bm_emit(const uint32_t *__restrict__ bm
       uint32_t e = (uint32_t)((w0 + k) * 32 + bit);
       ids[pos++] = e;
       ull fx;
-      if (weight_fx(w[e], wmode, &fx)) s += fx; else bad++;
+      if (weight_fx(src.W(e), wmode, &fx)) s += fx; else bad++;
     }
   }
   ull ts, tb;
@@ -433,24 +460,23 @@ extern "C" int phem_mst_workspace_bytes(uint32_t n, uint64_t m,
   return PHEM_OK;
 }
 
-extern "C" int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
-                              const uint32_t *d_v, const uint32_t *d_w,
-                              int wmode, void *d_ws, uint64_t ws_bytes,
-                              uint32_t *d_mst_ids, uint32_t *d_labels,
-                              phem_mst_stats *stats, void *stream) {
+namespace phem {
+
+template <class Src>
+static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
+                          void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
+                          uint32_t *d_labels, phem_mst_stats *stats,
+                          hipStream_t s) {
   if (m > PHEM_MAX_EDGES || n > PHEM_MAX_VERTICES)
     return set_err(PHEM_EINVAL, "n=%u m=%llu beyond limits", n,
                    (unsigned long long)m);
   if (wmode != PHEM_W_U32 && wmode != PHEM_W_F32)
     return set_err(PHEM_EINVAL, "unknown weight mode %d", wmode);
-  if (m && (!d_u || !d_v || !d_w))
-    return set_err(PHEM_EINVAL, "edge arrays are NULL");
   if (n && !d_mst_ids) return set_err(PHEM_EINVAL, "d_mst_ids is NULL");
   const MstLayout L = mst_layout(n, m);
   if (!d_ws || ws_bytes < L.total)
     return set_err(PHEM_ENOMEM, "workspace %llu B < required %llu B",
                    (unsigned long long)ws_bytes, (unsigned long long)L.total);
-  hipStream_t s = (hipStream_t)stream;
   char *ws = (char *)d_ws;
   uint32_t *par = (uint32_t *)(ws + L.par);
   ull *best = (ull *)(ws + L.best);
@@ -477,47 +503,45 @@ extern "C" int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
 
   int rounds = 0;
   if (m > 0 && n > 1) {
-    // round 0
+    // round 0: sweep, hook and compress over all vertices, compact the input
     {
       PHEM_PROF("k_sweep0", s);
-      hipLaunchKernelGGL(k_sweep0, dim3(grid_for(m, TILE)), dim3(BLK), 0, s, d_u,
-                         d_v, d_w, m, n, wmode, best);
+      hipLaunchKernelGGL(k_sweep0<Src>, dim3(grid_for(m, TILE)), dim3(BLK), 0,
+                         s, src, m, n, wmode, best);
     }
     PHEM_LAUNCHED("k_sweep0");
     {
       PHEM_PROF("k_hook", s);
-      hipLaunchKernelGGL(k_hook, dim3(grid_for(n, BLK * IPT)), dim3(BLK), 0, s,
-                         (const uint32_t *)nullptr, (const ull *)nullptr, n, best,
-                         d_u, d_v, par, bitmap, listA, ctr + cA(0));
+      hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(n, BLK * IPT)), dim3(BLK),
+                         0, s, (const uint32_t *)nullptr, (const ull *)nullptr,
+                         n, best, src, par, bitmap, listA, ctr + cA(0));
     }
     PHEM_LAUNCHED("k_hook");
     {
       PHEM_PROF("k_compress", s);
-      hipLaunchKernelGGL(k_compress, dim3(grid_for(n, BLK * IPT)), dim3(BLK), 0, s,
-                         listA, ctr + cA(0), par, listL, ctr + cL(1));
+      hipLaunchKernelGGL(k_compress, dim3(grid_for(n, BLK * IPT)), dim3(BLK), 0,
+                         s, listA, ctr + cA(0), par, listL, ctr + cL(1));
     }
     PHEM_LAUNCHED("k_compress");
     {
       PHEM_PROF("k_compact<R0>", s);
-      hipLaunchKernelGGL(k_compact<true>, dim3(grid_for(m, TILE)), dim3(BLK), 0,
-                         s, d_u, d_v, (const ull *)nullptr, d_w,
+      hipLaunchKernelGGL((k_compact<true, Src>), dim3(grid_for(m, TILE)),
+                         dim3(BLK), 0, s, src, (const uint32_t *)nullptr,
+                         (const uint32_t *)nullptr, (const ull *)nullptr,
                          (const ull *)nullptr, m, n, wmode, par, Ea[0], Eb[0],
                          Ek[0], ctr + cE(1), ctr + C_VALID, best);
     }
     PHEM_LAUNCHED("k_compact<R0>");
-    ull h[8];
-    PHEM_HIP(hipMemcpyAsync(h, ctr + cE(1), 4 * sizeof(ull), hipMemcpyDeviceToHost, s));
-    ull h0[8];
-    PHEM_HIP(hipMemcpyAsync(h0, ctr, 8 * sizeof(ull), hipMemcpyDeviceToHost, s));
-    ull hA0;
-    PHEM_HIP(hipMemcpyAsync(&hA0, ctr + cA(0), sizeof(ull), hipMemcpyDeviceToHost, s));
+    ull h[16];
+    PHEM_HIP(hipMemcpyAsync(h, ctr, 16 * sizeof(ull), hipMemcpyDeviceToHost, s));
     PHEM_HIP(hipStreamSynchronize(s));
-    uint64_t m_r = h[0], list_ub = h[2];
+    uint64_t m_r = h[cE(1)], list_ub = h[cL(1)];
     if (stats) {
-      stats->m_log[0] = h0[C_VALID];
-      stats->c_log[0] = hA0;
+      stats->m_log[0] = h[C_VALID];
+      stats->c_log[0] = h[cA(0)];
     }
-    rounds = 1;
+    // a round counts when it starts with at least one alive edge
+    rounds = h[C_VALID] ? 1 : 0;
     int cur = 0;
     while (m_r > 0) {
       if (rounds >= PHEM_MAX_ROUNDS)
@@ -525,21 +549,22 @@ extern "C" int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
       const int r = rounds;
       {
         PHEM_PROF("k_hook", s);
-        hipLaunchKernelGGL(k_hook, dim3(grid_for(list_ub, BLK * IPT)), dim3(BLK), 0, s,
-                           listL, ctr + cL(r), 0u, best, d_u, d_v, par, bitmap,
-                           listA, ctr + cA(r));
+        hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(list_ub, BLK * IPT)),
+                           dim3(BLK), 0, s, listL, ctr + cL(r), 0u, best, src,
+                           par, bitmap, listA, ctr + cA(r));
       }
       PHEM_LAUNCHED("k_hook");
       {
         PHEM_PROF("k_compress", s);
-        hipLaunchKernelGGL(k_compress, dim3(grid_for(list_ub, BLK * IPT)), dim3(BLK), 0,
-                           s, listA, ctr + cA(r), par, listL, ctr + cL(r + 1));
+        hipLaunchKernelGGL(k_compress, dim3(grid_for(list_ub, BLK * IPT)),
+                           dim3(BLK), 0, s, listA, ctr + cA(r), par, listL,
+                           ctr + cL(r + 1));
       }
       PHEM_LAUNCHED("k_compress");
       {
         PHEM_PROF("k_compact", s);
-        hipLaunchKernelGGL(k_compact<false>, dim3(grid_for(m_r, TILE)), dim3(BLK),
-                           0, s, Ea[cur], Eb[cur], Ek[cur], (const uint32_t *)nullptr,
+        hipLaunchKernelGGL((k_compact<false, Src>), dim3(grid_for(m_r, TILE)),
+                           dim3(BLK), 0, s, src, Ea[cur], Eb[cur], Ek[cur],
                            ctr + cE(r), 0ull, n, wmode, par, Ea[cur ^ 1],
                            Eb[cur ^ 1], Ek[cur ^ 1], ctr + cE(r + 1),
                            (ull *)nullptr, best);
@@ -570,8 +595,8 @@ extern "C" int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
   }
   {
     PHEM_PROF("k_bm_count", s);
-    hipLaunchKernelGGL(k_bm_count, dim3((unsigned)L.nb), dim3(BLK), 0, s, bitmap,
-                       L.words, blk);
+    hipLaunchKernelGGL(k_bm_count, dim3((unsigned)L.nb), dim3(BLK), 0, s,
+                       bitmap, L.words, blk);
   }
   PHEM_LAUNCHED("k_bm_count");
   {
@@ -582,9 +607,9 @@ extern "C" int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
   PHEM_LAUNCHED("k_scan_blocks");
   {
     PHEM_PROF("k_bm_emit", s);
-    hipLaunchKernelGGL(k_bm_emit, dim3((unsigned)L.nb), dim3(BLK), 0, s, bitmap,
-                       L.words, blk, d_mst_ids, d_w, wmode, ctr + C_WSUM,
-                       ctr + C_INEXACT);
+    hipLaunchKernelGGL(k_bm_emit<Src>, dim3((unsigned)L.nb), dim3(BLK), 0, s,
+                       bitmap, L.words, blk, d_mst_ids, src, wmode,
+                       ctr + C_WSUM, ctr + C_INEXACT);
   }
   PHEM_LAUNCHED("k_bm_emit");
   ull tail[4];
@@ -601,4 +626,27 @@ extern "C" int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
     }
   }
   return PHEM_OK;
+}
+
+}  // namespace phem
+
+extern "C" int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
+                              const uint32_t *d_v, const uint32_t *d_w,
+                              int wmode, void *d_ws, uint64_t ws_bytes,
+                              uint32_t *d_mst_ids, uint32_t *d_labels,
+                              phem_mst_stats *stats, void *stream) {
+  if (m && (!d_u || !d_v || !d_w))
+    return set_err(PHEM_EINVAL, "edge arrays are NULL");
+  return mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w}, wmode, d_ws, ws_bytes,
+                        d_mst_ids, d_labels, stats, (hipStream_t)stream);
+}
+
+extern "C" int phem_mst_solve_aos(uint32_t n, uint64_t m,
+                                  const uint32_t *d_edges, int wmode,
+                                  void *d_ws, uint64_t ws_bytes,
+                                  uint32_t *d_mst_ids, uint32_t *d_labels,
+                                  phem_mst_stats *stats, void *stream) {
+  if (m && !d_edges) return set_err(PHEM_EINVAL, "edge array is NULL");
+  return mst_solve_impl(n, m, SrcAoS{d_edges}, wmode, d_ws, ws_bytes, d_mst_ids,
+                        d_labels, stats, (hipStream_t)stream);
 }

@@ -65,6 +65,8 @@ def lib():
         "phem_mst_workspace_bytes": (i32, [u32, u64, c.POINTER(c.c_uint64)]),
         "phem_mst_solve": (i32, [u32, u64, vp, vp, vp, i32, vp, u64, vp, vp,
                                  c.POINTER(MstStats), vp]),
+        "phem_mst_solve_aos": (i32, [u32, u64, vp, i32, vp, u64, vp, vp,
+                                     c.POINTER(MstStats), vp]),
         "phem_cc_init": (i32, [u32, vp, vp, vp]),
         "phem_cc_union": (i32, [u32, vp, u64, vp, vp, vp]),
         "phem_cc_labels": (i32, [u32, vp, vp, vp]),
@@ -142,6 +144,20 @@ class MstSolver:
                                     self.ws_bytes, _p(self.ids),
                                     _p(self.labels) if labels else None,
                                     ctypes.byref(st), _stream(stream)), "phem_mst_solve")
+        s = st.as_dict()
+        return self.ids[: s["mst_count"]], (self.labels[: self.n] if labels else None), s
+
+    def solve_aos(self, edges, wmode=W_F32, labels=True, stream=None):
+        """edges: device tensor of m x 3 32-bit words {u, v, w}."""
+        m = edges.numel() // 3
+        if m > self.m:
+            raise PhemError(f"m={m} exceeds solver capacity {self.m}")
+        st = MstStats()
+        _check(lib().phem_mst_solve_aos(self.n, m, _p(edges), wmode, _p(self.ws),
+                                        self.ws_bytes, _p(self.ids),
+                                        _p(self.labels) if labels else None,
+                                        ctypes.byref(st), _stream(stream)),
+               "phem_mst_solve_aos")
         s = st.as_dict()
         return self.ids[: s["mst_count"]], (self.labels[: self.n] if labels else None), s
 

@@ -116,3 +116,15 @@ def test_mst_rmat_full_size_vs_parallel_oracle(P):
     assert np.array_equal(d_ids, p_ids)
     ws, exact = O.weight_sum(p_ids, w, 1)
     assert exact and st["weight_fx"] == ws
+
+
+@pytest.mark.parametrize("name", ["rmat_small", "grid_small", "out_of_range", "u32_weights"])
+def test_mst_aos_layout_matches_soa(P, name):
+    n, u, v, w, wmode = cases.make(name)
+    aos = np.stack([u, v, w], axis=1).reshape(-1).copy()
+    solver = P.MstSolver(n, len(u))
+    ids, lab, st = solver.solve_aos(dev(aos), wmode)
+    torch.cuda.synchronize()
+    k_ids, k_lab = O.kruskal(n, u, v, w, wmode)
+    assert np.array_equal(host(ids), k_ids)
+    assert np.array_equal(host(lab), k_lab)
