@@ -26,18 +26,35 @@ WORKLOADS = {
 }
 
 
-def mst_kernel_bytes(m_log, n):
-    """Algorithmic bytes per solve of each MST kernel from the round log.
-    sweep0 reads (u, v, w) once; compaction round r reads 12 B (r = 0) or
-    16 B (r > 0) per alive edge and writes 16 B per survivor; labels read the
-    parent array and write labels."""
-    m0 = m_log[0] if m_log else 0
-    nxt = list(m_log[1:]) + [0]
-    b = {"k_sweep0": 12 * m0,
-         "k_compact<R0>": 12 * m0 + 16 * nxt[0],
-         "k_compact": sum(16 * m_log[r] + 16 * nxt[r] for r in range(1, len(m_log))),
-         "k_labels": 8 * n}
+def mst_kernel_bytes(st, m, n):
+    """Algorithmic bytes per solve of each MST kernel, from the solve's log.
+    Input passes read 12 B per input edge (u, v, w; the id is the index) and
+    write 16 B (u', v', key) per edge they keep; a compaction round reads 16 B
+    per alive edge and writes 16 B per survivor; flatten and labels read and
+    write the 4 B parent / label of every vertex."""
+    ml, lr = st["m_log"], st["light_rounds"]
+    R = len(ml)
+    # survivors written by round r: the next round's alive count within a phase
+    out = [ml[r + 1] if (r + 1 < R and r + 1 != lr) else 0 for r in range(R)]
+    b = {"k_compact": sum(16 * ml[r] + 16 * out[r] for r in range(R)),
+         "k_labels": 8 * n, "k_flatten": 8 * n, "k_giant_bits": 4 * n + n // 8}
+    if st["tau"] == 0xFFFFFFFFFFFFFFFF:  # canonical: round 0 is sweep + compact<R0>
+        b["k_sweep0"] = 12 * m
+        b["k_compact<R0>"] = 12 * m + 16 * (ml[1] if R > 1 else 0)
+        b["k_compact"] -= 16 * ml[0] + 16 * out[0]
+    else:
+        b["k_light"] = 12 * m + 16 * st["light_edges"]
+        b["k_heavy"] = 12 * m + 16 * st["heavy_edges"]
     return b
+
+
+def b_actual(st, m, n):
+    """Algorithmic bytes of the solve as run: every edge pass and every
+    component-side pass (16 B per active component per round: best key and
+    parent), plus 12 B per vertex for parent init, labels and the forest
+    output."""
+    kb = mst_kernel_bytes(st, m, n)
+    return sum(kb.values()) - kb["k_labels"] + 16 * sum(st["c_log"]) + 12 * n
 
 
 def b_mst(m, n, m_log, c_log):
@@ -71,6 +88,9 @@ def main():
     ap.add_argument("--cpu-sample-log2", type=int, default=25,
                     help="edges of the CPU baseline sample (prefix of the stream)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device copy of this size before the timed region "
+                         "(a known-bytes reference for hardware counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -100,6 +120,13 @@ def main():
         P._check(P.lib().phem_cc_solve(n, P._p(de), m, P._p(cc_par), P._p(cc_forest),
                                        P._p(cc_lab), P._stream()), "phem_cc_solve")
         return st
+
+    if args.calib_copy_mib:
+        src = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for i in range(args.warmup):
         st = step()
@@ -133,8 +160,10 @@ def main():
     value = m / (ms_per_step * 1e-3)
 
     # roofline of the dominant kernel (largest share of the timed region)
-    kb = mst_kernel_bytes(st["m_log"], n)
-    kb["k_cc_union"] = 8 * m
+    kb = mst_kernel_bytes(st, m, n)
+    m1 = min(m, int(float(os.environ.get("PHEM_CC_SAMPLE_FACTOR", "2.0")) * n))
+    kb["k_cc_union"] = 8 * m1
+    kb["k_cc_union_filtered"] = 8 * (m - m1)
     kb["k_cc_labels"] = 8 * n
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     dname, (dlaunch, dms) = dom
@@ -160,7 +189,11 @@ def main():
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(per_launch_ms, 4)}
 
-    bm, bc = b_mst(m, n, st["m_log"], st["c_log"]), b_cc(m, n)
+    # canonical round log (untimed) for the formula 32 S_m - 20 m + 16 S_c + 12 n
+    can = solver.solve(du, dv, dw, P.W_F32, light_factor=0.0)[2]
+    torch.cuda.synchronize()
+    bm, bc = b_mst(m, n, can["m_log"], can["c_log"]), b_cc(m, n)
+    ba = b_actual(st, m, n)
     out = {
         "metric": "MST/CC edges/s at RMAT n=2^24 m=2^28; batched vs plain; p50/p99 latency",
         "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
@@ -171,9 +204,15 @@ def main():
                    "rmat_abcd": [0.57, 0.19, 0.19, 0.05], "weights": "k*2^-24 in (0,1]",
                    "step": "MST solve + union-find CC solve"},
         "mst": {"ms": round(mst_ms, 4), "edges_per_s": m / (mst_ms * 1e-3),
-                "rounds": st["rounds"], "forest": st["mst_count"],
-                "B_bytes": bm, "roofline_frac": round(bm / (mst_ms * 1e-3) / 8e12, 4),
-                "m_log": st["m_log"], "c_log": st["c_log"]},
+                "rounds": st["rounds"], "light_rounds": st["light_rounds"],
+                "forest": st["mst_count"], "light_edges": st["light_edges"],
+                "heavy_edges": st["heavy_edges"],
+                "B_canonical_bytes": bm,
+                "roofline_frac_canonical_bytes": round(bm / (mst_ms * 1e-3) / 8e12, 4),
+                "B_actual_bytes": ba,
+                "roofline_frac_actual_bytes": round(ba / (mst_ms * 1e-3) / 8e12, 4),
+                "m_log": st["m_log"], "c_log": st["c_log"],
+                "canonical_m_log": can["m_log"], "canonical_c_log": can["c_log"]},
         "cc": {"ms": round(cc_ms, 4), "edges_per_s": m / (cc_ms * 1e-3), "B_bytes": bc,
                "roofline_frac": round(bc / (cc_ms * 1e-3) / 8e12, 4)},
         "kernels_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in

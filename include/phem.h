@@ -77,16 +77,34 @@ int phem_prof_get(int i, char *name, int name_cap, uint64_t *launches,
  * ------------------------------------------------------------------------- */
 
 typedef struct phem_mst_stats {
-  uint32_t rounds;        /* Boruvka rounds executed */
+  uint32_t rounds;        /* Boruvka rounds executed (all phases) */
   uint32_t mst_count;     /* edges in the forest */
   uint64_t weight_fx;     /* exact total weight: U32 sum of w; F32 sum of w*2^24 */
   int32_t weight_exact;   /* F32: 0 if a weight is negative or off the 2^-24 grid */
-  int32_t reserved;
-  uint64_t sum_m;         /* S_m = sum over rounds of alive edges */
-  uint64_t sum_c;         /* S_c = sum over rounds of active components */
+  uint32_t light_rounds;  /* rounds of the light phase (all rounds: no split) */
+  uint64_t sum_m;         /* sum over rounds of alive edges */
+  uint64_t sum_c;         /* sum over rounds of active components */
+  uint64_t valid_edges;   /* input edges that are part of the graph */
+  uint64_t tau;           /* light/heavy key threshold (all ones: no split) */
+  uint64_t light_edges;   /* edges of the light phase */
+  uint64_t heavy_edges;   /* heavy edges left after the filter */
   uint64_t m_log[PHEM_MAX_ROUNDS]; /* alive edges at the start of round r */
   uint64_t c_log[PHEM_MAX_ROUNDS]; /* components with >= 1 alive edge in round r */
 } phem_mst_stats;
+
+/* Solver options (NULL = defaults).
+ * light_factor: the solve first runs Boruvka rounds over the ~light_factor*n
+ *   lightest edges (threshold from a sample of the keys), then drops every
+ *   heavier edge whose endpoints are already connected and runs rounds over
+ *   the rest (filter-Kruskal, as the CPU path does, MST/src/utils.cu:794-841).
+ *   <= 0 runs the canonical Boruvka round loop over all edges
+ *   (MST/src/kernels.cu:507-689); its per-round log is then the reference's.
+ *   Default 2.0, or the environment variable PHEM_MST_LIGHT_FACTOR. */
+typedef struct phem_mst_opts {
+  double light_factor;
+  uint32_t flags; /* reserved, 0 */
+  uint32_t reserved;
+} phem_mst_opts;
 
 /* Bytes of device workspace phem_mst_solve needs for (n, m). */
 int phem_mst_workspace_bytes(uint32_t n, uint64_t m, uint64_t *bytes);
@@ -99,7 +117,8 @@ int phem_mst_workspace_bytes(uint32_t n, uint64_t m, uint64_t *bytes);
 int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
                    const uint32_t *d_v, const uint32_t *d_w, int wmode,
                    void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
-                   uint32_t *d_labels, phem_mst_stats *stats, void *stream);
+                   uint32_t *d_labels, const phem_mst_opts *opts,
+                   phem_mst_stats *stats, void *stream);
 
 /* Same as phem_mst_solve over the reference's array-of-records edge layout:
  * d_edges holds m records {u, v, w} of three 32-bit words
@@ -107,7 +126,8 @@ int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
 int phem_mst_solve_aos(uint32_t n, uint64_t m, const uint32_t *d_edges,
                        int wmode, void *d_ws, uint64_t ws_bytes,
                        uint32_t *d_mst_ids, uint32_t *d_labels,
-                       phem_mst_stats *stats, void *stream);
+                       const phem_mst_opts *opts, phem_mst_stats *stats,
+                       void *stream);
 
 /* ---------------------------------------------------------------------------
  * Union-find connected components / spanning forest

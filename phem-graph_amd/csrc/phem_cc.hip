@@ -9,8 +9,11 @@
 // halving instead of the reference's two-pass compression
 // (custom_compress, CC/ExternalSpanningTree.cu:27-43): both only ever move a
 // pointer to an ancestor.
+#include <cstdlib>
+
 #include "phem_dev.h"
 #include "phem_internal.h"
+#include "phem_split.h"
 
 namespace phem {
 
@@ -36,9 +39,13 @@ __device__ __forceinline__ void cc_edge(ull e, uint32_t n, uint32_t *par,
   }
 }
 
+// FILTER: edges whose endpoints are both in the giant component of the
+// sampling phase (bitmap `giant`) are skipped without touching par.
+template <bool FILTER>
 __global__ __launch_bounds__(CBLK) void k_cc_union(const ull *__restrict__ edges,
                                                    uint64_t m, uint32_t n,
-                                                   uint32_t *par, ull *forest) {
+                                                   uint32_t *par, ull *forest,
+                                                   const uint32_t *__restrict__ giant) {
   for (uint64_t tile = blockIdx.x; tile * CTILE < m; tile += gridDim.x) {
     const uint64_t base = tile * CTILE + threadIdx.x;
     ull e[CEPT];
@@ -47,8 +54,30 @@ __global__ __launch_bounds__(CBLK) void k_cc_union(const ull *__restrict__ edges
       uint64_t i = base + (uint64_t)j * CBLK;
       e[j] = i < m ? __builtin_nontemporal_load(&edges[i]) : KEY_NONE;
     }
+    // Fast path: the parents of both endpoints of all edges are loaded
+    // together (independent loads in flight); an edge whose endpoints share a
+    // parent is inside one tree already.  A parent read here may be older than
+    // the current one, but every value it ever held is an ancestor, so equal
+    // parents still prove one tree.
+    uint32_t pa[CEPT], pb[CEPT];
+    bool ok[CEPT];
 #pragma unroll
-    for (int j = 0; j < CEPT; j++) cc_edge(e[j], n, par, forest);
+    for (int j = 0; j < CEPT; j++) {
+      const uint32_t a = (uint32_t)(e[j] >> 32), b = (uint32_t)e[j];
+      ok[j] = a < b && b < n;
+      if (FILTER && ok[j])
+        ok[j] = !(((giant[a >> 5] >> (a & 31)) & 1u) &&
+                  ((giant[b >> 5] >> (b & 31)) & 1u));
+    }
+#pragma unroll
+    for (int j = 0; j < CEPT; j++) {
+      const uint32_t a = (uint32_t)(e[j] >> 32), b = (uint32_t)e[j];
+      pa[j] = ok[j] ? par[a] : 0u;
+      pb[j] = ok[j] ? par[b] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < CEPT; j++)
+      if (pa[j] != pb[j]) cc_edge(e[j], n, par, forest);
   }
 }
 
@@ -101,9 +130,9 @@ extern "C" int phem_cc_union(uint32_t n, const uint64_t *d_edges, uint64_t m,
   if (!m || !n) return PHEM_OK;
   {
     PHEM_PROF("k_cc_union", (hipStream_t)stream);
-    hipLaunchKernelGGL(k_cc_union, dim3(grid_for(m, CTILE)), dim3(CBLK), 0,
-                       (hipStream_t)stream, (const ull *)d_edges, m, n, d_parent,
-                       (ull *)d_forest);
+    hipLaunchKernelGGL(k_cc_union<false>, dim3(grid_for(m, CTILE)), dim3(CBLK),
+                       0, (hipStream_t)stream, (const ull *)d_edges, m, n,
+                       d_parent, (ull *)d_forest, (const uint32_t *)nullptr);
   }
   PHEM_LAUNCHED("k_cc_union");
   return PHEM_OK;
@@ -123,14 +152,65 @@ extern "C" int phem_cc_labels(uint32_t n, const uint32_t *d_parent,
   return PHEM_OK;
 }
 
+namespace phem {
+double cc_default_sample_factor() {
+  static double f = [] {
+    const char *e = getenv("PHEM_CC_SAMPLE_FACTOR");
+    return e ? atof(e) : 2.0;
+  }();
+  return f;
+}
+}  // namespace phem
+
+// Two phases when m is well above n (sampling idea of Afforest, Sutton et al.,
+// IPDPS 2018): union the first ~2n records, flatten the parent forest, mark the
+// giant component, then union the remaining records, skipping those with both
+// endpoints in the giant component.  d_labels doubles as scratch for the
+// bitmap and the giant root before receiving the labels.
 extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
                              uint32_t *d_parent, uint64_t *d_forest,
                              uint32_t *d_labels, void *stream) {
   int rc = phem_cc_init(n, d_parent, d_forest, stream);
   if (rc) return rc;
-  rc = phem_cc_union(n, d_edges, m, d_parent, d_forest, stream);
+  const double f = cc_default_sample_factor();
+  const uint64_t m1 = f > 0 ? (uint64_t)(f * (double)n) : m;
+  if (!d_labels || m1 >= m || n < 64) {
+    rc = phem_cc_union(n, d_edges, m, d_parent, d_forest, stream);
+    if (rc) return rc;
+    return d_labels ? phem_cc_labels(n, d_parent, d_labels, stream) : PHEM_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t *giant_root = d_labels;  // 1 word
+  uint32_t *bits = d_labels + 2;    // ceil(n / 32) words <= n - 2
+  rc = phem_cc_union(n, d_edges, m1, d_parent, d_forest, stream);
   if (rc) return rc;
-  return d_labels ? phem_cc_labels(n, d_parent, d_labels, stream) : PHEM_OK;
+  {
+    PHEM_PROF("k_flatten", s);
+    hipLaunchKernelGGL(k_flatten, dim3(grid_for(n, SPLIT_BLK)), dim3(SPLIT_BLK),
+                       0, s, n, d_parent);
+  }
+  PHEM_LAUNCHED("k_flatten");
+  {
+    PHEM_PROF("k_pick_giant", s);
+    hipLaunchKernelGGL(k_pick_giant, dim3(1), dim3(SBLK), 0, s, n,
+                       (const uint32_t *)d_parent, giant_root);
+  }
+  PHEM_LAUNCHED("k_pick_giant");
+  {
+    PHEM_PROF("k_giant_bits", s);
+    hipLaunchKernelGGL(k_giant_bits, dim3(grid_for(n, SPLIT_BLK)),
+                       dim3(SPLIT_BLK), 0, s, n, (const uint32_t *)d_parent,
+                       (const uint32_t *)giant_root, bits);
+  }
+  PHEM_LAUNCHED("k_giant_bits");
+  {
+    PHEM_PROF("k_cc_union_filtered", s);
+    hipLaunchKernelGGL(k_cc_union<true>, dim3(grid_for(m - m1, CTILE)),
+                       dim3(CBLK), 0, s, (const ull *)d_edges + m1, m - m1, n,
+                       d_parent, (ull *)d_forest, (const uint32_t *)bits);
+  }
+  PHEM_LAUNCHED("k_cc_union_filtered");
+  return phem_cc_labels(n, d_parent, d_labels, stream);
 }
 
 extern "C" int phem_cc_merge_forests(uint32_t n, const uint64_t *d_forest,

@@ -2,6 +2,7 @@
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -20,6 +21,26 @@ int set_err(int code, const char *fmt, ...) {
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
   return code;
+}
+
+unsigned resident_blocks() {
+  static std::atomic<unsigned> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  unsigned v = cache[dev].load(std::memory_order_relaxed);
+  if (v) return v;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+          hipSuccess || cus <= 0)
+    cus = 256;
+  unsigned per_cu = 8;
+  if (const char *e = getenv("PHEM_BLOCKS_PER_CU")) {
+    const int x = atoi(e);
+    if (x > 0 && x <= 64) per_cu = (unsigned)x;
+  }
+  v = (unsigned)cus * per_cu;
+  cache[dev].store(v, std::memory_order_relaxed);
+  return v;
 }
 
 // ---- per-kernel event profiler --------------------------------------------

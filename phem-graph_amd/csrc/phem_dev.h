@@ -43,13 +43,67 @@ __device__ __forceinline__ uint64_t wave_append(bool pred, ull *counter) {
   return base + lanes_below(mask);
 }
 
-// Block-aggregated append of IPT items per thread: one global atomic per
-// block call.  Every thread of the block must call it (uniform control flow).
-// slot[j] receives the slot of item j where pred[j] holds.
+// ---- segmented append buffers ---------------------------------------------
+//
+// A buffer of KSEG segments: segment k holds cnt[k * CSTR] elements starting
+// at element k * cap.  A producer appends the output of its tile v to segment
+// v mod KSEG with one atomic on that segment's counter.  The counters sit on
+// separate 128-byte lines: atomics on one address serialize (about 12 ns each
+// on MI355X), atomics spread over 64 lines do not.
+
+constexpr int KSEG = 64;
+constexpr int CSTR = 16;  // ull per counter slot (128 B)
+
+// Per-block view of a segmented input cut into tiles of T elements.
+struct SegView {
+  uint32_t tpref[KSEG + 1];  // tiles before segment k
+  uint32_t len[KSEG];
+};
+
+// All threads call; the first wave reads the KSEG counters.
+template <int T>
+__device__ __forceinline__ void seg_load(SegView &sv, const ull *cnt) {
+  if (threadIdx.x < KSEG) {
+    const uint32_t k = threadIdx.x;
+    const uint32_t l = (uint32_t)cnt[k * CSTR];
+    uint32_t x = (l + T - 1) / T;
+#pragma unroll
+    for (int d = 1; d < KSEG; d <<= 1) {
+      uint32_t y = __shfl_up(x, d);
+      if (k >= (uint32_t)d) x += y;
+    }
+    sv.len[k] = l;
+    sv.tpref[k + 1] = x;
+    if (k == 0) sv.tpref[0] = 0;
+  }
+  __syncthreads();
+}
+
+// Tile v of the segmented input: first element and length.
+template <int T>
+__device__ __forceinline__ bool seg_tile(const SegView &sv, uint32_t v,
+                                         uint64_t cap, uint64_t *start,
+                                         uint32_t *len) {
+  if (v >= sv.tpref[KSEG]) return false;
+  int lo = 0, hi = KSEG - 1;  // last k with tpref[k] <= v
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sv.tpref[mid] <= v) lo = mid; else hi = mid - 1;
+  }
+  const uint32_t off = (v - sv.tpref[lo]) * T;
+  *start = (uint64_t)lo * cap + off;
+  const uint32_t rem = sv.len[lo] - off;
+  *len = rem < (uint32_t)T ? rem : (uint32_t)T;
+  return true;
+}
+
+// Block-aggregated append of IPT items per thread to one segment: one global
+// atomic per call.  Every thread of the block must call it (uniform control
+// flow).  slot[j] receives the element index of item j where pred[j] holds.
 template <int BLOCK, int IPT>
 __device__ __forceinline__ void block_append(const bool (&pred)[IPT],
                                              uint64_t (&slot)[IPT],
-                                             ull *counter) {
+                                             ull *counter, uint64_t seg_base) {
   constexpr int NW = BLOCK / WAVE;
   __shared__ uint32_t s_c[IPT * NW];
   __shared__ ull s_base;
@@ -73,7 +127,7 @@ __device__ __forceinline__ void block_append(const bool (&pred)[IPT],
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < IPT; j++)
-    slot[j] = s_base + s_c[j * NW + wv] + lanes_below(mask[j]);
+    slot[j] = seg_base + s_base + s_c[j * NW + wv] + lanes_below(mask[j]);
   __syncthreads();
 }
 
@@ -89,15 +143,19 @@ __device__ __forceinline__ void min_store(ull *best, uint32_t c, uint64_t key) {
 // ---- union-find over a parent array where every pointer goes to a smaller
 // id (roots are the minimum vertex of their tree) ---------------------------
 //
-// Reads may observe an older value of an entry.  Every value an entry ever
-// holds is an ancestor of the entry, so an older value still lies on a path to
-// the true root; only the compare-and-swap on a root is authoritative.
+// Reads are ordinary cached loads and may observe an older value of an entry
+// (another XCD's L2 is not kept coherent within a kernel).  Every value an
+// entry ever holds is an ancestor of the entry, so an older value still lies
+// on a path to the true root; only the compare-and-swap on a root, performed
+// at device scope, is authoritative, and a failed one returns the current
+// parent, which is strictly smaller.  Path-halving stores only write an
+// ancestor into a non-root entry, which no compare-and-swap ever targets.
 
 __device__ __forceinline__ uint32_t par_load(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void par_store(uint32_t *p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // find with path halving

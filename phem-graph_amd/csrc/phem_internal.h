@@ -14,11 +14,17 @@ int set_err(int code, const char *fmt, ...);
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 inline uint64_t align_up(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
 
-// Grid size for `work` items at `per_block` items per block; kernels loop
-// (grid-stride) over anything beyond the cap.
-inline unsigned grid_for(uint64_t work, uint64_t per_block,
-                         uint64_t cap = 1u << 20) {
-  uint64_t g = ceil_div(work ? work : 1, per_block);
+// Blocks kept resident on the current device for the library's looping
+// kernels: compute units x 8 (256-thread blocks, 32 waves per CU), or x the
+// environment variable PHEM_BLOCKS_PER_CU.
+unsigned resident_blocks();
+
+// Grid size for `work` items at `per_block` items per block, capped at the
+// resident block count: kernels loop over their work, so blocks stay alive
+// across many tiles instead of the dispatcher launching one block per tile.
+inline unsigned grid_for(uint64_t work, uint64_t per_block) {
+  const uint64_t g = ceil_div(work ? work : 1, per_block);
+  const uint64_t cap = resident_blocks();
   return (unsigned)(g < cap ? g : cap);
 }
 

@@ -3,36 +3,49 @@
 // Round structure (the reference's device_boruvka round loop,
 // MST/src/kernels.cu:507-689, re-planned for HBM traffic):
 //
-//   round 0   k_sweep0     : every input edge (u, v, w; id implicit) offers
-//                            key = (wkey(w) << 32) | id to best[u] and best[v]
-//                            (populate_min_weight + populate_eid,
-//                            kernels.cu:24-70, as one 64-bit atomicMin).
-//   round r   k_hook       : every active component c with best[c] = key of
-//                            edge e marks e in the forest bitmap and unions
-//                            the endpoints of e (new_grafting_edges,
-//                            kernels.cu:73-134); the larger root is hooked
-//                            under the smaller, so every root is the minimum
-//                            vertex of its component.
-//             k_compress   : every component active in round r is pointed at
-//                            its root (new_shortcutting_step,
-//                            kernels.cu:254-271) and the roots form the next
-//                            round's component list.
-//             k_compact    : one pass over the alive edges relabels both
-//                            endpoints to roots, drops edges inside one
-//                            component, writes the survivors (u', v', key)
-//                            through an LDS staging tile and offers each
-//                            survivor's key to best[] for round r + 1
-//                            (relabelling_step + filtering_step + the two
-//                            remove_if passes, kernels.cu:273-321,641-649,
-//                            fused with the next round's sweep).
+//   k_hook     : every active component c with best[c] = key of edge e marks
+//                e in the forest bitmap and unions the endpoints of e
+//                (new_grafting_edges, kernels.cu:73-134); the larger root is
+//                hooked under the smaller, so every root is the minimum
+//                vertex of its component.
+//   k_compress : every component active in the round is pointed at its root
+//                (new_shortcutting_step, kernels.cu:254-271) and the roots
+//                form the next round's component list.
+//   k_pass     : one pass over the alive edges relabels both endpoints to
+//                roots, drops edges inside one component, writes the
+//                survivors (u', v', key) through an LDS staging tile and
+//                offers each survivor's key = (wkey(w) << 32) | id to best[]
+//                of both components for the next hook (relabelling_step +
+//                filtering_step + the two remove_if passes,
+//                kernels.cu:273-321,641-649, fused with the next round's
+//                populate_min_weight / populate_eid sweep, kernels.cu:24-70).
 //
-// The packed key makes every component's choice unique, so the forest is the
-// unique minimum spanning forest under (weight, id) order; compaction need not
-// be stable because ids travel with the edges.
+// Two schedules:
+//   canonical  k_sweep0 over the input, then rounds over all edges; the
+//              per-round log equals the reference round loop's.
+//   split      (default) filter-Kruskal in front of the rounds, as the CPU
+//              path does (MST/src/utils.cu:794-841): a sample of the keys
+//              fixes a threshold tau; rounds run over the edges with key <
+//              tau; the parent forest is flattened and its giant component
+//              marked in a bitmap; one pass keeps the heavier edges whose
+//              endpoints are still in different components; rounds finish
+//              those.
+// The packed key makes every component's choice unique, so both give the
+// unique minimum spanning forest under (weight, id) order; compaction need
+// not be stable because ids travel with the edges.
+//
+// Every appended list (alive edges, component lists) is segmented: tile v of
+// a producer appends to segment v mod KSEG, whose counter has its own cache
+// line, so no global atomic is shared by all tiles.
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <initializer_list>
+#include <utility>
 
 #include "phem_dev.h"
 #include "phem_internal.h"
+#include "phem_split.h"
 
 namespace phem {
 
@@ -42,46 +55,106 @@ constexpr int TILE = BLK * EPT;
 constexpr int WPT = 8;         // bitmap words per thread
 constexpr int WPB = BLK * WPT; // bitmap words per block
 constexpr int IPT = 4;         // components per thread in hook / compress
+constexpr int LTILE = BLK * IPT;
 
-// counter slots (ull) in the workspace
-constexpr int C_VALID = 0;   // valid input edges (m_0)
-constexpr int C_WSUM = 1;    // exact weight sum
-constexpr int C_INEXACT = 2; // weights off the 2^-24 grid (F32)
-constexpr int C_MSTCNT = 3;  // forest size
-constexpr int C_ROUND = 8;   // + 4 r: [E(r), A(r), L(r), -]
-constexpr int NCTR = C_ROUND + 4 * (PHEM_MAX_ROUNDS + 1);
+// Counter area (ull), every counter on its own 128-byte line (CSTR ull):
+// scalars, then three blocks of KSEG partial sums (valid edges, weight sum,
+// inexact weights), then per round three segmented counters: E(r) alive edges
+// at the start of round r, A(r) components active in round r, L(r) component
+// list for round r.
+constexpr int C_MSTCNT = 0 * CSTR;  // forest size
+constexpr int C_TAU = 1 * CSTR;     // light / heavy key threshold
+constexpr int C_GIANT = 2 * CSTR;   // giant-component root (32-bit)
+constexpr int C_SUMS = 3 * CSTR;    // k_seg_sums output (up to 8 values)
+constexpr int C_PARTS = 4 * CSTR;
+constexpr int C_ROUND = C_PARTS + 3 * KSEG * CSTR;
+constexpr int RSTRIDE = 3 * KSEG * CSTR;
+constexpr int NCTR = C_ROUND + RSTRIDE * (PHEM_MAX_ROUNDS + 1);
 
-__host__ __device__ constexpr int cE(int r) { return C_ROUND + 4 * r + 0; }
-__host__ __device__ constexpr int cA(int r) { return C_ROUND + 4 * r + 1; }
-__host__ __device__ constexpr int cL(int r) { return C_ROUND + 4 * r + 2; }
+__host__ __device__ constexpr int cE(int r) { return C_ROUND + RSTRIDE * r; }
+__host__ __device__ constexpr int cA(int r) { return cE(r) + KSEG * CSTR; }
+__host__ __device__ constexpr int cL(int r) { return cE(r) + 2 * KSEG * CSTR; }
+// which: 0 valid edges, 1 weight sum, 2 inexact weights
+__host__ __device__ constexpr int cPart(int which) {
+  return C_PARTS + which * KSEG * CSTR;
+}
+
+__device__ __forceinline__ uint64_t ceil_div_dev(uint64_t a, uint64_t b) {
+  return (a + b - 1) / b;
+}
+__device__ __forceinline__ uint64_t min_u64(uint64_t a, uint64_t b) {
+  return a < b ? a : b;
+}
 
 // ---------------------------------------------------------------------------
 // Input edge sources: structure-of-arrays (u[], v[], w[]) or the reference's
-// array of `edge {u, v, w}` records (MST/src/constants.h:15-19).
+// array of `edge {u, v, w}` records (MST/src/constants.h:15-19).  load4 reads
+// the four consecutive edges i4 .. i4+3 below `end` (with 128-bit loads when
+// the arrays are 16-byte aligned and all four are in range).
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 
 struct SrcSoA {
   const uint32_t *u, *v, *w;
+  bool vec;
   __device__ __forceinline__ uint32_t U(uint64_t i) const { return u[i]; }
   __device__ __forceinline__ uint32_t V(uint64_t i) const { return v[i]; }
   __device__ __forceinline__ uint32_t W(uint64_t i) const { return w[i]; }
-  __device__ __forceinline__ void stream_load(uint64_t i, uint32_t &a,
-                                              uint32_t &b, uint32_t &c) const {
-    a = __builtin_nontemporal_load(&u[i]);
-    b = __builtin_nontemporal_load(&v[i]);
-    c = __builtin_nontemporal_load(&w[i]);
+  __device__ __forceinline__ void load4(uint64_t i4, uint64_t end, uint32_t *a,
+                                        uint32_t *b, uint32_t *c) const {
+    if (vec && i4 + 4 <= end) {
+      u32x4 x = __builtin_nontemporal_load((const u32x4 *)(u + i4));
+      u32x4 y = __builtin_nontemporal_load((const u32x4 *)(v + i4));
+      u32x4 z = __builtin_nontemporal_load((const u32x4 *)(w + i4));
+      a[0] = x.x; a[1] = x.y; a[2] = x.z; a[3] = x.w;
+      b[0] = y.x; b[1] = y.y; b[2] = y.z; b[3] = y.w;
+      c[0] = z.x; c[1] = z.y; c[2] = z.z; c[3] = z.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (i4 + q < end) {
+          a[q] = __builtin_nontemporal_load(&u[i4 + q]);
+          b[q] = __builtin_nontemporal_load(&v[i4 + q]);
+          c[q] = __builtin_nontemporal_load(&w[i4 + q]);
+        } else {
+          a[q] = b[q] = U32_NONE;
+          c[q] = 0;
+        }
+      }
+    }
   }
 };
 
 struct SrcAoS {
   const uint32_t *e;  // 3 words per edge
+  bool vec;
   __device__ __forceinline__ uint32_t U(uint64_t i) const { return e[3 * i]; }
   __device__ __forceinline__ uint32_t V(uint64_t i) const { return e[3 * i + 1]; }
   __device__ __forceinline__ uint32_t W(uint64_t i) const { return e[3 * i + 2]; }
-  __device__ __forceinline__ void stream_load(uint64_t i, uint32_t &a,
-                                              uint32_t &b, uint32_t &c) const {
-    a = __builtin_nontemporal_load(&e[3 * i]);
-    b = __builtin_nontemporal_load(&e[3 * i + 1]);
-    c = __builtin_nontemporal_load(&e[3 * i + 2]);
+  __device__ __forceinline__ void load4(uint64_t i4, uint64_t end, uint32_t *a,
+                                        uint32_t *b, uint32_t *c) const {
+    if (vec && i4 + 4 <= end) {
+      const u32x4 *p = (const u32x4 *)(e + 3 * i4);
+      u32x4 x = __builtin_nontemporal_load(p);
+      u32x4 y = __builtin_nontemporal_load(p + 1);
+      u32x4 z = __builtin_nontemporal_load(p + 2);
+      a[0] = x.x; b[0] = x.y; c[0] = x.z; a[1] = x.w;
+      b[1] = y.x; c[1] = y.y; a[2] = y.z; b[2] = y.w;
+      c[2] = z.x; a[3] = z.y; b[3] = z.z; c[3] = z.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (i4 + q < end) {
+          a[q] = __builtin_nontemporal_load(&e[3 * (i4 + q)]);
+          b[q] = __builtin_nontemporal_load(&e[3 * (i4 + q) + 1]);
+          c[q] = __builtin_nontemporal_load(&e[3 * (i4 + q) + 2]);
+        } else {
+          a[q] = b[q] = U32_NONE;
+          c[q] = 0;
+        }
+      }
+    }
   }
 };
 
@@ -104,26 +177,20 @@ __global__ __launch_bounds__(BLK) void k_mst_init(uint32_t n, uint32_t *par,
   }
 }
 
-// Round-0 sweep over the input edge arrays.
+// Round-0 sweep over the input edges (canonical schedule).
 template <class Src>
 __global__ __launch_bounds__(BLK) void k_sweep0(Src src, uint64_t m, uint32_t n,
                                                 int wmode, ull *best) {
   for (uint64_t tile = blockIdx.x; tile * TILE < m; tile += gridDim.x) {
-    const uint64_t base = tile * TILE + threadIdx.x;
     uint32_t a[EPT], b[EPT], ww[EPT];
 #pragma unroll
-    for (int j = 0; j < EPT; j++) {
-      uint64_t i = base + (uint64_t)j * BLK;
-      if (i < m) {
-        src.stream_load(i, a[j], b[j], ww[j]);
-      } else {
-        a[j] = b[j] = U32_NONE;
-        ww[j] = 0;
-      }
-    }
+    for (int g = 0; g < EPT / 4; g++)
+      src.load4(tile * TILE + ((uint64_t)g * BLK + threadIdx.x) * 4, m,
+                a + 4 * g, b + 4 * g, ww + 4 * g);
 #pragma unroll
     for (int j = 0; j < EPT; j++) {
-      uint64_t i = base + (uint64_t)j * BLK;
+      const uint64_t i =
+          tile * TILE + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
       if (a[j] < n && b[j] < n && a[j] != b[j]) {
         uint64_t key = ((uint64_t)wkey(ww[j], wmode) << 32) | i;
         min_store(best, a[j], key);
@@ -133,25 +200,36 @@ __global__ __launch_bounds__(BLK) void k_sweep0(Src src, uint64_t m, uint32_t n,
   }
 }
 
-// Hook: components of `list` (or 0..n_ident-1 when list is NULL).
+// Hook: the components of the segmented list (or 0..n_ident-1 when the list
+// counter is NULL).  Active components go to the segmented list `act`.
 template <class Src>
 __global__ __launch_bounds__(BLK) void k_hook(
-    const uint32_t *__restrict__ list, const ull *list_count, uint32_t n_ident,
-    ull *best, Src src, uint32_t *par, uint32_t *bitmap, uint32_t *act,
-    ull *act_count) {
-  const uint64_t cnt = list ? *list_count : (uint64_t)n_ident;
-  for (uint64_t base = (uint64_t)blockIdx.x * (BLK * IPT); base < cnt;
-       base += (uint64_t)gridDim.x * (BLK * IPT)) {
+    const uint32_t *__restrict__ list, const ull *list_cnt, uint64_t list_cap,
+    uint32_t n_ident, ull *best, Src src, uint32_t *par, uint32_t *bitmap,
+    uint32_t *act, ull *act_cnt, uint64_t act_cap) {
+  __shared__ SegView sv;
+  if (list_cnt) seg_load<LTILE>(sv, list_cnt);
+  const uint32_t ntiles_ident = (uint32_t)ceil_div_dev(n_ident, LTILE);
+  for (uint32_t v = blockIdx.x;; v += gridDim.x) {
+    uint64_t start;
+    uint32_t len;
+    if (list_cnt) {
+      if (!seg_tile<LTILE>(sv, v, list_cap, &start, &len)) break;
+    } else {
+      if (v >= ntiles_ident) break;
+      start = (uint64_t)v * LTILE;
+      len = (uint32_t)min_u64(LTILE, (uint64_t)n_ident - start);
+    }
     bool active[IPT];
     uint32_t c[IPT];
     uint64_t slot[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-      const uint64_t idx = base + (uint64_t)j * BLK + threadIdx.x;
+      const uint32_t idx = (uint32_t)j * BLK + threadIdx.x;
       active[j] = false;
       c[j] = 0;
-      if (idx < cnt) {
-        c[j] = list ? list[idx] : (uint32_t)idx;
+      if (idx < len) {
+        c[j] = list_cnt ? list[start + idx] : (uint32_t)(start + idx);
         ull key = best[c[j]];
         if (key != KEY_NONE) {
           active[j] = true;
@@ -162,7 +240,9 @@ __global__ __launch_bounds__(BLK) void k_hook(
         }
       }
     }
-    block_append<BLK, IPT>(active, slot, act_count);
+    const uint32_t seg = v % KSEG;
+    block_append<BLK, IPT>(active, slot, act_cnt + seg * CSTR,
+                           (uint64_t)seg * act_cap);
 #pragma unroll
     for (int j = 0; j < IPT; j++)
       if (active[j]) act[slot[j]] = c[j];
@@ -170,123 +250,200 @@ __global__ __launch_bounds__(BLK) void k_hook(
 }
 
 // Point every active component at its root; collect the roots.
-__global__ __launch_bounds__(BLK) void k_compress(const uint32_t *__restrict__ act,
-                                                  const ull *act_count,
-                                                  uint32_t *par, uint32_t *next,
-                                                  ull *next_count) {
-  const uint64_t cnt = *act_count;
-  for (uint64_t base = (uint64_t)blockIdx.x * (BLK * IPT); base < cnt;
-       base += (uint64_t)gridDim.x * (BLK * IPT)) {
+__global__ __launch_bounds__(BLK) void k_compress(
+    const uint32_t *__restrict__ act, const ull *act_cnt, uint64_t act_cap,
+    uint32_t *par, uint32_t *next, ull *next_cnt, uint64_t next_cap) {
+  __shared__ SegView sv;
+  seg_load<LTILE>(sv, act_cnt);
+  for (uint32_t v = blockIdx.x;; v += gridDim.x) {
+    uint64_t start;
+    uint32_t len;
+    if (!seg_tile<LTILE>(sv, v, act_cap, &start, &len)) break;
     bool root[IPT];
     uint32_t c[IPT];
     uint64_t slot[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-      const uint64_t idx = base + (uint64_t)j * BLK + threadIdx.x;
+      const uint32_t idx = (uint32_t)j * BLK + threadIdx.x;
       root[j] = false;
       c[j] = 0;
-      if (idx < cnt) {
-        c[j] = act[idx];
+      if (idx < len) {
+        c[j] = act[start + idx];
         uint32_t r = uf_root(par, c[j]);
         if (r != c[j]) par[c[j]] = r;
         root[j] = (r == c[j]);
       }
     }
-    block_append<BLK, IPT>(root, slot, next_count);
+    const uint32_t seg = v % KSEG;
+    block_append<BLK, IPT>(root, slot, next_cnt + seg * CSTR,
+                           (uint64_t)seg * next_cap);
 #pragma unroll
     for (int j = 0; j < IPT; j++)
       if (root[j]) next[slot[j]] = c[j];
   }
 }
 
-// Relabel + drop intra-component edges + compact + next-round sweep.
-// R0: the input is the original (u, v, w) arrays, keys are formed from w and
-// the index, and invalid edges are dropped.
-template <bool R0, class Src>
-__global__ __launch_bounds__(BLK) void k_compact(
+// Edge passes.  Every pass reads a stream of edges, decides which survive,
+// writes the survivors (component u, component v, key) through an LDS
+// staging tile (one atomic per tile, on the counter of the tile's output
+// segment) and offers each survivor's key to best[] of both components.
+//   P_ALL   input edges, relabelled through par (canonical round 0)
+//   P_LIGHT input edges with key < tau, not relabelled (par is the identity)
+//   P_HEAVY input edges with key >= tau whose endpoints are not both in the
+//           giant component (bitmap) and have different roots (par flattened)
+//   P_EDGES segmented alive edges of the previous round, relabelled
+enum PassMode : int { P_ALL = 0, P_LIGHT = 1, P_HEAVY = 2, P_EDGES = 3 };
+
+template <int MODE, class Src>
+__global__ __launch_bounds__(BLK) void k_pass(
     Src src, const uint32_t *__restrict__ ia, const uint32_t *__restrict__ ib,
-    const ull *__restrict__ ik, const ull *in_count, uint64_t m0, uint32_t n, int wmode,
-    const uint32_t *__restrict__ par, uint32_t *__restrict__ oa,
-    uint32_t *__restrict__ ob, ull *__restrict__ ok, ull *out_count,
-    ull *valid_count, ull *best) {
-  __shared__ uint32_t s_a[TILE];
-  __shared__ uint32_t s_b[TILE];
-  __shared__ ull s_k[TILE];
-  __shared__ uint32_t s_cnt, s_valid;
-  __shared__ ull s_base;
-  const uint64_t mr = R0 ? m0 : *in_count;
-  const uint32_t lane = lane_id();
-  for (uint64_t tile = blockIdx.x; tile * TILE < mr; tile += gridDim.x) {
-    if (threadIdx.x == 0) {
-      s_cnt = 0;
-      s_valid = 0;
+    const ull *__restrict__ ik, const ull *in_cnt, uint64_t in_cap,
+    uint64_t m0, uint32_t n, int wmode, const uint32_t *__restrict__ par,
+    const ull *tau_ptr, const uint32_t *__restrict__ giant,
+    uint32_t *__restrict__ oa, uint32_t *__restrict__ ob,
+    ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *valid_parts,
+    ull *best) {
+  __shared__ SegView sv;
+  constexpr bool INPUT = MODE != P_EDGES;
+  if (!INPUT) seg_load<TILE>(sv, in_cnt);
+  const ull tau = (MODE == P_LIGHT || MODE == P_HEAVY) ? *tau_ptr : 0ull;
+  const uint32_t ntiles_in = INPUT ? (uint32_t)ceil_div_dev(m0, TILE) : 0u;
+  uint32_t nval = 0;  // valid input edges seen by this thread
+  for (uint32_t v = blockIdx.x;; v += gridDim.x) {
+    uint64_t start;
+    uint32_t len;
+    if (INPUT) {
+      if (v >= ntiles_in) break;
+      start = (uint64_t)v * TILE;
+      len = (uint32_t)min_u64(TILE, m0 - start);
+    } else {
+      if (!seg_tile<TILE>(sv, v, in_cap, &start, &len)) break;
     }
-    __syncthreads();
-    const uint64_t base = tile * TILE + threadIdx.x;
+    const uint64_t end = start + len;
     uint32_t a[EPT], b[EPT];
     ull key[EPT];
     bool val[EPT];
 #pragma unroll
-    for (int j = 0; j < EPT; j++) {
-      uint64_t i = base + (uint64_t)j * BLK;
-      val[j] = false;
-      a[j] = b[j] = 0;
-      key[j] = KEY_NONE;
-      if (i < mr) {
-        if (R0) {
-          uint32_t wv;
-          src.stream_load(i, a[j], b[j], wv);
-          key[j] = ((ull)wkey(wv, wmode) << 32) | i;
-          val[j] = a[j] < n && b[j] < n && a[j] != b[j];
-        } else {
-          a[j] = __builtin_nontemporal_load(&ia[i]);
-          b[j] = __builtin_nontemporal_load(&ib[i]);
-          key[j] = __builtin_nontemporal_load(&ik[i]);
-          val[j] = true;
+    for (int g = 0; g < EPT / 4; g++) {
+      const uint64_t i4 = start + ((uint64_t)g * BLK + threadIdx.x) * 4;
+      if (INPUT) {
+        uint32_t wv[4];
+        src.load4(i4, end, a + 4 * g, b + 4 * g, wv);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          key[4 * g + q] = ((ull)wkey(wv[q], wmode) << 32) | (i4 + q);
+          val[4 * g + q] = a[4 * g + q] < n && b[4 * g + q] < n &&
+                           a[4 * g + q] != b[4 * g + q];
+        }
+      } else if (i4 + 4 <= end) {
+        u32x4 x = __builtin_nontemporal_load((const u32x4 *)(ia + i4));
+        u32x4 y = __builtin_nontemporal_load((const u32x4 *)(ib + i4));
+        u64x2 k0 = __builtin_nontemporal_load((const u64x2 *)(ik + i4));
+        u64x2 k1 = __builtin_nontemporal_load((const u64x2 *)(ik + i4 + 2));
+        a[4 * g] = x.x; a[4 * g + 1] = x.y; a[4 * g + 2] = x.z; a[4 * g + 3] = x.w;
+        b[4 * g] = y.x; b[4 * g + 1] = y.y; b[4 * g + 2] = y.z; b[4 * g + 3] = y.w;
+        key[4 * g] = k0.x; key[4 * g + 1] = k0.y;
+        key[4 * g + 2] = k1.x; key[4 * g + 3] = k1.y;
+#pragma unroll
+        for (int q = 0; q < 4; q++) val[4 * g + q] = true;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const bool in = i4 + q < end;
+          a[4 * g + q] = in ? ia[i4 + q] : 0u;
+          b[4 * g + q] = in ? ib[i4 + q] : 0u;
+          key[4 * g + q] = in ? ik[i4 + q] : KEY_NONE;
+          val[4 * g + q] = in;
         }
       }
     }
+    if (INPUT) {
 #pragma unroll
-    for (int j = 0; j < EPT; j++) {
-      if (val[j]) {
-        a[j] = par[a[j]];
-        b[j] = par[b[j]];
+      for (int j = 0; j < EPT; j++) nval += val[j];
+    }
+    if (MODE == P_LIGHT) {
+#pragma unroll
+      for (int j = 0; j < EPT; j++) val[j] = val[j] && key[j] < tau;
+    }
+    if (MODE == P_HEAVY) {
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        val[j] = val[j] && key[j] >= tau;
+        if (val[j]) {
+          const bool ga = (giant[a[j] >> 5] >> (a[j] & 31)) & 1u;
+          const bool gb = (giant[b[j] >> 5] >> (b[j] & 31)) & 1u;
+          val[j] = !(ga && gb);
+        }
       }
     }
+    if (MODE != P_LIGHT) {
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        if (val[j]) {
+          a[j] = par[a[j]];
+          b[j] = par[b[j]];
+        }
+      }
+    }
+    bool keep[EPT];
 #pragma unroll
     for (int j = 0; j < EPT; j++) {
-      const bool keep = val[j] && a[j] != b[j];
-      const uint64_t mask = __ballot(keep);
-      const uint64_t vmask = R0 ? __ballot(val[j]) : 0ull;
-      uint32_t wbase = 0;
-      if (lane == 0) {
-        if (mask) wbase = atomicAdd(&s_cnt, (uint32_t)__popcll(mask));
-        if (R0 && vmask) atomicAdd(&s_valid, (uint32_t)__popcll(vmask));
-      }
-      wbase = __shfl(wbase, 0);
-      if (keep) {
-        uint32_t pos = wbase + lanes_below(mask);
-        s_a[pos] = a[j];
-        s_b[pos] = b[j];
-        s_k[pos] = key[j];
+      keep[j] = val[j] && a[j] != b[j];
+      if (keep[j]) {
         min_store(best, a[j], key[j]);
         min_store(best, b[j], key[j]);
       }
     }
+    // survivors go straight from registers to their slots in the tile's
+    // output segment (one atomic per tile)
+    const uint32_t seg = v % KSEG;
+    uint64_t slot[EPT];
+    block_append<BLK, EPT>(keep, slot, out_cnt + seg * CSTR,
+                           (uint64_t)seg * out_cap);
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+      if (keep[j]) {
+        oa[slot[j]] = a[j];
+        ob[slot[j]] = b[j];
+        ok[slot[j]] = key[j];
+      }
+    }
+  }
+  if (INPUT && valid_parts) {
+    __shared__ ull s_w[BLK / WAVE];
+    ull x = nval;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
+    if (lane_id() == 0) s_w[threadIdx.x / WAVE] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
-      s_base = s_cnt ? atomicAdd(out_count, (ull)s_cnt) : 0ull;
-      if (R0 && s_valid) atomicAdd(valid_count, (ull)s_valid);
+      ull tot = 0;
+      for (int w = 0; w < BLK / WAVE; w++) tot += s_w[w];
+      if (tot) atomicAdd(valid_parts + (blockIdx.x % KSEG) * CSTR, tot);
     }
-    __syncthreads();
-    const uint32_t cnt = s_cnt;
-    const ull ob_base = s_base;
-    for (uint32_t k = threadIdx.x; k < cnt; k += BLK) {
-      oa[ob_base + k] = s_a[k];
-      ob[ob_base + k] = s_b[k];
-      ok[ob_base + k] = s_k[k];
-    }
-    __syncthreads();
+  }
+}
+
+// tau = key of rank ~light_edges in a strided sample of the input keys
+// (KEY_NONE: every edge is light).
+template <class Src>
+__global__ __launch_bounds__(SBLK) void k_threshold(Src src, uint64_t m,
+                                                    uint32_t n, int wmode,
+                                                    uint64_t light_edges,
+                                                    ull *tau_out) {
+  __shared__ ull s[NSAMP];
+  for (int j = threadIdx.x; j < NSAMP; j += SBLK) {
+    const uint64_t i = (uint64_t)j * m / NSAMP;
+    const uint32_t a = src.U(i), b = src.V(i);
+    s[j] = (a < n && b < n && a != b)
+               ? (((ull)wkey(src.W(i), wmode) << 32) | i)
+               : KEY_NONE;
+  }
+  __syncthreads();
+  bitonic_sort_lds(s);
+  if (threadIdx.x == 0) {
+    ull r = (ull)((double)light_edges * NSAMP / (double)m);
+    *tau_out = (light_edges >= m || r >= NSAMP) ? KEY_NONE : s[r];
   }
 }
 
@@ -369,14 +526,16 @@ __device__ __forceinline__ bool weight_fx(uint32_t bits, int wmode, ull *out) {
   return true;
 }
 
-// Bitmap -> ascending ids; exact weight sum of the forest.
+// Bitmap -> ascending ids; exact weight sum of the forest (partial sums
+// spread over KSEG counters).
 template <class Src>
 __global__ __launch_bounds__(BLK) void k_bm_emit(const uint32_t *__restrict__ bm,
                                                  uint64_t words,
                                                  const ull *__restrict__ blk,
                                                  uint32_t *__restrict__ ids,
-                                                 Src src, int wmode, ull *wsum,
-                                                 ull *inexact) {
+                                                 Src src, int wmode,
+                                                 ull *wsum_parts,
+                                                 ull *inexact_parts) {
   const uint64_t w0 = (uint64_t)blockIdx.x * WPB + (uint64_t)threadIdx.x * WPT;
   uint32_t word[WPT];
   ull c = 0;
@@ -404,8 +563,28 @@ __global__ __launch_bounds__(BLK) void k_bm_emit(const uint32_t *__restrict__ bm
   block_excl_scan(s, &ts);
   block_excl_scan(bad, &tb);
   if (threadIdx.x == 0) {
-    if (ts) atomicAdd(wsum, ts);
-    if (tb) atomicAdd(inexact, tb);
+    const uint32_t seg = blockIdx.x % KSEG;
+    if (ts) atomicAdd(wsum_parts + seg * CSTR, ts);
+    if (tb) atomicAdd(inexact_parts + seg * CSTR, tb);
+  }
+}
+
+// Sums of counters for the host: value i is ctr[off[i]] (segs[i] == 1) or
+// the sum of the KSEG segment counters at off[i].  One wave.
+struct SumReq {
+  int n;
+  int off[8];
+  int segs[8];
+};
+
+__global__ __launch_bounds__(WAVE) void k_seg_sums(const ull *ctr, SumReq q,
+                                                   ull *out) {
+  const uint32_t lane = threadIdx.x;
+  for (int i = 0; i < q.n; i++) {
+    ull x = (int)lane < q.segs[i] ? ctr[q.off[i] + lane * CSTR] : 0ull;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
+    if (lane == 0) out[i] = x;
   }
 }
 
@@ -414,8 +593,8 @@ __global__ __launch_bounds__(BLK) void k_bm_emit(const uint32_t *__restrict__ bm
 
 struct MstLayout {
   uint64_t par, best, listL, listA, ea_a, ea_b, ea_k, eb_a, eb_b, eb_k, bitmap,
-      ctr, blk, total;
-  uint64_t words, nb;
+      gbits, ctr, blk, total;
+  uint64_t words, nb, ecap, lcap;
 };
 
 static MstLayout mst_layout(uint32_t n, uint64_t m) {
@@ -429,17 +608,23 @@ static MstLayout mst_layout(uint32_t n, uint64_t m) {
   };
   L.words = ceil_div(m ? m : 1, 32);
   L.nb = ceil_div(L.words, WPB);
+  // Segment capacities: a producer pass over at most ceil(x / T) + KSEG tiles
+  // sends at most ceil(ceil(x / T) / KSEG) + 1 tiles to one segment.
+  L.ecap = (ceil_div(ceil_div(m ? m : 1, TILE), KSEG) + 2) * TILE;
+  L.lcap = (ceil_div(ceil_div(n ? n : 1, LTILE), KSEG) + 2) * LTILE;
+  const uint64_t E = L.ecap * KSEG, LN = L.lcap * KSEG;
   L.par = take(4ull * n);
   L.best = take(8ull * n);
-  L.listL = take(4ull * n);
-  L.listA = take(4ull * n);
-  L.ea_a = take(4ull * m);
-  L.ea_b = take(4ull * m);
-  L.ea_k = take(8ull * m);
-  L.eb_a = take(4ull * m);
-  L.eb_b = take(4ull * m);
-  L.eb_k = take(8ull * m);
+  L.listL = take(4ull * LN);
+  L.listA = take(4ull * LN);
+  L.ea_a = take(4ull * E);
+  L.ea_b = take(4ull * E);
+  L.ea_k = take(8ull * E);
+  L.eb_a = take(4ull * E);
+  L.eb_b = take(4ull * E);
+  L.eb_k = take(8ull * E);
   L.bitmap = take(4ull * L.words);
+  L.gbits = take(4ull * ceil_div(n ? n : 1, 32));
   L.ctr = take(8ull * NCTR);
   L.blk = take(8ull * L.nb);
   L.total = o;
@@ -462,11 +647,19 @@ extern "C" int phem_mst_workspace_bytes(uint32_t n, uint64_t m,
 
 namespace phem {
 
+double mst_default_light_factor() {
+  static double f = [] {
+    const char *e = getenv("PHEM_MST_LIGHT_FACTOR");
+    return e ? atof(e) : 2.0;
+  }();
+  return f;
+}
+
 template <class Src>
 static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                           void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
-                          uint32_t *d_labels, phem_mst_stats *stats,
-                          hipStream_t s) {
+                          uint32_t *d_labels, const phem_mst_opts *opts,
+                          phem_mst_stats *stats, hipStream_t s) {
   if (m > PHEM_MAX_EDGES || n > PHEM_MAX_VERTICES)
     return set_err(PHEM_EINVAL, "n=%u m=%llu beyond limits", n,
                    (unsigned long long)m);
@@ -477,6 +670,12 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   if (!d_ws || ws_bytes < L.total)
     return set_err(PHEM_ENOMEM, "workspace %llu B < required %llu B",
                    (unsigned long long)ws_bytes, (unsigned long long)L.total);
+  const double light_factor =
+      opts ? opts->light_factor : mst_default_light_factor();
+  const uint64_t light_edges =
+      light_factor > 0 ? (uint64_t)(light_factor * (double)n) : m;
+  const bool split = light_edges < m;
+
   char *ws = (char *)d_ws;
   uint32_t *par = (uint32_t *)(ws + L.par);
   ull *best = (ull *)(ws + L.best);
@@ -486,8 +685,11 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   uint32_t *Eb[2] = {(uint32_t *)(ws + L.ea_b), (uint32_t *)(ws + L.eb_b)};
   ull *Ek[2] = {(ull *)(ws + L.ea_k), (ull *)(ws + L.eb_k)};
   uint32_t *bitmap = (uint32_t *)(ws + L.bitmap);
+  uint32_t *gbits = (uint32_t *)(ws + L.gbits);
   ull *ctr = (ull *)(ws + L.ctr);
   ull *blk = (ull *)(ws + L.blk);
+  const uint32_t *nou = nullptr;
+  const ull *noull = nullptr;
 
   if (stats) std::memset(stats, 0, sizeof(*stats));
 
@@ -501,88 +703,191 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   }
   PHEM_LAUNCHED("k_mst_init");
 
-  int rounds = 0;
-  if (m > 0 && n > 1) {
-    // round 0: sweep, hook and compress over all vertices, compact the input
+  // Counter values for the host: summed on the device, copied to a pinned
+  // buffer (one per host thread, kept for the life of the process).
+  static thread_local ull *hpin = nullptr;
+  if (!hpin) PHEM_HIP(hipHostMalloc((void **)&hpin, 8 * sizeof(ull)));
+  auto read_sums = [&](std::initializer_list<std::pair<int, int>> req,
+                       ull *out) -> int {
+    SumReq q{};
+    for (auto &p : req) {
+      q.off[q.n] = p.first;
+      q.segs[q.n] = p.second;
+      q.n++;
+    }
+    hipLaunchKernelGGL(k_seg_sums, dim3(1), dim3(WAVE), 0, s, ctr, q,
+                       ctr + C_SUMS);
+    PHEM_LAUNCHED("k_seg_sums");
+    PHEM_HIP(hipMemcpyAsync(hpin, ctr + C_SUMS, q.n * sizeof(ull),
+                            hipMemcpyDeviceToHost, s));
+    PHEM_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < q.n; i++) out[i] = hpin[i];
+    return PHEM_OK;
+  };
+  ull hv[8];
+  int r = 0;    // round index (a round = hook + compress + compact)
+  int cur = 0;  // edge buffer holding the alive edges of round r
+  uint64_t m_r = 0;
+
+  // Rounds over the alive edges in buffer `cur` until none is left.  The
+  // first hook of a phase scans all n vertices (no list); later hooks scan
+  // the roots collected by the previous compress.
+  auto run_rounds = [&](uint64_t list_ub, bool first_all) -> int {
+    bool all = first_all;
+    while (m_r > 0) {
+      if (r >= PHEM_MAX_ROUNDS)
+        return set_err(PHEM_ENOCONV, "no convergence after %d rounds", r);
+      const uint64_t hook_tiles =
+          all ? ceil_div(n, LTILE) : ceil_div(list_ub, LTILE) + KSEG;
+      {
+        PHEM_PROF("k_hook", s);
+        hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(hook_tiles, 1)),
+                           dim3(BLK), 0, s, listL, all ? noull : ctr + cL(r),
+                           L.lcap, all ? n : 0u, best, src, par, bitmap, listA,
+                           ctr + cA(r), L.lcap);
+      }
+      PHEM_LAUNCHED("k_hook");
+      {
+        PHEM_PROF("k_compress", s);
+        hipLaunchKernelGGL(k_compress, dim3(grid_for(hook_tiles + KSEG, 1)),
+                           dim3(BLK), 0, s, listA, ctr + cA(r), L.lcap, par,
+                           listL, ctr + cL(r + 1), L.lcap);
+      }
+      PHEM_LAUNCHED("k_compress");
+      {
+        PHEM_PROF("k_compact", s);
+        hipLaunchKernelGGL((k_pass<P_EDGES, Src>),
+                           dim3(grid_for(ceil_div(m_r, TILE) + KSEG, 1)),
+                           dim3(BLK), 0, s, src, Ea[cur], Eb[cur], Ek[cur],
+                           ctr + cE(r), L.ecap, 0ull, n, wmode, par, noull, nou,
+                           Ea[cur ^ 1], Eb[cur ^ 1], Ek[cur ^ 1],
+                           ctr + cE(r + 1), L.ecap, (ull *)nullptr, best);
+      }
+      PHEM_LAUNCHED("k_compact");
+      int rc = read_sums({{cE(r), KSEG}, {cA(r), KSEG}, {cE(r + 1), KSEG},
+                          {cL(r + 1), KSEG}}, hv);
+      if (rc) return rc;
+      if (stats) {
+        stats->m_log[r] = hv[0];
+        stats->c_log[r] = hv[1];
+      }
+      m_r = hv[2];      // E(r + 1)
+      list_ub = hv[3];  // L(r + 1)
+      cur ^= 1;
+      r++;
+      all = false;
+    }
+    return PHEM_OK;
+  };
+
+  if (m > 0 && n > 1 && !split) {
+    // canonical Boruvka: round 0 sweeps the input, then hooks every vertex
     {
       PHEM_PROF("k_sweep0", s);
       hipLaunchKernelGGL(k_sweep0<Src>, dim3(grid_for(m, TILE)), dim3(BLK), 0,
                          s, src, m, n, wmode, best);
     }
     PHEM_LAUNCHED("k_sweep0");
+    const uint64_t t0 = ceil_div(n, LTILE);
     {
       PHEM_PROF("k_hook", s);
-      hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(n, BLK * IPT)), dim3(BLK),
-                         0, s, (const uint32_t *)nullptr, (const ull *)nullptr,
-                         n, best, src, par, bitmap, listA, ctr + cA(0));
+      hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(t0, 1)), dim3(BLK), 0, s,
+                         listL, noull, L.lcap, n, best, src, par, bitmap, listA,
+                         ctr + cA(0), L.lcap);
     }
     PHEM_LAUNCHED("k_hook");
     {
       PHEM_PROF("k_compress", s);
-      hipLaunchKernelGGL(k_compress, dim3(grid_for(n, BLK * IPT)), dim3(BLK), 0,
-                         s, listA, ctr + cA(0), par, listL, ctr + cL(1));
+      hipLaunchKernelGGL(k_compress, dim3(grid_for(t0 + KSEG, 1)), dim3(BLK), 0,
+                         s, listA, ctr + cA(0), L.lcap, par, listL, ctr + cL(1),
+                         L.lcap);
     }
     PHEM_LAUNCHED("k_compress");
     {
       PHEM_PROF("k_compact<R0>", s);
-      hipLaunchKernelGGL((k_compact<true, Src>), dim3(grid_for(m, TILE)),
-                         dim3(BLK), 0, s, src, (const uint32_t *)nullptr,
-                         (const uint32_t *)nullptr, (const ull *)nullptr,
-                         (const ull *)nullptr, m, n, wmode, par, Ea[0], Eb[0],
-                         Ek[0], ctr + cE(1), ctr + C_VALID, best);
+      hipLaunchKernelGGL((k_pass<P_ALL, Src>), dim3(grid_for(m, TILE)),
+                         dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
+                         n, wmode, par, noull, nou, Ea[0], Eb[0], Ek[0],
+                         ctr + cE(1), L.ecap, ctr + cPart(0), best);
     }
     PHEM_LAUNCHED("k_compact<R0>");
-    ull h[16];
-    PHEM_HIP(hipMemcpyAsync(h, ctr, 16 * sizeof(ull), hipMemcpyDeviceToHost, s));
-    PHEM_HIP(hipStreamSynchronize(s));
-    uint64_t m_r = h[cE(1)], list_ub = h[cL(1)];
+    int rc = read_sums({{cPart(0), KSEG}, {cA(0), KSEG}, {cE(1), KSEG},
+                        {cL(1), KSEG}}, hv);
+    if (rc) return rc;
+    const uint64_t valid = hv[0];
     if (stats) {
-      stats->m_log[0] = h[C_VALID];
-      stats->c_log[0] = h[cA(0)];
+      stats->m_log[0] = valid;
+      stats->c_log[0] = hv[1];
+      stats->light_edges = valid;
+      stats->tau = KEY_NONE;
     }
     // a round counts when it starts with at least one alive edge
-    rounds = h[C_VALID] ? 1 : 0;
-    int cur = 0;
-    while (m_r > 0) {
-      if (rounds >= PHEM_MAX_ROUNDS)
-        return set_err(PHEM_ENOCONV, "no convergence after %d rounds", rounds);
-      const int r = rounds;
-      {
-        PHEM_PROF("k_hook", s);
-        hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(list_ub, BLK * IPT)),
-                           dim3(BLK), 0, s, listL, ctr + cL(r), 0u, best, src,
-                           par, bitmap, listA, ctr + cA(r));
-      }
-      PHEM_LAUNCHED("k_hook");
-      {
-        PHEM_PROF("k_compress", s);
-        hipLaunchKernelGGL(k_compress, dim3(grid_for(list_ub, BLK * IPT)),
-                           dim3(BLK), 0, s, listA, ctr + cA(r), par, listL,
-                           ctr + cL(r + 1));
-      }
-      PHEM_LAUNCHED("k_compress");
-      {
-        PHEM_PROF("k_compact", s);
-        hipLaunchKernelGGL((k_compact<false, Src>), dim3(grid_for(m_r, TILE)),
-                           dim3(BLK), 0, s, src, Ea[cur], Eb[cur], Ek[cur],
-                           ctr + cE(r), 0ull, n, wmode, par, Ea[cur ^ 1],
-                           Eb[cur ^ 1], Ek[cur ^ 1], ctr + cE(r + 1),
-                           (ull *)nullptr, best);
-      }
-      PHEM_LAUNCHED("k_compact");
-      ull hr[8];
-      PHEM_HIP(hipMemcpyAsync(hr, ctr + cE(r), 8 * sizeof(ull),
-                              hipMemcpyDeviceToHost, s));
-      PHEM_HIP(hipStreamSynchronize(s));
-      if (stats) {
-        stats->m_log[r] = hr[0];
-        stats->c_log[r] = hr[1];
-      }
-      m_r = hr[4];      // E(r + 1)
-      list_ub = hr[6];  // L(r + 1)
-      cur ^= 1;
-      rounds++;
+    r = valid ? 1 : 0;
+    m_r = hv[2];
+    cur = 0;
+    rc = run_rounds(hv[3], false);
+    if (rc) return rc;
+    if (stats) stats->light_rounds = (uint32_t)r;
+  } else if (m > 0 && n > 1) {
+    // light phase: edges with key < tau (about light_edges of them)
+    {
+      PHEM_PROF("k_threshold", s);
+      hipLaunchKernelGGL(k_threshold<Src>, dim3(1), dim3(SBLK), 0, s, src, m,
+                         n, wmode, light_edges, ctr + C_TAU);
     }
+    PHEM_LAUNCHED("k_threshold");
+    {
+      PHEM_PROF("k_light", s);
+      hipLaunchKernelGGL((k_pass<P_LIGHT, Src>), dim3(grid_for(m, TILE)),
+                         dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
+                         n, wmode, par, ctr + C_TAU, nou, Ea[0], Eb[0], Ek[0],
+                         ctr + cE(0), L.ecap, ctr + cPart(0), best);
+    }
+    PHEM_LAUNCHED("k_light");
+    int rc = read_sums({{C_TAU, 1}, {cE(0), KSEG}}, hv);
+    if (rc) return rc;
+    if (stats) {
+      stats->tau = hv[0];
+      stats->light_edges = hv[1];
+    }
+    m_r = hv[1];
+    cur = 0;
+    rc = run_rounds(0, true);
+    if (rc) return rc;
+    if (stats) stats->light_rounds = (uint32_t)r;
+    // heavy phase: flatten, mark the giant component, filter the rest
+    {
+      PHEM_PROF("k_flatten", s);
+      hipLaunchKernelGGL(k_flatten, dim3(grid_for(n, BLK)), dim3(BLK), 0, s, n,
+                         par);
+    }
+    PHEM_LAUNCHED("k_flatten");
+    {
+      PHEM_PROF("k_pick_giant", s);
+      hipLaunchKernelGGL(k_pick_giant, dim3(1), dim3(SBLK), 0, s, n, par,
+                         (uint32_t *)(ctr + C_GIANT));
+    }
+    PHEM_LAUNCHED("k_pick_giant");
+    {
+      PHEM_PROF("k_giant_bits", s);
+      hipLaunchKernelGGL(k_giant_bits, dim3(grid_for(n, BLK)), dim3(BLK), 0, s,
+                         n, par, (const uint32_t *)(ctr + C_GIANT), gbits);
+    }
+    PHEM_LAUNCHED("k_giant_bits");
+    {
+      PHEM_PROF("k_heavy", s);
+      hipLaunchKernelGGL((k_pass<P_HEAVY, Src>), dim3(grid_for(m, TILE)),
+                         dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
+                         n, wmode, par, ctr + C_TAU, gbits, Ea[cur], Eb[cur],
+                         Ek[cur], ctr + cE(r), L.ecap, (ull *)nullptr, best);
+    }
+    PHEM_LAUNCHED("k_heavy");
+    rc = read_sums({{cE(r), KSEG}}, hv);
+    if (rc) return rc;
+    m_r = hv[0];
+    if (stats) stats->heavy_edges = m_r;
+    rc = run_rounds(0, true);
+    if (rc) return rc;
   }
 
   if (d_labels && n) {
@@ -609,24 +914,27 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     PHEM_PROF("k_bm_emit", s);
     hipLaunchKernelGGL(k_bm_emit<Src>, dim3((unsigned)L.nb), dim3(BLK), 0, s,
                        bitmap, L.words, blk, d_mst_ids, src, wmode,
-                       ctr + C_WSUM, ctr + C_INEXACT);
+                       ctr + cPart(1), ctr + cPart(2));
   }
   PHEM_LAUNCHED("k_bm_emit");
-  ull tail[4];
-  PHEM_HIP(hipMemcpyAsync(tail, ctr, 4 * sizeof(ull), hipMemcpyDeviceToHost, s));
-  PHEM_HIP(hipStreamSynchronize(s));
+  int rc = read_sums({{C_MSTCNT, 1}, {cPart(1), KSEG}, {cPart(2), KSEG},
+                      {cPart(0), KSEG}}, hv);
+  if (rc) return rc;
   if (stats) {
-    stats->rounds = (uint32_t)rounds;
-    stats->mst_count = (uint32_t)tail[C_MSTCNT];
-    stats->weight_fx = tail[C_WSUM];
-    stats->weight_exact = tail[C_INEXACT] == 0;
-    for (int r = 0; r < rounds; r++) {
-      stats->sum_m += stats->m_log[r];
-      stats->sum_c += stats->c_log[r];
+    stats->rounds = (uint32_t)r;
+    stats->mst_count = (uint32_t)hv[0];
+    stats->weight_fx = hv[1];
+    stats->weight_exact = hv[2] == 0;
+    for (int k = 0; k < r; k++) {
+      stats->sum_m += stats->m_log[k];
+      stats->sum_c += stats->c_log[k];
     }
+    stats->valid_edges = hv[3];
   }
   return PHEM_OK;
 }
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 }  // namespace phem
 
@@ -634,19 +942,24 @@ extern "C" int phem_mst_solve(uint32_t n, uint64_t m, const uint32_t *d_u,
                               const uint32_t *d_v, const uint32_t *d_w,
                               int wmode, void *d_ws, uint64_t ws_bytes,
                               uint32_t *d_mst_ids, uint32_t *d_labels,
-                              phem_mst_stats *stats, void *stream) {
+                              const phem_mst_opts *opts, phem_mst_stats *stats,
+                              void *stream) {
   if (m && (!d_u || !d_v || !d_w))
     return set_err(PHEM_EINVAL, "edge arrays are NULL");
-  return mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w}, wmode, d_ws, ws_bytes,
-                        d_mst_ids, d_labels, stats, (hipStream_t)stream);
+  const bool vec = aligned16(d_u) && aligned16(d_v) && aligned16(d_w);
+  return mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w, vec}, wmode, d_ws,
+                        ws_bytes, d_mst_ids, d_labels, opts, stats,
+                        (hipStream_t)stream);
 }
 
 extern "C" int phem_mst_solve_aos(uint32_t n, uint64_t m,
                                   const uint32_t *d_edges, int wmode,
                                   void *d_ws, uint64_t ws_bytes,
                                   uint32_t *d_mst_ids, uint32_t *d_labels,
+                                  const phem_mst_opts *opts,
                                   phem_mst_stats *stats, void *stream) {
   if (m && !d_edges) return set_err(PHEM_EINVAL, "edge array is NULL");
-  return mst_solve_impl(n, m, SrcAoS{d_edges}, wmode, d_ws, ws_bytes, d_mst_ids,
-                        d_labels, stats, (hipStream_t)stream);
+  return mst_solve_impl(n, m, SrcAoS{d_edges, aligned16(d_edges)}, wmode, d_ws,
+                        ws_bytes, d_mst_ids, d_labels, opts, stats,
+                        (hipStream_t)stream);
 }

@@ -1,0 +1,113 @@
+// phem_split.h -- kernels shared by the MST and CC solves: a one-block
+// bitonic sort of a sample, flattening of the parent forest, and the bitmap of
+// the giant component (the component holding most sampled vertices).
+#pragma once
+#include "phem_dev.h"
+
+namespace phem {
+namespace {
+
+constexpr int SPLIT_BLK = 256;
+
+constexpr int NSAMP = 4096;  // threshold sample size
+constexpr int SBLK = 1024;   // one block sorts the sample
+
+// In-LDS bitonic sort of NSAMP keys by SBLK threads.
+__device__ __forceinline__ void bitonic_sort_lds(ull *s) {
+  for (int k = 2; k <= NSAMP; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < NSAMP; i += SBLK) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          ull x = s[i], y = s[ixj];
+          bool up = (i & k) == 0;
+          if ((x > y) == up) {
+            s[i] = y;
+            s[ixj] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// par[x] = root(x) for every vertex.
+__global__ __launch_bounds__(SPLIT_BLK) void k_flatten(uint32_t n, uint32_t *par) {
+  for (uint64_t i = (uint64_t)blockIdx.x * SPLIT_BLK + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * SPLIT_BLK) {
+    uint32_t r = uf_root(par, (uint32_t)i);
+    if (r != i) par[i] = r;
+  }
+}
+
+// The most frequent root among NSAMP strided vertices (after k_flatten).
+__global__ __launch_bounds__(SBLK) void k_pick_giant(uint32_t n,
+                                                     const uint32_t *par,
+                                                     uint32_t *giant_out) {
+  __shared__ ull s[NSAMP];
+  for (int j = threadIdx.x; j < NSAMP; j += SBLK)
+    s[j] = par[(uint64_t)j * n / NSAMP];
+  __syncthreads();
+  bitonic_sort_lds(s);
+  // every run start finds its run's end by binary search; the longest run
+  // (smallest root on ties) wins
+  ull best_len = 0, best_val = ~0ull;
+  for (int j = threadIdx.x; j < NSAMP; j += SBLK) {
+    if (j > 0 && s[j] == s[j - 1]) continue;
+    int lo = j, hi = NSAMP;  // first index > j with a different value
+    while (lo + 1 < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s[mid] == s[j]) lo = mid; else hi = mid;
+    }
+    const ull len = (ull)(hi - j);
+    if (len > best_len || (len == best_len && s[j] < best_val)) {
+      best_len = len;
+      best_val = s[j];
+    }
+  }
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const ull ol = __shfl_xor(best_len, d), ov = __shfl_xor(best_val, d);
+    if (ol > best_len || (ol == best_len && ov < best_val)) {
+      best_len = ol;
+      best_val = ov;
+    }
+  }
+  __shared__ ull s_len[SBLK / WAVE], s_val[SBLK / WAVE];
+  if (lane_id() == 0) {
+    s_len[threadIdx.x / WAVE] = best_len;
+    s_val[threadIdx.x / WAVE] = best_val;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ull bl = 0, bv = ~0ull;
+    for (int w = 0; w < SBLK / WAVE; w++)
+      if (s_len[w] > bl || (s_len[w] == bl && s_val[w] < bv)) {
+        bl = s_len[w];
+        bv = s_val[w];
+      }
+    *giant_out = (uint32_t)bv;
+  }
+}
+
+// bit x of `bits` = (par[x] == giant root); one ballot per 64 vertices.
+__global__ __launch_bounds__(SPLIT_BLK) void k_giant_bits(uint32_t n,
+                                                    const uint32_t *par,
+                                                    const uint32_t *giant,
+                                                    uint32_t *bits) {
+  const uint32_t g = *giant;
+  const uint64_t words = ((uint64_t)n + 31) / 32;
+  for (uint64_t base = (uint64_t)blockIdx.x * SPLIT_BLK; base < n;
+       base += (uint64_t)gridDim.x * SPLIT_BLK) {
+    const uint64_t i = base + threadIdx.x;
+    const uint64_t mask = __ballot(i < n && par[i] == g);
+    if (lane_id() == 0) {
+      const uint64_t w = i >> 5;  // i is a multiple of 64 for lane 0
+      if (w < words) bits[w] = (uint32_t)mask;
+      if (w + 1 < words) bits[w + 1] = (uint32_t)(mask >> 32);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace phem

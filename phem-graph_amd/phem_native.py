@@ -24,15 +24,24 @@ class PhemError(RuntimeError):
     pass
 
 
+class MstOpts(ctypes.Structure):
+    _fields_ = [("light_factor", ctypes.c_double), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
 class MstStats(ctypes.Structure):
     _fields_ = [
         ("rounds", ctypes.c_uint32),
         ("mst_count", ctypes.c_uint32),
         ("weight_fx", ctypes.c_uint64),
         ("weight_exact", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("light_rounds", ctypes.c_uint32),
         ("sum_m", ctypes.c_uint64),
         ("sum_c", ctypes.c_uint64),
+        ("valid_edges", ctypes.c_uint64),
+        ("tau", ctypes.c_uint64),
+        ("light_edges", ctypes.c_uint64),
+        ("heavy_edges", ctypes.c_uint64),
         ("m_log", ctypes.c_uint64 * MAX_ROUNDS),
         ("c_log", ctypes.c_uint64 * MAX_ROUNDS),
     ]
@@ -41,7 +50,10 @@ class MstStats(ctypes.Structure):
         r = self.rounds
         return dict(rounds=r, mst_count=self.mst_count, weight_fx=self.weight_fx,
                     weight_exact=bool(self.weight_exact), sum_m=self.sum_m,
-                    sum_c=self.sum_c, m_log=list(self.m_log[:r]),
+                    sum_c=self.sum_c, valid_edges=self.valid_edges, tau=self.tau,
+                    light_edges=self.light_edges, heavy_edges=self.heavy_edges,
+                    light_rounds=self.light_rounds,
+                    m_log=list(self.m_log[:r]),
                     c_log=list(self.c_log[:r]))
 
 
@@ -64,9 +76,9 @@ def lib():
         "phem_device_check": (i32, [c.POINTER(c.c_int)]),
         "phem_mst_workspace_bytes": (i32, [u32, u64, c.POINTER(c.c_uint64)]),
         "phem_mst_solve": (i32, [u32, u64, vp, vp, vp, i32, vp, u64, vp, vp,
-                                 c.POINTER(MstStats), vp]),
+                                 c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
         "phem_mst_solve_aos": (i32, [u32, u64, vp, i32, vp, u64, vp, vp,
-                                     c.POINTER(MstStats), vp]),
+                                     c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
         "phem_cc_init": (i32, [u32, vp, vp, vp]),
         "phem_cc_union": (i32, [u32, vp, u64, vp, vp, vp]),
         "phem_cc_labels": (i32, [u32, vp, vp, vp]),
@@ -135,35 +147,39 @@ class MstSolver:
         self.ids = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
         self.labels = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
 
-    def solve(self, u, v, w, wmode=W_F32, labels=True, stream=None):
+    def solve(self, u, v, w, wmode=W_F32, labels=True, stream=None, light_factor=None):
+        """light_factor: None = library default; <= 0 = canonical Boruvka."""
         m = u.numel()
         if m > self.m:
             raise PhemError(f"m={m} exceeds solver capacity {self.m}")
         st = MstStats()
+        opts = None if light_factor is None else ctypes.byref(MstOpts(light_factor, 0, 0))
         _check(lib().phem_mst_solve(self.n, m, _p(u), _p(v), _p(w), wmode, _p(self.ws),
                                     self.ws_bytes, _p(self.ids),
-                                    _p(self.labels) if labels else None,
+                                    _p(self.labels) if labels else None, opts,
                                     ctypes.byref(st), _stream(stream)), "phem_mst_solve")
         s = st.as_dict()
         return self.ids[: s["mst_count"]], (self.labels[: self.n] if labels else None), s
 
-    def solve_aos(self, edges, wmode=W_F32, labels=True, stream=None):
+    def solve_aos(self, edges, wmode=W_F32, labels=True, stream=None, light_factor=None):
         """edges: device tensor of m x 3 32-bit words {u, v, w}."""
         m = edges.numel() // 3
         if m > self.m:
             raise PhemError(f"m={m} exceeds solver capacity {self.m}")
         st = MstStats()
+        opts = None if light_factor is None else ctypes.byref(MstOpts(light_factor, 0, 0))
         _check(lib().phem_mst_solve_aos(self.n, m, _p(edges), wmode, _p(self.ws),
                                         self.ws_bytes, _p(self.ids),
-                                        _p(self.labels) if labels else None,
+                                        _p(self.labels) if labels else None, opts,
                                         ctypes.byref(st), _stream(stream)),
                "phem_mst_solve_aos")
         s = st.as_dict()
         return self.ids[: s["mst_count"]], (self.labels[: self.n] if labels else None), s
 
 
-def mst_solve(n, u, v, w, wmode=W_F32, labels=True):
-    return MstSolver(n, u.numel(), device=u.device).solve(u, v, w, wmode, labels)
+def mst_solve(n, u, v, w, wmode=W_F32, labels=True, light_factor=None):
+    return MstSolver(n, u.numel(), device=u.device).solve(u, v, w, wmode, labels,
+                                                          light_factor=light_factor)
 
 
 def cc_solve(n, edges, forest=True, stream=None):

@@ -28,17 +28,20 @@ def host(t):
     return t.cpu().numpy().view(np.uint32)
 
 
-def run_mst(P, n, u, v, w, wmode):
-    ids, lab, st = P.mst_solve(n, dev(u), dev(v), dev(w), wmode)
+def run_mst(P, n, u, v, w, wmode, light_factor=None):
+    ids, lab, st = P.mst_solve(n, dev(u), dev(v), dev(w), wmode, light_factor=light_factor)
     torch.cuda.synchronize()
     return host(ids), host(lab), st
 
 
 @pytest.mark.parametrize("name", list(cases.CASES))
-def test_mst_matches_oracle(P, name):
+def test_mst_canonical_rounds_match_oracle(P, name):
+    """Canonical Boruvka (no light/heavy split): forest, labels, weight and
+    the per-round (alive edges, active components) log equal the oracle's
+    restatement of the reference round loop."""
     n, u, v, w, wmode = cases.make(name)
     print(f"case {name}: n={n} m={len(u)}", flush=True)
-    ids, lab, st = run_mst(P, n, u, v, w, wmode)
+    ids, lab, st = run_mst(P, n, u, v, w, wmode, light_factor=0.0)
     k_ids, k_lab = O.kruskal(n, u, v, w, wmode)
     assert np.array_equal(ids, k_ids)
     assert np.array_equal(lab, k_lab)
@@ -46,6 +49,22 @@ def test_mst_matches_oracle(P, name):
     assert st["rounds"] == b["rounds"]
     assert st["m_log"] == b["log_m"].tolist()
     assert st["c_log"] == b["log_c"].tolist()
+    ws, exact = O.weight_sum(k_ids, w, wmode)
+    assert st["weight_exact"] == exact
+    if exact:
+        assert st["weight_fx"] == ws
+
+
+@pytest.mark.parametrize("lf", [None, 0.25, 1.0, 4.0])
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_mst_split_matches_oracle(P, name, lf):
+    """Light/heavy split (default path): same forest, labels and weight."""
+    n, u, v, w, wmode = cases.make(name)
+    print(f"case {name}: n={n} m={len(u)} light_factor={lf}", flush=True)
+    ids, lab, st = run_mst(P, n, u, v, w, wmode, light_factor=lf)
+    k_ids, k_lab = O.kruskal(n, u, v, w, wmode)
+    assert np.array_equal(ids, k_ids)
+    assert np.array_equal(lab, k_lab)
     ws, exact = O.weight_sum(k_ids, w, wmode)
     assert st["weight_exact"] == exact
     if exact:
