@@ -99,7 +99,7 @@ typedef struct phem_mst_stats {
  *   the rest (filter-Kruskal, as the CPU path does, MST/src/utils.cu:794-841).
  *   <= 0 runs the canonical Boruvka round loop over all edges
  *   (MST/src/kernels.cu:507-689); its per-round log is then the reference's.
- *   Default 2.0, or the environment variable PHEM_MST_LIGHT_FACTOR. */
+ *   Default 1.0, or the environment variable PHEM_MST_LIGHT_FACTOR. */
 typedef struct phem_mst_opts {
   double light_factor;
   uint32_t flags; /* reserved, 0 */

@@ -33,7 +33,7 @@ unsigned resident_blocks() {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
           hipSuccess || cus <= 0)
     cus = 256;
-  unsigned per_cu = 8;
+  unsigned per_cu = 16;
   if (const char *e = getenv("PHEM_BLOCKS_PER_CU")) {
     const int x = atoi(e);
     if (x > 0 && x <= 64) per_cu = (unsigned)x;

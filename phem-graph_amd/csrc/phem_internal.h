@@ -14,8 +14,8 @@ int set_err(int code, const char *fmt, ...);
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 inline uint64_t align_up(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
 
-// Blocks kept resident on the current device for the library's looping
-// kernels: compute units x 8 (256-thread blocks, 32 waves per CU), or x the
+// Blocks launched for the library's looping kernels: compute units x 16
+// (256-thread blocks; the dispatcher backfills as blocks finish), or x the
 // environment variable PHEM_BLOCKS_PER_CU.
 unsigned resident_blocks();
 

@@ -650,7 +650,7 @@ namespace phem {
 double mst_default_light_factor() {
   static double f = [] {
     const char *e = getenv("PHEM_MST_LIGHT_FACTOR");
-    return e ? atof(e) : 2.0;
+    return e ? atof(e) : 1.0;
   }();
   return f;
 }
