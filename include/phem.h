@@ -129,6 +129,12 @@ int phem_mst_solve_aos(uint32_t n, uint64_t m, const uint32_t *d_edges,
                        const phem_mst_opts *opts, phem_mst_stats *stats,
                        void *stream);
 
+/* d_out[k] = record d_ids[k] of the 3-word {u, v, w} records in d_src, for
+ * k < count: materialises forest edges as the reference's `edge` records
+ * (the MSF copy-out of device_boruvka, MST/src/kernels.cu:590-596). */
+int phem_gather_edges_aos(const uint32_t *d_src, const uint32_t *d_ids,
+                          uint64_t count, uint32_t *d_out, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Union-find connected components / spanning forest
  *

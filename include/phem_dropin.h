@@ -1,0 +1,143 @@
+/*
+ * phem_dropin.h -- C++ entry points with the reference's names, argument
+ * meaning and error behaviour, implemented over the C ABI of phem.h, so that
+ * the reference's callers (its main programs and loaders) link against this
+ * library in place of its CUDA sources:
+ *
+ *   MST/src/kernels.cu, MST/src/new_boruvka.cu   -> boruvka_for_edge_array,
+ *                                                   device_boruvka
+ *   CC/ExternalSpanningTree.cu                    -> SpanningTree
+ *   CC/GPUSpanningTree.cu                         -> gpu_spanning_tree
+ *
+ * The CPU path (CC/host_spanning_tree.cu with its ParlayLib) stays the
+ * caller's: SpanningTree calls the caller's host_spanning_tree() on the CPU
+ * share of the edges while the GPU batches run.  As in the reference, a HIP
+ * failure in these entry points prints a message and exits the process
+ * (CC/include/cuda_utility.cuh:7-17); the C ABI itself never exits.
+ */
+#ifndef PHEM_DROPIN_H
+#define PHEM_DROPIN_H
+
+#include <hip/hip_runtime.h>
+#include <thrust/device_ptr.h>
+
+#include <cstdint>
+#include <limits>
+#include <string>
+
+/* ---- MST types (MST/src/constants.h:8-24) ------------------------------- */
+typedef uint32_t vertex;
+typedef uint32_t weight;
+typedef uint64_t large_vertex;
+const weight sentinel = std::numeric_limits<weight>::max();
+const vertex sentinel_vertex = std::numeric_limits<vertex>::max();
+struct edge {
+  vertex u;
+  vertex v;
+  weight w;
+};
+struct outgoing_edge_id {
+  vertex e_id;
+  weight w;
+};
+
+/* Minimum spanning forest of host edges (MST/src/new_boruvka.cu:40-179).
+ * Weights are unsigned integers (the reference's `weight`).  Writes the
+ * forest's edges (u, v, w) in ascending input order to
+ * final_msf[number_of_filled_edges ...] (the caller provides num_nodes
+ * slots) and adds their count to number_of_filled_edges.  `result` is
+ * accepted and unused, as in the reference's plain path. */
+void boruvka_for_edge_array(edge *edges, vertex num_nodes, vertex num_edges,
+                            edge *final_msf, vertex &number_of_filled_edges,
+                            std::string result = "res.txt");
+
+/* Minimum spanning forest of device edges (MST/src/kernels.cu:507-689).
+ * Reads d_edges[0 .. num_edges); appends the forest's edges to
+ * d_final_msf + number_of_filled_edges; writes min-id component labels to
+ * d_representatives; sets num_edges to 0 (every edge consumed).  The other
+ * buffers (d_iter_edges, d_msf, d_new_representatives,
+ * d_iter_representatives, d_outgoing_edges) are accepted for signature
+ * compatibility and left untouched; the solver's workspace is allocated and
+ * cached by the library. */
+void device_boruvka(edge *d_edges, edge *d_final_msf, edge *d_iter_edges,
+                    edge *d_msf, vertex *d_representatives,
+                    vertex *d_new_representatives,
+                    vertex *d_iter_representatives,
+                    outgoing_edge_id *d_outgoing_edges, vertex &num_nodes,
+                    vertex &num_edges, vertex &number_of_filled_edges,
+                    hipStream_t kernel_stream = 0);
+
+/* The reference's full parameter list (its four thrust::device_ptr views of
+ * the buffers above are ignored). */
+void device_boruvka(edge *d_edges, edge *d_final_msf, edge *d_iter_edges,
+                    edge *d_msf, vertex *d_representatives,
+                    vertex *d_new_representatives,
+                    vertex *d_iter_representatives,
+                    outgoing_edge_id *d_outgoing_edges, vertex &num_nodes,
+                    vertex &num_edges, vertex &number_of_filled_edges,
+                    thrust::device_ptr<vertex> d_new_representatives_ptr,
+                    thrust::device_ptr<vertex> d_representatives_ptr,
+                    thrust::device_ptr<vertex> d_iter_representatives_ptr,
+                    thrust::device_ptr<edge> d_final_msf_ptr,
+                    hipStream_t kernel_stream = 0);
+
+/* ---- CC types (CC/include/hbcg_utils.cuh:4-42) -------------------------- */
+struct graph_data_host; /* the caller's (ParlayLib sequences) */
+struct graph_data {
+  uint64_t *edges;
+  uint64_t *edges2;
+  int *temp_label;
+  int *label;
+  int *V;
+  long *E;
+  long *size;
+  long *size2;
+  uint64_t *T1edges;
+  uint64_t *T2edges;
+  int *first_occ;
+  int *last_occ;
+  int *parent;
+  int *w1;
+  int *w2;
+  int *w1_copy;
+  int *w2_copy;
+  uint64_t *SpanningForest;
+  int *low;
+  int *high;
+  bool *fg1;
+};
+
+/* Batch size and GPU fraction of the edges (CC/ExternalSpanningTree.cu:7-8);
+ * defined by the caller's main program (the library holds weak defaults:
+ * 2^20 and 1.0). */
+extern int Batch_Size;
+extern float GPU_share;
+
+/* The caller's CPU union-find (CC/host_spanning_tree.cu:89-92), found by
+ * symbol when the caller's program defines it. */
+int host_spanning_tree(struct graph_data_host *);
+
+/* Registers the CPU union-find explicitly (no counterpart in the reference;
+ * for callers that load this library and their CPU path as separate shared
+ * objects, where the symbol above is not visible to the library).  NULL
+ * restores the symbol lookup. */
+extern "C" void phem_set_host_spanning_tree(
+    int (*fn)(struct graph_data_host *));
+
+/* Batched union-find over the GPU share of the edges, double-buffered on two
+ * streams, overlapped with host_spanning_tree on the CPU share
+ * (CC/ExternalSpanningTree.cu:133-218).  h_input->edges: host records
+ * (pinned for overlap); d_input: caller-allocated device buffers (edges and
+ * edges2 of Batch_Size records; temp_label = parent, label, T1edges of V).
+ * Returns the elapsed milliseconds.  Unhooked T1edges slots hold all ones. */
+float SpanningTree(struct graph_data *h_input, struct graph_data *d_input,
+                   struct graph_data_host *h_input_host);
+
+/* Merges the GPU forest d_input->T1edges with the vert candidate edges in
+ * extraedges (the CPU forest; all ones or INT_MAX mark empty slots) into
+ * d_input->T2edges, leaving labels (minimum vertex ids) in d_input->label
+ * (CC/GPUSpanningTree.cu:152-198).  Returns the elapsed milliseconds. */
+float gpu_spanning_tree(struct graph_data *d_input, uint64_t *extraedges,
+                        int vert);
+
+#endif /* PHEM_DROPIN_H */

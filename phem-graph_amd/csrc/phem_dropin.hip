@@ -1,0 +1,272 @@
+// phem_dropin.hip -- the reference's C++ entry points over the C ABI.
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+
+#include "phem.h"
+#include "phem_dev.h"
+#include "phem_dropin.h"
+#include "phem_internal.h"
+
+namespace phem {
+
+constexpr int DBLK = 256;
+
+// out[k] = record ids[k] of the 3-word records in src
+__global__ __launch_bounds__(DBLK) void k_gather_aos(
+    const uint32_t *__restrict__ src, const uint32_t *__restrict__ ids,
+    uint64_t count, uint32_t *__restrict__ out) {
+  for (uint64_t k = (uint64_t)blockIdx.x * DBLK + threadIdx.x; k < count;
+       k += (uint64_t)gridDim.x * DBLK) {
+    const uint64_t e = ids[k];
+    out[3 * k] = src[3 * e];
+    out[3 * k + 1] = src[3 * e + 1];
+    out[3 * k + 2] = src[3 * e + 2];
+  }
+}
+
+namespace {
+
+// The reference's error convention for these entry points
+// (CC/include/cuda_utility.cuh:7-17): report and exit.
+void die_hip(hipError_t e, const char *what) {
+  if (e == hipSuccess) return;
+  std::fprintf(stderr, "HIP error in %s: %s\n", what, hipGetErrorString(e));
+  std::exit(EXIT_FAILURE);
+}
+void die_phem(int rc, const char *what) {
+  if (rc == PHEM_OK) return;
+  std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, phem_last_error());
+  std::exit(EXIT_FAILURE);
+}
+
+// Grow-only solver workspace per device, kept for the life of the process.
+struct CachedWs {
+  void *p = nullptr;
+  size_t cap = 0;
+};
+std::mutex g_ws_mu;
+CachedWs g_ws[64];
+
+void *cached_workspace(size_t bytes) {
+  int dev = 0;
+  die_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= 64) dev = 0;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  CachedWs &w = g_ws[dev];
+  if (w.cap < bytes) {
+    if (w.p) die_hip(hipFree(w.p), "hipFree");
+    w.p = nullptr;
+    die_hip(hipMalloc(&w.p, bytes), "hipMalloc (MST workspace)");
+    w.cap = bytes;
+  }
+  return w.p;
+}
+
+}  // namespace
+}  // namespace phem
+
+using namespace phem;
+
+extern "C" int phem_gather_edges_aos(const uint32_t *d_src, const uint32_t *d_ids,
+                                     uint64_t count, uint32_t *d_out,
+                                     void *stream) {
+  if (count && (!d_src || !d_ids || !d_out))
+    return set_err(PHEM_EINVAL, "NULL array");
+  if (!count) return PHEM_OK;
+  {
+    PHEM_PROF("k_gather_aos", (hipStream_t)stream);
+    hipLaunchKernelGGL(k_gather_aos, dim3(grid_for(count, DBLK)), dim3(DBLK),
+                       0, (hipStream_t)stream, d_src, d_ids, count, d_out);
+  }
+  PHEM_LAUNCHED("k_gather_aos");
+  return PHEM_OK;
+}
+
+// ---- MST --------------------------------------------------------------------
+
+void device_boruvka(edge *d_edges, edge *d_final_msf, edge * /*d_iter_edges*/,
+                    edge * /*d_msf*/, vertex *d_representatives,
+                    vertex * /*d_new_representatives*/,
+                    vertex * /*d_iter_representatives*/,
+                    outgoing_edge_id * /*d_outgoing_edges*/, vertex &num_nodes,
+                    vertex &num_edges, vertex &number_of_filled_edges,
+                    hipStream_t kernel_stream) {
+  const uint32_t n = num_nodes;
+  const uint64_t m = num_edges;
+  uint64_t wsb = 0;
+  die_phem(phem_mst_workspace_bytes(n, m, &wsb), "phem_mst_workspace_bytes");
+  const uint64_t ids_off = align_up(wsb, 256);
+  char *ws = (char *)cached_workspace(ids_off + 4ull * (n ? n : 1));
+  uint32_t *ids = (uint32_t *)(ws + ids_off);
+  phem_mst_stats st;
+  die_phem(phem_mst_solve_aos(n, m, (const uint32_t *)d_edges, PHEM_W_U32, ws,
+                              wsb, ids, d_representatives, nullptr, &st,
+                              kernel_stream),
+           "phem_mst_solve_aos");
+  if ((uint64_t)number_of_filled_edges + st.mst_count > (uint64_t)n) {
+    std::fprintf(stderr,
+                 "device_boruvka: %u forest edges do not fit after %u filled "
+                 "slots of %u\n",
+                 st.mst_count, number_of_filled_edges, n);
+    std::exit(EXIT_FAILURE);
+  }
+  die_phem(phem_gather_edges_aos((const uint32_t *)d_edges, ids, st.mst_count,
+                                 (uint32_t *)(d_final_msf + number_of_filled_edges),
+                                 kernel_stream),
+           "phem_gather_edges_aos");
+  die_hip(hipStreamSynchronize(kernel_stream), "hipStreamSynchronize");
+  number_of_filled_edges += st.mst_count;
+  num_edges = 0;
+}
+
+void device_boruvka(edge *d_edges, edge *d_final_msf, edge *d_iter_edges,
+                    edge *d_msf, vertex *d_representatives,
+                    vertex *d_new_representatives,
+                    vertex *d_iter_representatives,
+                    outgoing_edge_id *d_outgoing_edges, vertex &num_nodes,
+                    vertex &num_edges, vertex &number_of_filled_edges,
+                    thrust::device_ptr<vertex>, thrust::device_ptr<vertex>,
+                    thrust::device_ptr<vertex>, thrust::device_ptr<edge>,
+                    hipStream_t kernel_stream) {
+  device_boruvka(d_edges, d_final_msf, d_iter_edges, d_msf, d_representatives,
+                 d_new_representatives, d_iter_representatives,
+                 d_outgoing_edges, num_nodes, num_edges, number_of_filled_edges,
+                 kernel_stream);
+}
+
+void boruvka_for_edge_array(edge *edges, vertex num_nodes, vertex num_edges,
+                            edge *final_msf, vertex &number_of_filled_edges,
+                            std::string /*result*/) {
+  const uint32_t n = num_nodes;
+  edge *d_edges = nullptr, *d_out = nullptr;
+  vertex *d_rep = nullptr;
+  die_hip(hipMalloc(&d_edges, sizeof(edge) * (num_edges ? num_edges : 1)),
+          "hipMalloc");
+  die_hip(hipMalloc(&d_out, sizeof(edge) * (n ? n : 1)), "hipMalloc");
+  die_hip(hipMalloc(&d_rep, sizeof(vertex) * (n ? n : 1)), "hipMalloc");
+  if (num_edges)
+    die_hip(hipMemcpy(d_edges, edges, sizeof(edge) * num_edges,
+                      hipMemcpyHostToDevice),
+            "hipMemcpy H2D");
+  vertex nn = num_nodes, mm = num_edges, filled = 0;
+  device_boruvka(d_edges, d_out, nullptr, nullptr, d_rep, nullptr, nullptr,
+                 nullptr, nn, mm, filled, 0);
+  if ((uint64_t)number_of_filled_edges + filled > (uint64_t)n) {
+    std::fprintf(stderr, "boruvka_for_edge_array: forest does not fit\n");
+    std::exit(EXIT_FAILURE);
+  }
+  if (filled)
+    die_hip(hipMemcpy(final_msf + number_of_filled_edges, d_out,
+                      sizeof(edge) * filled, hipMemcpyDeviceToHost),
+            "hipMemcpy D2H");
+  number_of_filled_edges += filled;
+  die_hip(hipFree(d_edges), "hipFree");
+  die_hip(hipFree(d_out), "hipFree");
+  die_hip(hipFree(d_rep), "hipFree");
+}
+
+// ---- CC ---------------------------------------------------------------------
+
+// Defaults for programs that do not define the reference's globals; the
+// caller's definitions take precedence.
+__attribute__((weak)) int Batch_Size = 1 << 20;
+__attribute__((weak)) float GPU_share = 1.0f;
+// The caller's CPU path; absent when the whole graph goes to the GPU.
+__attribute__((weak)) int host_spanning_tree(struct graph_data_host *);
+
+static int (*g_host_spanning_tree)(struct graph_data_host *) = nullptr;
+
+extern "C" void phem_set_host_spanning_tree(
+    int (*fn)(struct graph_data_host *)) {
+  g_host_spanning_tree = fn;
+}
+
+float SpanningTree(struct graph_data *h_input, struct graph_data *d_input,
+                   struct graph_data_host *h_input_host) {
+  const uint32_t V = (uint32_t)h_input->V[0];
+  const long E = h_input->E[0];
+  uint32_t *parent = (uint32_t *)d_input->temp_label;
+  die_phem(phem_cc_init(V, parent, d_input->T1edges, nullptr), "phem_cc_init");
+  die_phem(phem_cc_labels(V, parent, (uint32_t *)d_input->label, nullptr),
+           "phem_cc_labels");
+  die_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+
+  hipStream_t s1, s2;
+  die_hip(hipStreamCreate(&s1), "hipStreamCreate");
+  die_hip(hipStreamCreate(&s2), "hipStreamCreate");
+  hipEvent_t start, stop, d1, d2;
+  die_hip(hipEventCreate(&start), "hipEventCreate");
+  die_hip(hipEventCreate(&stop), "hipEventCreate");
+  die_hip(hipEventCreate(&d1), "hipEventCreate");
+  die_hip(hipEventCreate(&d2), "hipEventCreate");
+
+  const long batch = Batch_Size > 0 ? Batch_Size : 1;
+  // GPU share of the records, computed in double precision (a float product
+  // loses low bits above 2^24 records)
+  long edges_for_gpu = GPU_share >= 1.0f ? E : (long)((double)GPU_share * (double)E);
+  if (edges_for_gpu < 0) edges_for_gpu = 0;
+  if (edges_for_gpu > E) edges_for_gpu = E;
+
+  die_hip(hipEventRecord(start, nullptr), "hipEventRecord");
+  long k = 0;
+  for (long i = 0; i < edges_for_gpu; i += batch, k++) {
+    const long len = edges_for_gpu - i < batch ? edges_for_gpu - i : batch;
+    // alternate buffers and streams: batch k + 2 reuses batch k's buffer
+    // after it on the same stream
+    uint64_t *buf = (k & 1) ? d_input->edges2 : d_input->edges;
+    hipStream_t s = (k & 1) ? s2 : s1;
+    die_hip(hipMemcpyAsync(buf, h_input->edges + i, sizeof(uint64_t) * len,
+                           hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync");
+    die_phem(phem_cc_union(V, buf, (uint64_t)len, parent, d_input->T1edges, s),
+             "phem_cc_union");
+  }
+  if (edges_for_gpu < E) {
+    int (*cpu)(struct graph_data_host *) =
+        g_host_spanning_tree ? g_host_spanning_tree : host_spanning_tree;
+    if (!cpu) {
+      std::fprintf(stderr,
+                   "SpanningTree: GPU_share < 1 but no host_spanning_tree is "
+                   "linked\n");
+      std::exit(EXIT_FAILURE);
+    }
+    cpu(h_input_host);
+  }
+  die_hip(hipEventRecord(d1, s1), "hipEventRecord");
+  die_hip(hipEventRecord(d2, s2), "hipEventRecord");
+  die_hip(hipStreamWaitEvent(nullptr, d1, 0), "hipStreamWaitEvent");
+  die_hip(hipStreamWaitEvent(nullptr, d2, 0), "hipStreamWaitEvent");
+  die_hip(hipEventRecord(stop, nullptr), "hipEventRecord");
+  die_hip(hipEventSynchronize(stop), "hipEventSynchronize");
+  float ms = 0.f;
+  die_hip(hipEventElapsedTime(&ms, start, stop), "hipEventElapsedTime");
+  die_hip(hipStreamDestroy(s1), "hipStreamDestroy");
+  die_hip(hipStreamDestroy(s2), "hipStreamDestroy");
+  die_hip(hipEventDestroy(start), "hipEventDestroy");
+  die_hip(hipEventDestroy(stop), "hipEventDestroy");
+  die_hip(hipEventDestroy(d1), "hipEventDestroy");
+  die_hip(hipEventDestroy(d2), "hipEventDestroy");
+  return ms;
+}
+
+float gpu_spanning_tree(struct graph_data *d_input, uint64_t *extraedges,
+                        int vert) {
+  hipEvent_t start, stop;
+  die_hip(hipEventCreate(&start), "hipEventCreate");
+  die_hip(hipEventCreate(&stop), "hipEventCreate");
+  die_hip(hipEventRecord(start, nullptr), "hipEventRecord");
+  die_phem(phem_cc_merge_forests((uint32_t)vert, d_input->T1edges, extraedges,
+                                 (uint32_t *)d_input->temp_label,
+                                 d_input->T2edges, (uint32_t *)d_input->label,
+                                 nullptr),
+           "phem_cc_merge_forests");
+  die_hip(hipEventRecord(stop, nullptr), "hipEventRecord");
+  die_hip(hipEventSynchronize(stop), "hipEventSynchronize");
+  float ms = 0.f;
+  die_hip(hipEventElapsedTime(&ms, start, stop), "hipEventElapsedTime");
+  die_hip(hipEventDestroy(start), "hipEventDestroy");
+  die_hip(hipEventDestroy(stop), "hipEventDestroy");
+  return ms;
+}

@@ -1,0 +1,45 @@
+// Test harness: C entry points around the C++ drop-in MST functions of
+// libphem.so (boruvka_for_edge_array, device_boruvka), for ctypes tests.
+#include <cstring>
+#include <thrust/device_ptr.h>
+
+#include "phem_dropin.h"
+
+extern "C" int harness_boruvka_for_edge_array(const uint32_t *h_aos, uint32_t n,
+                                              uint32_t m, uint32_t *out_aos,
+                                              uint32_t *filled_io) {
+  vertex filled = *filled_io;
+  boruvka_for_edge_array((edge *)h_aos, n, m, (edge *)out_aos, filled);
+  *filled_io = filled;
+  return 0;
+}
+
+// device_boruvka with the reference's full parameter list on device copies.
+extern "C" int harness_device_boruvka(const uint32_t *h_aos, uint32_t n,
+                                      uint32_t m, uint32_t *out_aos,
+                                      uint32_t *labels, uint32_t *filled_io,
+                                      uint32_t *m_after) {
+  edge *d_edges, *d_final, *d_iter, *d_msf;
+  vertex *d_rep, *d_new, *d_it;
+  outgoing_edge_id *d_out;
+  const size_t nn = n ? n : 1, mm = m ? m : 1;
+  if (hipMalloc(&d_edges, sizeof(edge) * mm) || hipMalloc(&d_final, sizeof(edge) * nn) ||
+      hipMalloc(&d_iter, sizeof(edge) * mm) || hipMalloc(&d_msf, sizeof(edge) * nn) ||
+      hipMalloc(&d_rep, 4 * nn) || hipMalloc(&d_new, 4 * nn) || hipMalloc(&d_it, 4 * nn) ||
+      hipMalloc(&d_out, 8 * nn))
+    return -1;
+  if (m && hipMemcpy(d_edges, h_aos, sizeof(edge) * m, hipMemcpyHostToDevice)) return -1;
+  vertex vn = n, vm = m, filled = 0;
+  device_boruvka(d_edges, d_final, d_iter, d_msf, d_rep, d_new, d_it, d_out, vn, vm,
+                 filled, thrust::device_ptr<vertex>(d_new),
+                 thrust::device_ptr<vertex>(d_rep), thrust::device_ptr<vertex>(d_it),
+                 thrust::device_ptr<edge>(d_final), 0);
+  if (filled && hipMemcpy(out_aos, d_final, sizeof(edge) * filled, hipMemcpyDeviceToHost))
+    return -1;
+  if (n && hipMemcpy(labels, d_rep, 4 * n, hipMemcpyDeviceToHost)) return -1;
+  *filled_io = filled;
+  *m_after = vm;
+  (void)hipFree(d_edges); (void)hipFree(d_final); (void)hipFree(d_iter); (void)hipFree(d_msf);
+  (void)hipFree(d_rep); (void)hipFree(d_new); (void)hipFree(d_it); (void)hipFree(d_out);
+  return 0;
+}

@@ -1,0 +1,123 @@
+"""GPU tests of the C++ drop-in entry points (reference names and argument
+meaning) in libphem.so:
+
+- boruvka_for_edge_array / device_boruvka (through tests/harness) against the
+  oracle's Kruskal forest, as the reference's `edge {u, v, w}` records;
+- the reference's CC driver flow (its headers and its ParlayLib CPU path,
+  CC/host_spanning_tree.cu, built in oracle/_ref) calling this library's
+  SpanningTree and gpu_spanning_tree, for several GPU shares and batch sizes.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(ROOT, "tests", "harness", "libdropin_mst.so")
+HAST = os.path.join(ROOT, "oracle", "_ref", "libref_hast.so")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+@pytest.fixture(scope="module")
+def H():
+    import phem_native
+    phem_native.require_device()
+    if not os.path.exists(HARNESS):
+        pytest.fail(f"drop-in harness not built: {HARNESS}")
+    L = ctypes.CDLL(HARNESS)
+    L.harness_boruvka_for_edge_array.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                                 _u32p, _u32p]
+    L.harness_device_boruvka.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p,
+                                         _u32p, _u32p, _u32p]
+    return L
+
+
+MST_CASES = ["u32_weights", "rmat_small", "grid_small", "duplicates_equal_w",
+             "path_chords_small", "many_components", "out_of_range", "empty"]
+
+
+def expected_records(n, u, v, w, wmode):
+    ids, lab = O.kruskal(n, u, v, w, wmode)
+    rec = np.stack([u[ids], v[ids], w[ids]], axis=1).astype(np.uint32)
+    return ids, rec, lab
+
+
+@pytest.mark.parametrize("name", MST_CASES)
+def test_boruvka_for_edge_array(H, name):
+    n, u, v, w, wmode = cases.make(name)
+    # the reference's weights are unsigned integers; non-negative float bit
+    # patterns order the same way as unsigned integers
+    if wmode == 1 and (w & np.uint32(0x80000000)).any():
+        pytest.skip("negative floats are not unsigned weights")
+    aos = np.ascontiguousarray(np.stack([u, v, w], axis=1).reshape(-1), np.uint32)
+    out = np.zeros(3 * max(n, 1), np.uint32)
+    filled = np.array([0], np.uint32)
+    assert H.harness_boruvka_for_edge_array(aos, n, len(u), out, filled) == 0
+    ids, rec, _ = expected_records(n, u, v, w, 0)
+    assert filled[0] == len(ids)
+    assert np.array_equal(out[: 3 * len(ids)].reshape(-1, 3), rec)
+
+
+@pytest.mark.parametrize("name", MST_CASES)
+def test_device_boruvka_full_signature(H, name):
+    n, u, v, w, wmode = cases.make(name)
+    if wmode == 1 and (w & np.uint32(0x80000000)).any():
+        pytest.skip("negative floats are not unsigned weights")
+    aos = np.ascontiguousarray(np.stack([u, v, w], axis=1).reshape(-1), np.uint32)
+    out = np.zeros(3 * max(n, 1), np.uint32)
+    labels = np.zeros(max(n, 1), np.uint32)
+    filled = np.array([0], np.uint32)
+    m_after = np.array([7], np.uint32)
+    assert H.harness_device_boruvka(aos, n, len(u), out, labels, filled, m_after) == 0
+    ids, rec, lab = expected_records(n, u, v, w, 0)
+    assert filled[0] == len(ids) and m_after[0] == 0
+    assert np.array_equal(out[: 3 * len(ids)].reshape(-1, 3), rec)
+    assert np.array_equal(labels[:n], lab)
+
+
+@pytest.fixture(scope="module")
+def HAST_LIB():
+    import phem_native
+    phem_native.require_device()
+    if not os.path.exists(HAST):
+        pytest.skip("oracle/_ref/libref_hast.so not built (needs the reference tree)")
+    L = ctypes.CDLL(HAST)
+    L.ref_hast_run.restype = ctypes.c_int
+    L.ref_hast_run.argtypes = [ctypes.c_int, ctypes.c_long, _u64p, ctypes.c_float,
+                               ctypes.c_int, _u32p, _u64p, _u64p,
+                               ctypes.POINTER(ctypes.c_float)]
+    return L
+
+
+@pytest.mark.parametrize("share,batch", [(1.0, 1 << 20), (1.0, 4096), (0.6, 8192),
+                                         (0.25, 1 << 20)])
+@pytest.mark.parametrize("name", ["uniform_c1", "rmat_small", "grid_small", "many_components"])
+def test_reference_cc_driver_over_dropin(HAST_LIB, name, share, batch):
+    n, u, v, w, wmode = cases.make(name)
+    rec = ((np.minimum(u, v).astype(np.uint64) << np.uint64(32))
+           | np.maximum(u, v).astype(np.uint64))
+    rec = np.ascontiguousarray(rec)
+    labels = np.zeros(n, np.uint32)
+    t1 = np.zeros(n, np.uint64)
+    t2 = np.zeros(n, np.uint64)
+    ms = ctypes.c_float(0)
+    print(f"case {name} share={share} batch={batch}", flush=True)
+    assert HAST_LIB.ref_hast_run(n, len(rec), rec, share, batch, labels, t1, t2,
+                                 ctypes.byref(ms)) == 0
+    want = O.bfs_labels(n, u, v)
+    assert np.array_equal(labels, want)
+    hooked = t2 != np.uint64(0xFFFFFFFFFFFFFFFF)
+    assert int(hooked.sum()) == n - len(np.unique(want))
+    # the merged forest's records are input edges that span the components
+    f = t2[hooked]
+    fu = (f >> np.uint64(32)).astype(np.uint32)
+    fv = (f & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    assert np.array_equal(O.bfs_labels(n, fu, fv), want)
+    assert np.isin(f, rec).all()
