@@ -174,6 +174,70 @@ int phem_cc_merge_forests(uint32_t n, const uint64_t *d_forest,
                           void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Streaming spanning forest / connected components with device-resident
+ * state (the reference's chunked MSF, MSF(chunk u previous MSF),
+ * MST/src/kernels.cu:942-1102, and batch-by-batch union-find,
+ * CC/ExternalSpanningTree.cu:170-204).  After each batch the forest is the
+ * minimum spanning forest of every edge inserted so far, ordered by
+ * (weight, global id) with global ids in insertion order; labels are the
+ * minimum vertex id of each component.  The handle owns its device memory
+ * (forest and parent array stay in HBM between batches).
+ * ------------------------------------------------------------------------- */
+
+typedef struct phem_stream phem_stream_t;
+
+/* Device memory for n vertices and batches of up to max_batch edges
+ * (n + max_batch <= PHEM_MAX_EDGES), on the current device. */
+int phem_stream_create(uint32_t n, uint64_t max_batch, int wmode,
+                       phem_stream_t **out);
+int phem_stream_destroy(phem_stream_t *s);
+/* Inserts mb edges (device arrays); returns once forest and parent array are
+ * updated. */
+int phem_stream_apply(phem_stream_t *s, const uint32_t *d_u,
+                      const uint32_t *d_v, const uint32_t *d_w, uint64_t mb,
+                      void *stream);
+/* Device pointers to the current forest: *count edges (u, v, w, global id),
+ * ascending global id.  Valid until the next phem_stream_apply. */
+int phem_stream_forest(const phem_stream_t *s, uint64_t *count,
+                       const uint32_t **d_u, const uint32_t **d_v,
+                       const uint32_t **d_w, const uint64_t **d_gid);
+/* Copies the current forest into caller buffers (NULL entries skipped). */
+int phem_stream_copy_forest(const phem_stream_t *s, uint32_t *d_u,
+                            uint32_t *d_v, uint32_t *d_w, uint64_t *d_gid,
+                            void *stream);
+/* Statistics of the last solve (its weight_fx is the forest's total weight),
+ * edges inserted and batches applied so far. */
+int phem_stream_stats(const phem_stream_t *s, phem_mst_stats *stats,
+                      uint64_t *edges_inserted, uint64_t *batches);
+/* Connected-component labels of every edge inserted so far. */
+int phem_stream_labels(const phem_stream_t *s, uint32_t *d_labels,
+                       void *stream);
+/* Union-find spanning forest (packed hooking records, n slots). */
+int phem_stream_cc_forest(const phem_stream_t *s, const uint64_t **d_forest);
+
+/* ---------------------------------------------------------------------------
+ * Batched solve: k independent graphs in one pass (one disjoint union).
+ * Graph g has h_voff[g+1] - h_voff[g] vertices and edges
+ * [h_eoff[g], h_eoff[g+1]) of the concatenated arrays d_u, d_v, d_w, with
+ * vertex ids local to the graph (0-based).  Results are per graph and local:
+ * forest edge ids (local, ascending) at d_ids[h_voff[g] ...] (d_counts[g] of
+ * them), min-id labels at d_labels[h_voff[g] ...] (may be NULL), exact
+ * forest weights d_weights[g] with d_inexact[g] != 0 where a float weight was
+ * off the 2^-24 grid (both may be NULL).  h_voff / h_eoff: k + 1 host
+ * offsets starting at 0.  Blocks until done, like phem_mst_solve.
+ * ------------------------------------------------------------------------- */
+int phem_batch_workspace_bytes(uint64_t n_total, uint64_t m_total, uint32_t k,
+                               uint64_t *bytes);
+int phem_batch_solve(uint32_t k, const uint64_t *h_voff,
+                     const uint64_t *h_eoff, const uint32_t *d_u,
+                     const uint32_t *d_v, const uint32_t *d_w, int wmode,
+                     void *d_ws, uint64_t ws_bytes, uint32_t *d_ids,
+                     uint32_t *d_counts, uint32_t *d_labels,
+                     uint64_t *d_weights, uint64_t *d_inexact,
+                     const phem_mst_opts *opts, phem_mst_stats *stats,
+                     void *stream);
+
+/* ---------------------------------------------------------------------------
  * Synthetic inputs on the device (counter-based splitmix64 streams; the same
  * bytes as the host generator on every host).  Edges are stored with u < v.
  * ------------------------------------------------------------------------- */

@@ -200,6 +200,23 @@ __device__ __forceinline__ uint32_t uf_union(uint32_t *par, uint32_t a,
   return U32_NONE;
 }
 
+// Exact weight of one edge as an integer: the unsigned weight itself
+// (wmode 0), or w * 2^24 for a float32 weight that is non-negative and a
+// multiple of 2^-24 (false otherwise).
+__device__ __forceinline__ bool weight_fx(uint32_t bits, int wmode, ull *out) {
+  if (wmode == 0) {
+    *out = bits;
+    return true;
+  }
+  float f = __builtin_bit_cast(float, bits);
+  if (!(f >= 0.0f)) return false;
+  float s = f * 16777216.0f;  // exact scaling by 2^24
+  if (!(s < 1.8446744e19f)) return false;
+  if (truncf(s) != s) return false;
+  *out = (ull)s;
+  return true;
+}
+
 // ---- counter-based random streams (splitmix64; Steele, Lea, Flood 2014) ----
 
 constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;

@@ -512,20 +512,6 @@ __global__ __launch_bounds__(BLK) void k_scan_blocks(ull *blk, uint64_t nb,
   if (threadIdx.x == 0) *sum = carry;
 }
 
-__device__ __forceinline__ bool weight_fx(uint32_t bits, int wmode, ull *out) {
-  if (wmode == 0) {
-    *out = bits;
-    return true;
-  }
-  float f = __builtin_bit_cast(float, bits);
-  if (!(f >= 0.0f)) return false;
-  float s = f * 16777216.0f;  // exact scaling by 2^24
-  if (!(s < 1.8446744e19f)) return false;
-  if (truncf(s) != s) return false;
-  *out = (ull)s;
-  return true;
-}
-
 // Bitmap -> ascending ids; exact weight sum of the forest (partial sums
 // spread over KSEG counters).
 template <class Src>

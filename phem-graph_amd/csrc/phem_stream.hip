@@ -1,0 +1,255 @@
+// phem_stream.hip -- streaming minimum spanning forest and connected
+// components with device-resident state.
+//
+// The forest after batch t is F_t = MSF(F_{t-1} u B_t), which equals the
+// minimum spanning forest of every edge inserted so far (cycle property): an
+// edge left out of F_{t-1} is the heaviest edge of a cycle of edges inserted
+// by then, and stays excluded.  Edges carry a global id (insertion order);
+// the solve orders ties by position in [F_{t-1} | B_t], and F_{t-1} is kept
+// sorted by global id ahead of the batch, so position order is global-id
+// order.  This is the chunked MSF of the reference's streamline path
+// (d_edges + chunk_size <- previous MSF, MST/src/kernels.cu:942-1102,
+// 949-953) with the forest kept in HBM between batches.  Connected components
+// follow the batch-by-batch union-find of the reference's SpanningTree
+// (CC/ExternalSpanningTree.cu:170-204) on a parent array kept in HBM.
+#include <cstring>
+#include <new>
+
+#include "phem_dev.h"
+#include "phem_internal.h"
+
+struct phem_stream {
+  uint32_t n = 0;
+  uint64_t max_batch = 0, cap = 0;
+  int wmode = 0;
+  int device = 0;
+  // two [forest | batch] buffers; `cur` holds the forest in [0, f)
+  uint32_t *u[2] = {nullptr, nullptr}, *v[2] = {nullptr, nullptr},
+           *w[2] = {nullptr, nullptr};
+  unsigned long long *gid[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint64_t f = 0;
+  uint64_t next_gid = 0;
+  uint64_t batches = 0;
+  void *ws = nullptr;
+  uint64_t ws_bytes = 0;
+  uint32_t *ids = nullptr;      // forest positions of the last solve
+  uint32_t *labels = nullptr;   // min-id labels of the last solve
+  uint32_t *parent = nullptr;   // union-find parent (connected components)
+  unsigned long long *cc_forest = nullptr;
+  unsigned long long *packed = nullptr;  // batch records (u << 32) | v
+  phem_mst_stats last{};
+};
+
+namespace phem {
+
+constexpr int SBLK_S = 256;
+
+__global__ __launch_bounds__(SBLK_S) void k_stream_stage(
+    const uint32_t *__restrict__ bu, const uint32_t *__restrict__ bv,
+    const uint32_t *__restrict__ bw, uint64_t mb, uint64_t gid0,
+    uint32_t *__restrict__ u, uint32_t *__restrict__ v, uint32_t *__restrict__ w,
+    ull *__restrict__ gid, ull *__restrict__ packed) {
+  for (uint64_t i = (uint64_t)blockIdx.x * SBLK_S + threadIdx.x; i < mb;
+       i += (uint64_t)gridDim.x * SBLK_S) {
+    const uint32_t a = bu[i], b = bv[i];
+    u[i] = a;
+    v[i] = b;
+    w[i] = bw[i];
+    gid[i] = gid0 + i;
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    packed[i] = ((ull)lo << 32) | hi;
+  }
+}
+
+__global__ __launch_bounds__(SBLK_S) void k_stream_keep(
+    const uint32_t *__restrict__ ids, uint64_t count,
+    const uint32_t *__restrict__ su, const uint32_t *__restrict__ sv,
+    const uint32_t *__restrict__ sw, const ull *__restrict__ sg,
+    uint32_t *__restrict__ du, uint32_t *__restrict__ dv,
+    uint32_t *__restrict__ dw, ull *__restrict__ dg) {
+  for (uint64_t k = (uint64_t)blockIdx.x * SBLK_S + threadIdx.x; k < count;
+       k += (uint64_t)gridDim.x * SBLK_S) {
+    const uint32_t p = ids[k];
+    du[k] = su[p];
+    dv[k] = sv[p];
+    dw[k] = sw[p];
+    dg[k] = sg[p];
+  }
+}
+
+static void stream_free(phem_stream *s) {
+  for (int b = 0; b < 2; b++) {
+    (void)hipFree(s->u[b]);
+    (void)hipFree(s->v[b]);
+    (void)hipFree(s->w[b]);
+    (void)hipFree(s->gid[b]);
+  }
+  (void)hipFree(s->ws);
+  (void)hipFree(s->ids);
+  (void)hipFree(s->labels);
+  (void)hipFree(s->parent);
+  (void)hipFree(s->cc_forest);
+  (void)hipFree(s->packed);
+  delete s;
+}
+
+}  // namespace phem
+
+using namespace phem;
+
+extern "C" int phem_stream_create(uint32_t n, uint64_t max_batch, int wmode,
+                                  phem_stream_t **out) {
+  if (!out) return set_err(PHEM_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (n < 1 || n > PHEM_MAX_VERTICES || max_batch < 1)
+    return set_err(PHEM_EINVAL, "bad n=%u / max_batch=%llu", n,
+                   (unsigned long long)max_batch);
+  if (wmode != PHEM_W_U32 && wmode != PHEM_W_F32)
+    return set_err(PHEM_EINVAL, "unknown weight mode %d", wmode);
+  const uint64_t cap = (uint64_t)n + max_batch;
+  if (cap > PHEM_MAX_EDGES)
+    return set_err(PHEM_EINVAL, "n + max_batch exceeds the edge-id range");
+  phem_stream *s = new (std::nothrow) phem_stream();
+  if (!s) return set_err(PHEM_ENOMEM, "host allocation failed");
+  s->n = n;
+  s->max_batch = max_batch;
+  s->cap = cap;
+  s->wmode = wmode;
+  (void)hipGetDevice(&s->device);
+  int rc = phem_mst_workspace_bytes(n, cap, &s->ws_bytes);
+  if (rc) {
+    stream_free(s);
+    return rc;
+  }
+  bool ok = true;
+  for (int b = 0; b < 2 && ok; b++) {
+    ok = hipMalloc(&s->u[b], 4 * cap) == hipSuccess &&
+         hipMalloc(&s->v[b], 4 * cap) == hipSuccess &&
+         hipMalloc(&s->w[b], 4 * cap) == hipSuccess &&
+         hipMalloc(&s->gid[b], 8 * cap) == hipSuccess;
+  }
+  ok = ok && hipMalloc(&s->ws, s->ws_bytes) == hipSuccess &&
+       hipMalloc(&s->ids, 4ull * n) == hipSuccess &&
+       hipMalloc(&s->labels, 4ull * n) == hipSuccess &&
+       hipMalloc(&s->parent, 4ull * n) == hipSuccess &&
+       hipMalloc(&s->cc_forest, 8ull * n) == hipSuccess &&
+       hipMalloc(&s->packed, 8 * max_batch) == hipSuccess;
+  if (!ok) {
+    stream_free(s);
+    return set_err(PHEM_ENOMEM, "device allocation for the stream failed");
+  }
+  rc = phem_cc_init(n, s->parent, (uint64_t *)s->cc_forest, nullptr);
+  if (rc == PHEM_OK && hipDeviceSynchronize() != hipSuccess)
+    rc = set_err(PHEM_EHIP, "stream initialisation failed");
+  if (rc) {
+    stream_free(s);
+    return rc;
+  }
+  *out = s;
+  return PHEM_OK;
+}
+
+extern "C" int phem_stream_destroy(phem_stream_t *s) {
+  if (!s) return PHEM_OK;
+  stream_free(s);
+  return PHEM_OK;
+}
+
+extern "C" int phem_stream_apply(phem_stream_t *s, const uint32_t *d_u,
+                                 const uint32_t *d_v, const uint32_t *d_w,
+                                 uint64_t mb, void *stream) {
+  if (!s) return set_err(PHEM_EINVAL, "stream handle is NULL");
+  if (mb > s->max_batch)
+    return set_err(PHEM_EINVAL, "batch of %llu edges exceeds max_batch %llu",
+                   (unsigned long long)mb, (unsigned long long)s->max_batch);
+  if (mb && (!d_u || !d_v || !d_w)) return set_err(PHEM_EINVAL, "NULL batch");
+  if (s->next_gid + mb > 0xFFFFFFFFFFFFFFFFull - 1)
+    return set_err(PHEM_EINVAL, "global edge ids exhausted");
+  hipStream_t st = (hipStream_t)stream;
+  const int c = s->cur;
+  if (mb) {
+    PHEM_PROF("k_stream_stage", st);
+    hipLaunchKernelGGL(k_stream_stage, dim3(grid_for(mb, SBLK_S)), dim3(SBLK_S), 0,
+                       st, d_u, d_v, d_w, mb, (ull)s->next_gid, s->u[c] + s->f,
+                       s->v[c] + s->f, s->w[c] + s->f, s->gid[c] + s->f,
+                       s->packed);
+  }
+  PHEM_LAUNCHED("k_stream_stage");
+  const uint64_t total = s->f + mb;
+  int rc = phem_mst_solve(s->n, total, s->u[c], s->v[c], s->w[c], s->wmode,
+                          s->ws, s->ws_bytes, s->ids, s->labels, nullptr,
+                          &s->last, stream);
+  if (rc) return rc;
+  const uint64_t count = s->last.mst_count;
+  if (count) {
+    PHEM_PROF("k_stream_keep", st);
+    hipLaunchKernelGGL(k_stream_keep, dim3(grid_for(count, SBLK_S)), dim3(SBLK_S),
+                       0, st, s->ids, count, s->u[c], s->v[c], s->w[c],
+                       s->gid[c], s->u[c ^ 1], s->v[c ^ 1], s->w[c ^ 1],
+                       s->gid[c ^ 1]);
+  }
+  PHEM_LAUNCHED("k_stream_keep");
+  if (mb) {
+    rc = phem_cc_union(s->n, (const uint64_t *)s->packed, mb, s->parent,
+                       (uint64_t *)s->cc_forest, stream);
+    if (rc) return rc;
+  }
+  PHEM_HIP(hipStreamSynchronize(st));
+  s->cur ^= 1;
+  s->f = count;
+  s->next_gid += mb;
+  s->batches++;
+  return PHEM_OK;
+}
+
+extern "C" int phem_stream_forest(const phem_stream_t *s, uint64_t *count,
+                                  const uint32_t **d_u, const uint32_t **d_v,
+                                  const uint32_t **d_w,
+                                  const uint64_t **d_gid) {
+  if (!s) return set_err(PHEM_EINVAL, "stream handle is NULL");
+  if (count) *count = s->f;
+  if (d_u) *d_u = s->u[s->cur];
+  if (d_v) *d_v = s->v[s->cur];
+  if (d_w) *d_w = s->w[s->cur];
+  if (d_gid) *d_gid = (const uint64_t *)s->gid[s->cur];
+  return PHEM_OK;
+}
+
+extern "C" int phem_stream_stats(const phem_stream_t *s, phem_mst_stats *stats,
+                                 uint64_t *edges_inserted, uint64_t *batches) {
+  if (!s) return set_err(PHEM_EINVAL, "stream handle is NULL");
+  if (stats) *stats = s->last;
+  if (edges_inserted) *edges_inserted = s->next_gid;
+  if (batches) *batches = s->batches;
+  return PHEM_OK;
+}
+
+extern "C" int phem_stream_labels(const phem_stream_t *s, uint32_t *d_labels,
+                                  void *stream) {
+  if (!s) return set_err(PHEM_EINVAL, "stream handle is NULL");
+  return phem_cc_labels(s->n, s->parent, d_labels, stream);
+}
+
+extern "C" int phem_stream_cc_forest(const phem_stream_t *s,
+                                     const uint64_t **d_forest) {
+  if (!s) return set_err(PHEM_EINVAL, "stream handle is NULL");
+  if (d_forest) *d_forest = (const uint64_t *)s->cc_forest;
+  return PHEM_OK;
+}
+
+extern "C" int phem_stream_copy_forest(const phem_stream_t *s, uint32_t *d_u,
+                                       uint32_t *d_v, uint32_t *d_w,
+                                       uint64_t *d_gid, void *stream) {
+  if (!s) return set_err(PHEM_EINVAL, "stream handle is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  const int c = s->cur;
+  if (s->f) {
+    if (d_u) PHEM_HIP(hipMemcpyAsync(d_u, s->u[c], 4 * s->f, hipMemcpyDeviceToDevice, st));
+    if (d_v) PHEM_HIP(hipMemcpyAsync(d_v, s->v[c], 4 * s->f, hipMemcpyDeviceToDevice, st));
+    if (d_w) PHEM_HIP(hipMemcpyAsync(d_w, s->w[c], 4 * s->f, hipMemcpyDeviceToDevice, st));
+    if (d_gid)
+      PHEM_HIP(hipMemcpyAsync(d_gid, s->gid[c], 8 * s->f, hipMemcpyDeviceToDevice, st));
+  }
+  return PHEM_OK;
+}

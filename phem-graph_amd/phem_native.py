@@ -87,6 +87,20 @@ def lib():
         "phem_gen_uniform": (i32, [u32, u64, u64, u64, i32, vp, vp, vp, vp]),
         "phem_gen_rmat": (i32, [u32, u64, u64, u32, u32, u32, i32, vp, vp, vp, vp]),
         "phem_pack_edges": (i32, [vp, vp, u64, vp, vp]),
+        "phem_gather_edges_aos": (i32, [vp, vp, u64, vp, vp]),
+        "phem_stream_create": (i32, [u32, u64, i32, c.POINTER(vp)]),
+        "phem_stream_destroy": (i32, [vp]),
+        "phem_stream_apply": (i32, [vp, vp, vp, vp, u64, vp]),
+        "phem_stream_forest": (i32, [vp, c.POINTER(c.c_uint64), vp, vp, vp, vp]),
+        "phem_stream_copy_forest": (i32, [vp, vp, vp, vp, vp, vp]),
+        "phem_stream_stats": (i32, [vp, c.POINTER(MstStats), c.POINTER(c.c_uint64),
+                                    c.POINTER(c.c_uint64)]),
+        "phem_stream_labels": (i32, [vp, vp, vp]),
+        "phem_stream_cc_forest": (i32, [vp, c.POINTER(vp)]),
+        "phem_batch_workspace_bytes": (i32, [u64, u64, u32, c.POINTER(c.c_uint64)]),
+        "phem_batch_solve": (i32, [u32, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64),
+                                   vp, vp, vp, i32, vp, u64, vp, vp, vp, vp, vp,
+                                   c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
         "phem_prof_enable": (i32, [i32]),
         "phem_prof_reset": (i32, []),
         "phem_prof_count": (i32, []),
@@ -257,3 +271,102 @@ def prof_read():
         _check(L.phem_prof_get(i, name, 64, ctypes.byref(n), ctypes.byref(ms)), "phem_prof_get")
         out[name.value.decode()] = (n.value, ms.value)
     return out
+
+
+def batch_solve(voff, eoff, u, v, w, wmode=W_F32, labels=True, weights=True,
+                stream=None, light_factor=None, ws=None):
+    """k independent graphs in one solve (phem_batch_solve).
+
+    voff / eoff: k + 1 offsets (python ints) into the concatenated vertex and
+    edge ranges; u, v, w: concatenated int32 device tensors with graph-local
+    vertex ids.  Returns a dict with per-graph lists: ids (local forest edge
+    ids, ascending), labels (local min-id labels), weight_fx and inexact, plus
+    the solve's stats."""
+    k = len(voff) - 1
+    n_total, m_total = int(voff[-1]), int(eoff[-1])
+    dev = u.device
+    hv = (ctypes.c_uint64 * (k + 1))(*[int(x) for x in voff])
+    he = (ctypes.c_uint64 * (k + 1))(*[int(x) for x in eoff])
+    need = ctypes.c_uint64(0)
+    _check(lib().phem_batch_workspace_bytes(n_total, m_total, k, ctypes.byref(need)),
+           "phem_batch_workspace_bytes")
+    if ws is None or ws.numel() < need.value:
+        ws = torch.empty(max(need.value, 1), dtype=torch.uint8, device=dev)
+    ids = torch.empty(max(n_total, 1), dtype=torch.int32, device=dev)
+    cnt = torch.empty(k, dtype=torch.int32, device=dev)
+    lab = torch.empty(max(n_total, 1), dtype=torch.int32, device=dev) if labels else None
+    wfx = torch.empty(k, dtype=torch.int64, device=dev) if weights else None
+    bad = torch.empty(k, dtype=torch.int64, device=dev) if weights else None
+    opts = None if light_factor is None else ctypes.byref(MstOpts(light_factor, 0, 0))
+    st = MstStats()
+    _check(lib().phem_batch_solve(k, hv, he, _p(u), _p(v), _p(w), wmode, _p(ws),
+                                  ws.numel(), _p(ids), _p(cnt), _p(lab), _p(wfx), _p(bad),
+                                  opts, ctypes.byref(st), _stream(stream)),
+           "phem_batch_solve")
+    cnt_h = cnt.cpu().tolist()
+    ids_h = ids.cpu()
+    lab_h = lab.cpu() if labels else None
+    out = {"ids": [], "labels": [], "stats": st.as_dict(), "counts": cnt_h}
+    for g in range(k):
+        a = int(voff[g])
+        out["ids"].append(ids_h[a: a + cnt_h[g]])
+        if labels:
+            out["labels"].append(lab_h[a: int(voff[g + 1])])
+    if weights:
+        out["weight_fx"] = [x & 0xFFFFFFFFFFFFFFFF for x in wfx.cpu().tolist()]
+        out["inexact"] = bad.cpu().tolist()
+    return out
+
+
+class Stream:
+    """Streaming minimum spanning forest + connected components with the
+    forest and parent array resident in HBM (phem_stream_*)."""
+
+    def __init__(self, n, max_batch, wmode=W_F32, device="cuda"):
+        self.n, self.device = n, torch.device(device)
+        h = ctypes.c_void_p()
+        _check(lib().phem_stream_create(n, max_batch, wmode, ctypes.byref(h)),
+               "phem_stream_create")
+        self.h = h
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib().phem_stream_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, u, v, w, stream=None):
+        _check(lib().phem_stream_apply(self.h, _p(u), _p(v), _p(w), u.numel(),
+                                       _stream(stream)), "phem_stream_apply")
+
+    def stats(self):
+        st = MstStats()
+        e = ctypes.c_uint64(0)
+        b = ctypes.c_uint64(0)
+        _check(lib().phem_stream_stats(self.h, ctypes.byref(st), ctypes.byref(e),
+                                       ctypes.byref(b)), "phem_stream_stats")
+        d = st.as_dict()
+        d["edges_inserted"], d["batches"] = e.value, b.value
+        return d
+
+    def forest(self):
+        """(u, v, w, gid) device tensors of the current forest (copies)."""
+        cnt = ctypes.c_uint64(0)
+        _check(lib().phem_stream_forest(self.h, ctypes.byref(cnt), None, None, None, None),
+               "phem_stream_forest")
+        k = cnt.value
+        out = [torch.empty(max(k, 1), dtype=torch.int32, device=self.device) for _ in range(3)]
+        g = torch.empty(max(k, 1), dtype=torch.int64, device=self.device)
+        _check(lib().phem_stream_copy_forest(self.h, _p(out[0]), _p(out[1]), _p(out[2]),
+                                             _p(g), _stream()), "phem_stream_copy_forest")
+        return out[0][:k], out[1][:k], out[2][:k], g[:k]
+
+    def labels(self):
+        lab = torch.empty(max(self.n, 1), dtype=torch.int32, device=self.device)
+        _check(lib().phem_stream_labels(self.h, _p(lab), _stream()), "phem_stream_labels")
+        return lab[: self.n]
