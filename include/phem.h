@@ -236,6 +236,89 @@ int phem_batch_solve(uint32_t k, const uint64_t *h_voff,
                      uint64_t *d_weights, uint64_t *d_inexact,
                      const phem_mst_opts *opts, phem_mst_stats *stats,
                      void *stream);
+/* The same with graph g's edges in its own device arrays h_pu[g], h_pv[g],
+ * h_pw[g] (host tables of k device pointers; h_eoff gives the edge counts). */
+int phem_batch_solve_ptrs(uint32_t k, const uint64_t *h_voff,
+                          const uint64_t *h_eoff, const uint32_t *const *h_pu,
+                          const uint32_t *const *h_pv,
+                          const uint32_t *const *h_pw, int wmode, void *d_ws,
+                          uint64_t ws_bytes, uint32_t *d_ids,
+                          uint32_t *d_counts, uint32_t *d_labels,
+                          uint64_t *d_weights, uint64_t *d_inexact,
+                          const phem_mst_opts *opts, phem_mst_stats *stats,
+                          void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Serving: a request loop for MST + CC queries on independent graphs.
+ * No counterpart in the reference (its drivers solve one graph per process
+ * run, MST/main.cu:5-30, CC/main.cu); the batched solve above does the work.
+ *
+ * Requests are queued as they arrive.  Each of `slots` workers (one host
+ * thread and one HIP stream each, its own device memory of mem_bytes / slots)
+ * takes the queued requests in arrival order while their combined device
+ * footprint fits its memory and there are at most max_graphs of them, and
+ * runs them as one batch; a request that arrives while batches are running
+ * joins the next batch a worker forms, so work from several batches overlaps
+ * on the device.  Results go to the request's own host buffers.
+ * A request larger than one slot can hold is refused with PHEM_EBUSY.
+ * Times are seconds on phem_server_clock() (a monotonic clock).
+ * ------------------------------------------------------------------------- */
+typedef struct phem_server phem_server;
+
+typedef struct {
+  uint64_t mem_bytes;       /* device memory for all slots; 0: 1/4 of free */
+  uint64_t max_batch_edges; /* edges per batch; 0: 2^27 */
+  uint32_t slots;           /* batches in flight; 0: 2 */
+  uint32_t max_graphs;      /* graphs per batch; 0: 4096 */
+  uint32_t window_us;       /* an idle worker that finds fewer than max_graphs
+                               requests waits up to this long for more */
+  int32_t wmode;            /* PHEM_W_U32 or PHEM_W_F32 for every request */
+} phem_server_opts;
+
+#define PHEM_REQ_DEVICE 1u /* u, v, w are device pointers */
+
+typedef struct {
+  uint32_t n;
+  uint32_t flags;
+  uint64_t m;
+  const uint32_t *u, *v, *w; /* edges with local ids; valid until done */
+  uint32_t *forest_ids;      /* host, n slots: forest edge ids ascending
+                                (NULL: not returned) */
+  uint32_t *labels;          /* host, n slots: min-id labels (or NULL) */
+  double arrival;            /* < 0: the time of submission */
+} phem_request;
+
+typedef struct {
+  int32_t status;        /* PHEM_OK or the error of its batch */
+  uint32_t forest_count; /* edges in forest_ids */
+  uint64_t weight_fx;    /* exact forest weight (x 2^24 for float weights) */
+  int32_t weight_exact;
+  uint32_t batch_graphs; /* graphs in the batch the request ran in */
+  double arrival, start, done;
+} phem_result;
+
+typedef struct {
+  uint64_t submitted, completed, refused, batches, edges;
+  double first_arrival, last_done;          /* of completed requests */
+  double lat_p50, lat_p99, lat_max, lat_mean; /* done - arrival, seconds */
+  double mean_batch_graphs;
+  uint64_t slot_bytes, slot_edges;          /* capacity of one slot */
+} phem_server_stats;
+
+double phem_server_clock(void);
+int phem_server_create(const phem_server_opts *opts, phem_server **out);
+int phem_server_destroy(phem_server *srv);
+int phem_server_submit(phem_server *srv, const phem_request *req,
+                       uint64_t *ticket);
+/* Blocks until the request is done, fills *res and forgets the ticket. */
+int phem_server_wait(phem_server *srv, uint64_t ticket, phem_result *res);
+/* *done = 1 and *res filled (ticket forgotten) if done, else *done = 0. */
+int phem_server_poll(phem_server *srv, uint64_t ticket, phem_result *res,
+                     int *done);
+/* Blocks until every submitted request is done. */
+int phem_server_drain(phem_server *srv);
+int phem_server_get_stats(phem_server *srv, phem_server_stats *st);
+int phem_server_reset_stats(phem_server *srv);
 
 /* ---------------------------------------------------------------------------
  * Synthetic inputs on the device (counter-based splitmix64 streams; the same

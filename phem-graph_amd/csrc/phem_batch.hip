@@ -7,8 +7,14 @@
 // order, so one solve gives every graph's exact forest; a graph whose
 // components are finished simply has no alive edges or active components in
 // later rounds.  Labels stay minimum vertex ids (voff[g] + local minimum).
+//
+// The graphs come either concatenated (one u / v / w array, graph g at
+// eoff[g]) or as k separate device arrays (a pointer table per field); the
+// rebase pass reads either and writes the global endpoint arrays the solve
+// runs on, so scattered requests are gathered in the same single pass.
 #include <cstring>
 
+#include "phem_batch_core.h"
 #include "phem_dev.h"
 #include "phem_internal.h"
 
@@ -16,11 +22,12 @@ namespace phem {
 
 constexpr int BBLK = 256;
 
-// index g with off[g] <= x < off[g + 1] (off ascending, off[0] = 0)
 __device__ __forceinline__ uint64_t ceil_div_u64(uint64_t a, uint64_t b) {
   return (a + b - 1) / b;
 }
 
+// index g with off[g] <= x < off[g + 1] (off ascending, off[0] = 0); empty
+// ranges are skipped because the last such g is taken
 __device__ __forceinline__ uint32_t seg_of(const ull *__restrict__ off,
                                            uint32_t k, ull x) {
   uint32_t lo = 0, hi = k - 1;
@@ -31,18 +38,39 @@ __device__ __forceinline__ uint32_t seg_of(const ull *__restrict__ off,
   return lo;
 }
 
-// global endpoints; a local id outside its graph maps to an invalid vertex
+// Global endpoints (a local id outside its graph maps to an invalid vertex)
+// and, from pointer tables, the gathered weights.  A thread handles 4
+// consecutive edges; a group that lies inside one graph (the common case)
+// does one graph lookup.
+template <bool TABLE>
 __global__ __launch_bounds__(BBLK) void k_batch_rebase(
     const uint32_t *__restrict__ u, const uint32_t *__restrict__ v,
+    const uint32_t *__restrict__ w, const uint64_t *__restrict__ pu,
+    const uint64_t *__restrict__ pv, const uint64_t *__restrict__ pw,
     uint64_t m, const ull *__restrict__ voff, const ull *__restrict__ eoff,
-    uint32_t k, uint32_t *__restrict__ gu, uint32_t *__restrict__ gv) {
-  for (uint64_t i = (uint64_t)blockIdx.x * BBLK + threadIdx.x; i < m;
-       i += (uint64_t)gridDim.x * BBLK) {
-    const uint32_t g = seg_of(eoff, k, i);
-    const ull base = voff[g], ng = voff[g + 1] - base;
-    const uint32_t a = u[i], b = v[i];
-    gu[i] = a < ng ? (uint32_t)(base + a) : U32_NONE;
-    gv[i] = b < ng ? (uint32_t)(base + b) : U32_NONE;
+    uint32_t k, uint32_t *__restrict__ gu, uint32_t *__restrict__ gv,
+    uint32_t *__restrict__ gw) {
+  const uint64_t groups = ceil_div_u64(m, 4);
+  for (uint64_t t = (uint64_t)blockIdx.x * BBLK + threadIdx.x; t < groups;
+       t += (uint64_t)gridDim.x * BBLK) {
+    const uint64_t i0 = t * 4, i1 = i0 + 4 < m ? i0 + 4 : m;
+    uint32_t g = seg_of(eoff, k, i0);
+    for (uint64_t i = i0; i < i1; i++) {
+      while (g + 1 < k && eoff[g + 1] <= i) g++;
+      const ull base = voff[g], ng = voff[g + 1] - base;
+      uint32_t a, b;
+      if (TABLE) {
+        const uint64_t j = i - eoff[g];
+        a = ((const uint32_t *)pu[g])[j];
+        b = ((const uint32_t *)pv[g])[j];
+        gw[i] = ((const uint32_t *)pw[g])[j];
+      } else {
+        a = u[i];
+        b = v[i];
+      }
+      gu[i] = a < ng ? (uint32_t)(base + a) : U32_NONE;
+      gv[i] = b < ng ? (uint32_t)(base + b) : U32_NONE;
+    }
   }
 }
 
@@ -127,7 +155,8 @@ __global__ __launch_bounds__(BBLK) void k_batch_zero(ull *a, uint64_t cnt) {
 }
 
 struct BatchLayout {
-  uint64_t mst_ws, mst_bytes, gu, gv, ids, glab, voff, eoff, first, total;
+  uint64_t mst_ws, mst_bytes, gu, gv, gw, ids, glab, voff, eoff, ptrs, first,
+      total;
 };
 
 static BatchLayout batch_layout(uint64_t n_total, uint64_t m_total, uint32_t k,
@@ -143,21 +172,19 @@ static BatchLayout batch_layout(uint64_t n_total, uint64_t m_total, uint32_t k,
   L.mst_ws = take(mst_bytes);
   L.gu = take(4 * m_total);
   L.gv = take(4 * m_total);
+  L.gw = take(4 * m_total);
   L.ids = take(4 * n_total);
   L.glab = take(4 * n_total);
   L.voff = take(8ull * (k + 1));
   L.eoff = take(8ull * (k + 1));
+  L.ptrs = take(24ull * k);
   L.first = take(4ull * k);
   L.total = o;
   return L;
 }
 
-}  // namespace phem
-
-using namespace phem;
-
-extern "C" int phem_batch_workspace_bytes(uint64_t n_total, uint64_t m_total,
-                                          uint32_t k, uint64_t *bytes) {
+int batch_workspace_bytes(uint64_t n_total, uint64_t m_total, uint32_t k,
+                          uint64_t *bytes) {
   if (!bytes) return set_err(PHEM_EINVAL, "bytes is NULL");
   if (n_total > PHEM_MAX_VERTICES || m_total > PHEM_MAX_EDGES || k == 0)
     return set_err(PHEM_EINVAL, "batch of %u graphs, %llu vertices, %llu edges "
@@ -170,14 +197,9 @@ extern "C" int phem_batch_workspace_bytes(uint64_t n_total, uint64_t m_total,
   return PHEM_OK;
 }
 
-extern "C" int phem_batch_solve(uint32_t k, const uint64_t *h_voff,
-                                const uint64_t *h_eoff, const uint32_t *d_u,
-                                const uint32_t *d_v, const uint32_t *d_w,
-                                int wmode, void *d_ws, uint64_t ws_bytes,
-                                uint32_t *d_ids, uint32_t *d_counts,
-                                uint32_t *d_labels, uint64_t *d_weights,
-                                uint64_t *d_inexact, const phem_mst_opts *opts,
-                                phem_mst_stats *stats, void *stream) {
+int batch_solve(const BatchArgs &A, hipStream_t s) {
+  const uint32_t k = A.k;
+  const uint64_t *h_voff = A.h_voff, *h_eoff = A.h_eoff;
   if (k == 0 || !h_voff || !h_eoff)
     return set_err(PHEM_EINVAL, "empty batch or NULL offsets");
   if (h_voff[0] != 0 || h_eoff[0] != 0)
@@ -187,63 +209,112 @@ extern "C" int phem_batch_solve(uint32_t k, const uint64_t *h_voff,
       return set_err(PHEM_EINVAL, "offsets of graph %u decrease", g);
   const uint64_t n_total = h_voff[k], m_total = h_eoff[k];
   uint64_t need = 0;
-  int rc = phem_batch_workspace_bytes(n_total, m_total, k, &need);
+  int rc = batch_workspace_bytes(n_total, m_total, k, &need);
   if (rc) return rc;
-  if (!d_ws || ws_bytes < need)
+  if (!A.d_ws || A.ws_bytes < need)
     return set_err(PHEM_ENOMEM, "batch workspace %llu B < required %llu B",
-                   (unsigned long long)ws_bytes, (unsigned long long)need);
-  if ((m_total && (!d_u || !d_v || !d_w)) || (n_total && !d_ids) || !d_counts)
-    return set_err(PHEM_EINVAL, "NULL array");
+                   (unsigned long long)A.ws_bytes, (unsigned long long)need);
+  const bool table = A.h_pu != nullptr;
+  if (table) {
+    if (!A.h_pv || !A.h_pw)
+      return set_err(PHEM_EINVAL, "pointer tables must all be given");
+    for (uint32_t g = 0; g < k; g++)
+      if (h_eoff[g + 1] > h_eoff[g] && (!A.h_pu[g] || !A.h_pv[g] || !A.h_pw[g]))
+        return set_err(PHEM_EINVAL, "graph %u has edges but a NULL array", g);
+  } else if (m_total && (!A.d_u || !A.d_v || !A.d_w)) {
+    return set_err(PHEM_EINVAL, "NULL edge array");
+  }
+  if ((n_total && !A.d_ids) || !A.d_counts)
+    return set_err(PHEM_EINVAL, "NULL output array");
+  if ((A.d_weights != nullptr) != (A.d_inexact != nullptr))
+    return set_err(PHEM_EINVAL, "weights and inexact go together");
   uint64_t mb = 0;
   rc = phem_mst_workspace_bytes((uint32_t)n_total, m_total, &mb);
   if (rc) return rc;
   const BatchLayout L = batch_layout(n_total, m_total, k, mb);
-  hipStream_t s = (hipStream_t)stream;
-  char *ws = (char *)d_ws;
+  char *ws = (char *)A.d_ws;
   uint32_t *gu = (uint32_t *)(ws + L.gu), *gv = (uint32_t *)(ws + L.gv);
   uint32_t *ids = (uint32_t *)(ws + L.ids), *glab = (uint32_t *)(ws + L.glab);
   ull *voff = (ull *)(ws + L.voff), *eoff = (ull *)(ws + L.eoff);
+  uint64_t *ptrs = (uint64_t *)(ws + L.ptrs);
   uint32_t *first = (uint32_t *)(ws + L.first);
-  PHEM_HIP(hipMemcpyAsync(voff, h_voff, 8ull * (k + 1), hipMemcpyHostToDevice, s));
-  PHEM_HIP(hipMemcpyAsync(eoff, h_eoff, 8ull * (k + 1), hipMemcpyHostToDevice, s));
+  const uint32_t *gw = table ? (const uint32_t *)(ws + L.gw) : A.d_w;
+  // offsets (and pointer tables) go up in one copy from a pinned buffer
+  // owned by the calling thread, so the copy is asynchronous and the next
+  // call cannot overwrite it before the solve below has synchronized
+  struct Stage {
+    uint64_t *p = nullptr;
+    uint64_t cap = 0;
+  };
+  static thread_local Stage st_buf;
+  const uint64_t words = 2ull * (k + 1) + (table ? 3ull * k : 0ull);
+  if (st_buf.cap < words) {
+    if (st_buf.p) (void)hipHostFree(st_buf.p);
+    st_buf.p = nullptr;
+    st_buf.cap = 0;
+    PHEM_HIP(hipHostMalloc((void **)&st_buf.p, 8 * words));
+    st_buf.cap = words;
+  }
+  uint64_t *hstage = st_buf.p;
+  std::memcpy(hstage, h_voff, 8ull * (k + 1));
+  std::memcpy(hstage + (k + 1), h_eoff, 8ull * (k + 1));
+  PHEM_HIP(hipMemcpyAsync(voff, hstage, 8ull * (k + 1), hipMemcpyHostToDevice, s));
+  PHEM_HIP(hipMemcpyAsync(eoff, hstage + (k + 1), 8ull * (k + 1),
+                          hipMemcpyHostToDevice, s));
+  if (table) {
+    uint64_t *t = hstage + 2ull * (k + 1);
+    for (uint32_t g = 0; g < k; g++) {
+      t[g] = (uint64_t)(uintptr_t)A.h_pu[g];
+      t[k + g] = (uint64_t)(uintptr_t)A.h_pv[g];
+      t[2ull * k + g] = (uint64_t)(uintptr_t)A.h_pw[g];
+    }
+    PHEM_HIP(hipMemcpyAsync(ptrs, t, 24ull * k, hipMemcpyHostToDevice, s));
+  }
   if (m_total) {
     PHEM_PROF("k_batch_rebase", s);
-    hipLaunchKernelGGL(k_batch_rebase, dim3(grid_for(m_total, BBLK)), dim3(BBLK),
-                       0, s, d_u, d_v, m_total, voff, eoff, k, gu, gv);
+    if (table)
+      hipLaunchKernelGGL(k_batch_rebase<true>,
+                         dim3(grid_for(ceil_div(m_total, 4), BBLK)), dim3(BBLK),
+                         0, s, nullptr, nullptr, nullptr, ptrs, ptrs + k,
+                         ptrs + 2ull * k, m_total, voff, eoff, k, gu, gv,
+                         (uint32_t *)(ws + L.gw));
+    else
+      hipLaunchKernelGGL(k_batch_rebase<false>,
+                         dim3(grid_for(ceil_div(m_total, 4), BBLK)), dim3(BBLK),
+                         0, s, A.d_u, A.d_v, A.d_w, nullptr, nullptr, nullptr,
+                         m_total, voff, eoff, k, gu, gv, nullptr);
   }
   PHEM_LAUNCHED("k_batch_rebase");
   phem_mst_stats st;
-  rc = phem_mst_solve((uint32_t)n_total, m_total, gu, gv, d_w, wmode,
-                      ws + L.mst_ws, L.mst_bytes, ids, glab, opts, &st, stream);
+  rc = phem_mst_solve((uint32_t)n_total, m_total, gu, gv, gw, A.wmode,
+                      ws + L.mst_ws, L.mst_bytes, ids, glab, A.opts, &st, s);
   if (rc) return rc;
   {
     PHEM_PROF("k_batch_counts", s);
     hipLaunchKernelGGL(k_batch_counts, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,
-                       ids, (uint64_t)st.mst_count, eoff, k, first, d_counts);
+                       ids, (uint64_t)st.mst_count, eoff, k, first, A.d_counts);
   }
   PHEM_LAUNCHED("k_batch_counts");
   if (st.mst_count) {
     PHEM_PROF("k_batch_split_ids", s);
     hipLaunchKernelGGL(k_batch_split_ids, dim3(grid_for(st.mst_count, BBLK)),
                        dim3(BBLK), 0, s, ids, (uint64_t)st.mst_count, voff,
-                       eoff, k, first, d_ids);
+                       eoff, k, first, A.d_ids);
   }
   PHEM_LAUNCHED("k_batch_split_ids");
-  if (d_labels && n_total) {
+  if (A.d_labels && n_total) {
     PHEM_PROF("k_batch_split_labels", s);
     hipLaunchKernelGGL(k_batch_split_labels, dim3(grid_for(n_total, BBLK)),
-                       dim3(BBLK), 0, s, glab, n_total, voff, k, d_labels);
+                       dim3(BBLK), 0, s, glab, n_total, voff, k, A.d_labels);
   }
   PHEM_LAUNCHED("k_batch_split_labels");
-  if (d_weights || d_inexact) {
-    if (!d_weights || !d_inexact)
-      return set_err(PHEM_EINVAL, "weights and inexact go together");
+  if (A.d_weights) {
     {
       PHEM_PROF("k_batch_zero", s);
       hipLaunchKernelGGL(k_batch_zero, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,
-                         (ull *)d_weights, (uint64_t)k);
+                         (ull *)A.d_weights, (uint64_t)k);
       hipLaunchKernelGGL(k_batch_zero, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,
-                         (ull *)d_inexact, (uint64_t)k);
+                         (ull *)A.d_inexact, (uint64_t)k);
     }
     PHEM_LAUNCHED("k_batch_zero");
     if (st.mst_count) {
@@ -251,10 +322,75 @@ extern "C" int phem_batch_solve(uint32_t k, const uint64_t *h_voff,
       hipLaunchKernelGGL(k_batch_weights,
                          dim3(grid_for(ceil_div(st.mst_count, WCH), BBLK)),
                          dim3(BBLK), 0, s, ids, (uint64_t)st.mst_count, eoff, k,
-                         d_w, wmode, (ull *)d_weights, (ull *)d_inexact);
+                         gw, A.wmode, (ull *)A.d_weights, (ull *)A.d_inexact);
     }
     PHEM_LAUNCHED("k_batch_weights");
   }
-  if (stats) *stats = st;
+  if (A.stats) *A.stats = st;
   return PHEM_OK;
+}
+
+}  // namespace phem
+
+using namespace phem;
+
+extern "C" int phem_batch_workspace_bytes(uint64_t n_total, uint64_t m_total,
+                                          uint32_t k, uint64_t *bytes) {
+  return batch_workspace_bytes(n_total, m_total, k, bytes);
+}
+
+extern "C" int phem_batch_solve(uint32_t k, const uint64_t *h_voff,
+                                const uint64_t *h_eoff, const uint32_t *d_u,
+                                const uint32_t *d_v, const uint32_t *d_w,
+                                int wmode, void *d_ws, uint64_t ws_bytes,
+                                uint32_t *d_ids, uint32_t *d_counts,
+                                uint32_t *d_labels, uint64_t *d_weights,
+                                uint64_t *d_inexact, const phem_mst_opts *opts,
+                                phem_mst_stats *stats, void *stream) {
+  BatchArgs A{};
+  A.k = k;
+  A.h_voff = h_voff;
+  A.h_eoff = h_eoff;
+  A.d_u = d_u;
+  A.d_v = d_v;
+  A.d_w = d_w;
+  A.wmode = wmode;
+  A.d_ws = d_ws;
+  A.ws_bytes = ws_bytes;
+  A.d_ids = d_ids;
+  A.d_counts = d_counts;
+  A.d_labels = d_labels;
+  A.d_weights = d_weights;
+  A.d_inexact = d_inexact;
+  A.opts = opts;
+  A.stats = stats;
+  return batch_solve(A, (hipStream_t)stream);
+}
+
+extern "C" int phem_batch_solve_ptrs(
+    uint32_t k, const uint64_t *h_voff, const uint64_t *h_eoff,
+    const uint32_t *const *h_pu, const uint32_t *const *h_pv,
+    const uint32_t *const *h_pw, int wmode, void *d_ws, uint64_t ws_bytes,
+    uint32_t *d_ids, uint32_t *d_counts, uint32_t *d_labels,
+    uint64_t *d_weights, uint64_t *d_inexact, const phem_mst_opts *opts,
+    phem_mst_stats *stats, void *stream) {
+  if (!h_pu) return set_err(PHEM_EINVAL, "pointer tables are NULL");
+  BatchArgs A{};
+  A.k = k;
+  A.h_voff = h_voff;
+  A.h_eoff = h_eoff;
+  A.h_pu = h_pu;
+  A.h_pv = h_pv;
+  A.h_pw = h_pw;
+  A.wmode = wmode;
+  A.d_ws = d_ws;
+  A.ws_bytes = ws_bytes;
+  A.d_ids = d_ids;
+  A.d_counts = d_counts;
+  A.d_labels = d_labels;
+  A.d_weights = d_weights;
+  A.d_inexact = d_inexact;
+  A.opts = opts;
+  A.stats = stats;
+  return batch_solve(A, (hipStream_t)stream);
 }

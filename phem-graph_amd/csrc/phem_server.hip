@@ -1,0 +1,470 @@
+// phem_server.hip -- request loop with continuous batching (host code).
+//
+// Requests wait in one arrival-ordered queue.  Each slot owns a worker thread,
+// a HIP stream, one block of device memory and pinned host staging buffers
+// sized at creation, so serving allocates nothing per request.  An idle worker
+// takes the queued requests in order while the batch's device footprint (the
+// batched solve's workspace, staged host inputs and outputs) fits its memory
+// and the graph and edge caps hold, runs them as one batched solve and copies
+// every result to its request's host buffers.  Several slots run at once, on
+// their own streams, so the H2D staging and host-side scatter of one batch
+// overlap the kernels of another; requests that arrive meanwhile form the
+// next batch.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "phem_batch_core.h"
+#include "phem_internal.h"
+
+namespace phem {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double now_s() {
+  return std::chrono::duration<double>(Clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+  phem_request q;
+  phem_result res;
+  uint64_t ticket;
+  bool done = false;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  void *dev = nullptr;
+  uint32_t *hin = nullptr;  // pinned: staged host edges (u | v | w)
+  char *hout = nullptr;     // pinned: results on their way to the requests
+  uint64_t hout_bytes = 0;
+  std::thread th;
+};
+
+uint64_t a256(uint64_t x) { return align_up(x ? x : 1, 256); }
+
+}  // namespace
+}  // namespace phem
+
+using namespace phem;
+
+struct phem_server {
+  int dev = 0;
+  phem_server_opts o{};
+  uint64_t slot_bytes = 0;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req *> queue;
+  std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+  uint64_t next_ticket = 1, in_flight = 0;
+  bool stop = false;
+  // statistics since the last reset
+  uint64_t submitted = 0, completed = 0, refused = 0, batches = 0, edges = 0,
+           batch_graphs = 0;
+  double first_arrival = 0, last_done = 0;
+  std::vector<double> lat;
+
+  // Device bytes of a batch of k graphs with n vertices, m edges of which
+  // mh come from host memory.
+  int footprint(uint64_t n, uint64_t m, uint64_t mh, uint32_t k,
+                uint64_t *bytes) const {
+    uint64_t ws = 0;
+    int rc = batch_workspace_bytes(n, m, k, &ws);
+    if (rc) return rc;
+    *bytes = a256(ws) + 3 * a256(4 * mh) + 2 * a256(4 * n) + a256(4ull * k) +
+             2 * a256(8ull * k);
+    return PHEM_OK;
+  }
+  bool fits(uint64_t n, uint64_t m, uint64_t mh, uint32_t k) const {
+    if (k > o.max_graphs || m > o.max_batch_edges || n > o.max_batch_edges)
+      return false;
+    uint64_t b = 0;
+    if (footprint(n, m, mh, k, &b) != PHEM_OK) return false;
+    return b <= slot_bytes;
+  }
+  void worker(int si);
+  void run_batch(Slot &S, std::vector<Req *> &batch);
+};
+
+void phem_server::worker(int si) {
+  (void)hipSetDevice(dev);
+  Slot &S = slots[si];
+  std::vector<Req *> batch;
+  for (;;) {
+    batch.clear();
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+      if (queue.empty()) return;  // stop requested and nothing left
+      if (o.window_us && queue.size() < o.max_graphs && !stop) {
+        const auto until = Clock::now() + std::chrono::microseconds(o.window_us);
+        cv_work.wait_until(lk, until, [&] {
+          return stop || queue.size() >= o.max_graphs;
+        });
+        if (queue.empty()) continue;  // another worker took them
+      }
+      uint64_t n = 0, m = 0, mh = 0;
+      while (!queue.empty()) {
+        Req *r = queue.front();
+        const uint64_t rn = r->q.n, rm = r->q.m;
+        const uint64_t rmh = (r->q.flags & PHEM_REQ_DEVICE) ? 0 : rm;
+        if (!fits(n + rn, m + rm, mh + rmh, (uint32_t)batch.size() + 1)) break;
+        n += rn;
+        m += rm;
+        mh += rmh;
+        batch.push_back(r);
+        queue.pop_front();
+      }
+      in_flight += batch.size();
+    }
+    if (!batch.empty()) run_batch(S, batch);
+  }
+}
+
+void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
+  const double t0 = now_s();
+  const uint32_t k = (uint32_t)batch.size();
+  std::vector<uint64_t> voff(k + 1, 0), eoff(k + 1, 0), hoff(k + 1, 0);
+  for (uint32_t g = 0; g < k; g++) {
+    const phem_request &q = batch[g]->q;
+    voff[g + 1] = voff[g] + q.n;
+    eoff[g + 1] = eoff[g] + q.m;
+    hoff[g + 1] = hoff[g] + ((q.flags & PHEM_REQ_DEVICE) ? 0 : q.m);
+  }
+  const uint64_t n = voff[k], m = eoff[k], mh = hoff[k];
+  uint64_t ws = 0;
+  int rc = batch_workspace_bytes(n, m, k, &ws);
+  // carve the slot's device memory
+  char *base = (char *)S.dev;
+  uint64_t off = 0;
+  auto take = [&](uint64_t bytes) {
+    char *p = base + off;
+    off += a256(bytes);
+    return p;
+  };
+  char *d_ws = take(ws);
+  uint32_t *d_su = (uint32_t *)take(4 * mh), *d_sv = (uint32_t *)take(4 * mh),
+           *d_sw = (uint32_t *)take(4 * mh);
+  uint32_t *d_ids = (uint32_t *)take(4 * n), *d_lab = (uint32_t *)take(4 * n);
+  uint32_t *d_cnt = (uint32_t *)take(4ull * k);
+  uint64_t *d_wfx = (uint64_t *)take(8ull * k), *d_bad = (uint64_t *)take(8ull * k);
+  // host inputs: into pinned staging (several threads for large batches),
+  // then one copy per field
+  std::vector<const uint32_t *> pu(k), pv(k), pw(k);
+  if (rc == PHEM_OK && mh) {
+    uint32_t *hu = S.hin, *hv = S.hin + mh, *hw = S.hin + 2 * mh;
+    auto stage = [&](uint32_t g0, uint32_t g1) {
+      for (uint32_t g = g0; g < g1; g++) {
+        const phem_request &q = batch[g]->q;
+        if ((q.flags & PHEM_REQ_DEVICE) || !q.m) continue;
+        std::memcpy(hu + hoff[g], q.u, 4 * q.m);
+        std::memcpy(hv + hoff[g], q.v, 4 * q.m);
+        std::memcpy(hw + hoff[g], q.w, 4 * q.m);
+      }
+    };
+    const uint64_t bytes = 12 * mh;
+    const unsigned nth = bytes < (8u << 20) ? 1u : 4u;
+    if (nth == 1 || k == 1) {
+      stage(0, k);
+    } else {
+      // contiguous runs of graphs with about the same edge count per thread
+      std::vector<std::thread> ts;
+      uint32_t g = 0;
+      for (unsigned t = 0; t < nth; t++) {
+        const uint64_t target = mh * (t + 1) / nth;
+        uint32_t h = (t + 1 == nth) ? k : g;
+        while (h < k && hoff[h] < target) h++;
+        if (h > g) {
+          ts.emplace_back(stage, g, h);
+          g = h;
+        }
+      }
+      for (auto &t : ts) t.join();
+    }
+    hipError_t e = hipMemcpyAsync(d_su, hu, 4 * mh, hipMemcpyHostToDevice, S.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_sv, hv, 4 * mh, hipMemcpyHostToDevice, S.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_sw, hw, 4 * mh, hipMemcpyHostToDevice, S.stream);
+    if (e != hipSuccess) rc = set_err(PHEM_EHIP, "staging copy: %s", hipGetErrorString(e));
+  }
+  for (uint32_t g = 0; g < k; g++) {
+    const phem_request &q = batch[g]->q;
+    if (q.flags & PHEM_REQ_DEVICE) {
+      pu[g] = q.u;
+      pv[g] = q.v;
+      pw[g] = q.w;
+    } else {
+      pu[g] = d_su + hoff[g];
+      pv[g] = d_sv + hoff[g];
+      pw[g] = d_sw + hoff[g];
+    }
+  }
+  phem_mst_stats st{};
+  if (rc == PHEM_OK) {
+    BatchArgs A{};
+    A.k = k;
+    A.h_voff = voff.data();
+    A.h_eoff = eoff.data();
+    A.h_pu = pu.data();
+    A.h_pv = pv.data();
+    A.h_pw = pw.data();
+    A.wmode = o.wmode;
+    A.d_ws = d_ws;
+    A.ws_bytes = ws;
+    A.d_ids = d_ids;
+    A.d_counts = d_cnt;
+    A.d_labels = d_lab;
+    A.d_weights = d_wfx;
+    A.d_inexact = d_bad;
+    A.stats = &st;
+    rc = batch_solve(A, S.stream);
+  }
+  // results: one copy per field into the pinned buffer, then to each request
+  char *h = S.hout;
+  uint32_t *h_cnt = (uint32_t *)h;
+  uint64_t *h_wfx = (uint64_t *)(h + a256(4ull * k));
+  uint64_t *h_bad = (uint64_t *)((char *)h_wfx + a256(8ull * k));
+  uint32_t *h_ids = (uint32_t *)((char *)h_bad + a256(8ull * k));
+  uint32_t *h_lab = (uint32_t *)((char *)h_ids + a256(4 * n));
+  if (rc == PHEM_OK) {
+    hipError_t e = hipMemcpyAsync(h_cnt, d_cnt, 4ull * k, hipMemcpyDeviceToHost, S.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h_wfx, d_wfx, 8ull * k, hipMemcpyDeviceToHost, S.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h_bad, d_bad, 8ull * k, hipMemcpyDeviceToHost, S.stream);
+    if (e == hipSuccess && n)
+      e = hipMemcpyAsync(h_ids, d_ids, 4 * n, hipMemcpyDeviceToHost, S.stream);
+    if (e == hipSuccess && n)
+      e = hipMemcpyAsync(h_lab, d_lab, 4 * n, hipMemcpyDeviceToHost, S.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(S.stream);
+    if (e != hipSuccess) rc = set_err(PHEM_EHIP, "result copy: %s", hipGetErrorString(e));
+  }
+  if (rc == PHEM_OK) {
+    for (uint32_t g = 0; g < k; g++) {
+      const phem_request &q = batch[g]->q;
+      if (q.forest_ids && h_cnt[g])
+        std::memcpy(q.forest_ids, h_ids + voff[g], 4ull * h_cnt[g]);
+      if (q.labels && q.n) std::memcpy(q.labels, h_lab + voff[g], 4ull * q.n);
+    }
+  } else {
+    (void)hipStreamSynchronize(S.stream);
+  }
+  const double t1 = now_s();
+  std::lock_guard<std::mutex> lk(mu);
+  for (uint32_t g = 0; g < k; g++) {
+    Req *r = batch[g];
+    r->res.status = rc;
+    r->res.forest_count = rc == PHEM_OK ? h_cnt[g] : 0;
+    r->res.weight_fx = rc == PHEM_OK ? h_wfx[g] : 0;
+    r->res.weight_exact = rc == PHEM_OK && h_bad[g] == 0;
+    r->res.batch_graphs = k;
+    r->res.start = t0;
+    r->res.done = t1;
+    r->done = true;
+    if (rc == PHEM_OK) {
+      if (completed == 0 || r->res.arrival < first_arrival)
+        first_arrival = r->res.arrival;
+      completed++;
+      edges += r->q.m;
+      lat.push_back(t1 - r->res.arrival);
+    }
+  }
+  if (rc == PHEM_OK) {
+    last_done = std::max(last_done, t1);
+    batches++;
+    batch_graphs += k;
+  }
+  in_flight -= k;
+  cv_done.notify_all();
+}
+
+extern "C" double phem_server_clock(void) { return now_s(); }
+
+extern "C" int phem_server_create(const phem_server_opts *opts,
+                                  phem_server **out) {
+  if (!out) return set_err(PHEM_EINVAL, "out is NULL");
+  *out = nullptr;
+  phem_server_opts o{};
+  if (opts) o = *opts;
+  if (o.wmode != PHEM_W_U32 && o.wmode != PHEM_W_F32)
+    return set_err(PHEM_EINVAL, "unknown weight mode %d", o.wmode);
+  if (!o.slots) o.slots = 2;
+  if (o.slots > 16) return set_err(PHEM_EINVAL, "at most 16 slots");
+  if (!o.max_graphs) o.max_graphs = 4096;
+  if (!o.max_batch_edges) o.max_batch_edges = 1ull << 27;
+  if (o.max_batch_edges > PHEM_MAX_EDGES)
+    return set_err(PHEM_EINVAL, "max_batch_edges beyond %llu",
+                   (unsigned long long)PHEM_MAX_EDGES);
+  int dev = 0;
+  PHEM_HIP(hipGetDevice(&dev));
+  if (!o.mem_bytes) {
+    size_t fr = 0, tot = 0;
+    PHEM_HIP(hipMemGetInfo(&fr, &tot));
+    o.mem_bytes = fr / 4;
+  }
+  std::unique_ptr<phem_server> S(new phem_server());
+  S->dev = dev;
+  S->o = o;
+  S->slot_bytes = o.mem_bytes / o.slots / 256 * 256;
+  S->slots.resize(o.slots);
+  for (auto &sl : S->slots) {
+    hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&sl.dev, S->slot_bytes);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void **)&sl.hin, 12 * o.max_batch_edges);
+    sl.hout_bytes = a256(4ull * o.max_graphs) + 2 * a256(8ull * o.max_graphs) +
+                    2 * a256(4 * o.max_batch_edges);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.hout, sl.hout_bytes);
+    if (e != hipSuccess) {
+      int rc = set_err(PHEM_ENOMEM, "slot allocation (%llu device bytes): %s",
+                       (unsigned long long)S->slot_bytes, hipGetErrorString(e));
+      phem_server_destroy(S.release());
+      return rc;
+    }
+  }
+  for (int i = 0; i < (int)o.slots; i++)
+    S->slots[i].th = std::thread(&phem_server::worker, S.get(), i);
+  *out = S.release();
+  return PHEM_OK;
+}
+
+extern "C" int phem_server_destroy(phem_server *srv) {
+  if (!srv) return PHEM_OK;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stop = true;
+  }
+  srv->cv_work.notify_all();
+  for (auto &sl : srv->slots)
+    if (sl.th.joinable()) sl.th.join();
+  for (auto &sl : srv->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.dev) (void)hipFree(sl.dev);
+    if (sl.hin) (void)hipHostFree(sl.hin);
+    if (sl.hout) (void)hipHostFree(sl.hout);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  delete srv;
+  return PHEM_OK;
+}
+
+extern "C" int phem_server_submit(phem_server *srv, const phem_request *req,
+                                  uint64_t *ticket) {
+  if (!srv || !req || !ticket) return set_err(PHEM_EINVAL, "NULL argument");
+  if (req->m && (!req->u || !req->v || !req->w))
+    return set_err(PHEM_EINVAL, "request has edges but NULL arrays");
+  const double t = req->arrival >= 0 ? req->arrival : now_s();
+  const uint64_t mh = (req->flags & PHEM_REQ_DEVICE) ? 0 : req->m;
+  std::unique_ptr<Req> r(new Req());
+  r->q = *req;
+  r->res = phem_result{};
+  r->res.arrival = t;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (srv->stop) return set_err(PHEM_EINVAL, "server is shutting down");
+  if (!srv->fits(req->n, req->m, mh, 1)) {
+    srv->refused++;
+    return set_err(PHEM_EBUSY,
+                   "request of %u vertices, %llu edges exceeds one slot "
+                   "(%llu device bytes, %llu edges)", req->n,
+                   (unsigned long long)req->m,
+                   (unsigned long long)srv->slot_bytes,
+                   (unsigned long long)srv->o.max_batch_edges);
+  }
+  r->ticket = srv->next_ticket++;
+  *ticket = r->ticket;
+  srv->submitted++;
+  srv->queue.push_back(r.get());
+  srv->reqs.emplace(r->ticket, std::move(r));
+  srv->cv_work.notify_one();
+  return PHEM_OK;
+}
+
+extern "C" int phem_server_wait(phem_server *srv, uint64_t ticket,
+                                phem_result *res) {
+  if (!srv) return set_err(PHEM_EINVAL, "NULL server");
+  std::unique_lock<std::mutex> lk(srv->mu);
+  auto it = srv->reqs.find(ticket);
+  if (it == srv->reqs.end())
+    return set_err(PHEM_ENOTFOUND, "unknown ticket %llu", (unsigned long long)ticket);
+  Req *r = it->second.get();
+  srv->cv_done.wait(lk, [&] { return r->done; });
+  if (res) *res = r->res;
+  srv->reqs.erase(it);
+  return PHEM_OK;
+}
+
+extern "C" int phem_server_poll(phem_server *srv, uint64_t ticket,
+                                phem_result *res, int *done) {
+  if (!srv || !done) return set_err(PHEM_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->reqs.find(ticket);
+  if (it == srv->reqs.end())
+    return set_err(PHEM_ENOTFOUND, "unknown ticket %llu", (unsigned long long)ticket);
+  *done = it->second->done ? 1 : 0;
+  if (*done) {
+    if (res) *res = it->second->res;
+    srv->reqs.erase(it);
+  }
+  return PHEM_OK;
+}
+
+extern "C" int phem_server_drain(phem_server *srv) {
+  if (!srv) return set_err(PHEM_EINVAL, "NULL server");
+  std::unique_lock<std::mutex> lk(srv->mu);
+  srv->cv_done.wait(lk, [&] { return srv->queue.empty() && srv->in_flight == 0; });
+  return PHEM_OK;
+}
+
+extern "C" int phem_server_get_stats(phem_server *srv, phem_server_stats *st) {
+  if (!srv || !st) return set_err(PHEM_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  phem_server_stats s{};
+  s.submitted = srv->submitted;
+  s.completed = srv->completed;
+  s.refused = srv->refused;
+  s.batches = srv->batches;
+  s.edges = srv->edges;
+  s.first_arrival = srv->first_arrival;
+  s.last_done = srv->last_done;
+  if (!srv->lat.empty()) {
+    std::vector<double> l = srv->lat;
+    std::sort(l.begin(), l.end());
+    // nearest-rank percentiles
+    auto pct = [&](double p) {
+      size_t i = (size_t)std::ceil(p * l.size());
+      return l[i ? i - 1 : 0];
+    };
+    s.lat_p50 = pct(0.50);
+    s.lat_p99 = pct(0.99);
+    s.lat_max = l.back();
+    double sum = 0;
+    for (double x : l) sum += x;
+    s.lat_mean = sum / l.size();
+  }
+  s.mean_batch_graphs = srv->batches ? (double)srv->batch_graphs / srv->batches : 0;
+  s.slot_bytes = srv->slot_bytes;
+  s.slot_edges = srv->o.max_batch_edges;
+  *st = s;
+  return PHEM_OK;
+}
+
+extern "C" int phem_server_reset_stats(phem_server *srv) {
+  if (!srv) return set_err(PHEM_EINVAL, "NULL server");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  srv->submitted = srv->completed = srv->refused = srv->batches = srv->edges = 0;
+  srv->batch_graphs = 0;
+  srv->first_arrival = srv->last_done = 0;
+  srv->lat.clear();
+  return PHEM_OK;
+}

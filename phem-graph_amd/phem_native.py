@@ -21,7 +21,13 @@ RMAT_T = (2448131359, 3264175145, 4080218931)
 
 
 class PhemError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+EINVAL, ENOMEM, EHIP, ENOCONV, ENODEV, EBUSY, ENOTFOUND = -1, -2, -3, -4, -5, -6, -7
+REQ_DEVICE = 1
 
 
 class MstOpts(ctypes.Structure):
@@ -55,6 +61,41 @@ class MstStats(ctypes.Structure):
                     light_rounds=self.light_rounds,
                     m_log=list(self.m_log[:r]),
                     c_log=list(self.c_log[:r]))
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("mem_bytes", ctypes.c_uint64), ("max_batch_edges", ctypes.c_uint64),
+                ("slots", ctypes.c_uint32), ("max_graphs", ctypes.c_uint32),
+                ("window_us", ctypes.c_uint32), ("wmode", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("m", ctypes.c_uint64),
+                ("u", ctypes.c_void_p), ("v", ctypes.c_void_p), ("w", ctypes.c_void_p),
+                ("forest_ids", ctypes.c_void_p), ("labels", ctypes.c_void_p),
+                ("arrival", ctypes.c_double)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("forest_count", ctypes.c_uint32),
+                ("weight_fx", ctypes.c_uint64), ("weight_exact", ctypes.c_int32),
+                ("batch_graphs", ctypes.c_uint32), ("arrival", ctypes.c_double),
+                ("start", ctypes.c_double), ("done", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in
+                ("submitted", "completed", "refused", "batches", "edges")] + \
+               [(f, ctypes.c_double) for f in
+                ("first_arrival", "last_done", "lat_p50", "lat_p99", "lat_max", "lat_mean",
+                 "mean_batch_graphs")] + \
+               [("slot_bytes", ctypes.c_uint64), ("slot_edges", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 _lib = None
@@ -101,6 +142,19 @@ def lib():
         "phem_batch_solve": (i32, [u32, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64),
                                    vp, vp, vp, i32, vp, u64, vp, vp, vp, vp, vp,
                                    c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
+        "phem_batch_solve_ptrs": (i32, [u32, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64),
+                                        c.POINTER(vp), c.POINTER(vp), c.POINTER(vp),
+                                        i32, vp, u64, vp, vp, vp, vp, vp,
+                                        c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
+        "phem_server_clock": (c.c_double, []),
+        "phem_server_create": (i32, [c.POINTER(ServerOpts), c.POINTER(vp)]),
+        "phem_server_destroy": (i32, [vp]),
+        "phem_server_submit": (i32, [vp, c.POINTER(Request), c.POINTER(c.c_uint64)]),
+        "phem_server_wait": (i32, [vp, u64, c.POINTER(Result)]),
+        "phem_server_poll": (i32, [vp, u64, c.POINTER(Result), c.POINTER(c.c_int)]),
+        "phem_server_drain": (i32, [vp]),
+        "phem_server_get_stats": (i32, [vp, c.POINTER(ServerStats)]),
+        "phem_server_reset_stats": (i32, [vp]),
         "phem_prof_enable": (i32, [i32]),
         "phem_prof_reset": (i32, []),
         "phem_prof_count": (i32, []),
@@ -118,7 +172,7 @@ def lib():
 def _check(rc, what):
     if rc != 0:
         msg = lib().phem_last_error().decode(errors="replace")
-        raise PhemError(f"{what} returned {rc}: {msg}")
+        raise PhemError(f"{what} returned {rc}: {msg}", rc)
 
 
 def require_device():
@@ -370,3 +424,83 @@ class Stream:
         lab = torch.empty(max(self.n, 1), dtype=torch.int32, device=self.device)
         _check(lib().phem_stream_labels(self.h, _p(lab), _stream()), "phem_stream_labels")
         return lab[: self.n]
+
+
+def _host_ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+class Server:
+    """Request loop with continuous batching (phem_server_*).
+
+    submit() takes a graph either as int32 device tensors (u, v, w) or as
+    numpy uint32 host arrays, plus optional numpy uint32 host arrays that
+    receive the forest edge ids (n slots) and labels (n slots).  The arrays
+    are kept referenced until the request is waited for."""
+
+    def __init__(self, mem_bytes=0, slots=2, max_graphs=4096, max_batch_edges=0,
+                 window_us=0, wmode=W_F32):
+        o = ServerOpts(mem_bytes, max_batch_edges, slots, max_graphs, window_us, wmode)
+        h = ctypes.c_void_p()
+        _check(lib().phem_server_create(ctypes.byref(o), ctypes.byref(h)), "phem_server_create")
+        self.h = h
+        self.keep = {}
+
+    @staticmethod
+    def clock():
+        return lib().phem_server_clock()
+
+    def submit(self, n, u, v, w, forest_ids=None, labels=None, arrival=-1.0):
+        dev = isinstance(u, torch.Tensor)
+        if dev:
+            ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t.numel() else None
+            m = u.numel()
+        else:
+            ptr = _host_ptr
+            m = u.size
+        r = Request(n, REQ_DEVICE if dev else 0, m, ptr(u), ptr(v), ptr(w),
+                    _host_ptr(forest_ids), _host_ptr(labels), arrival)
+        t = ctypes.c_uint64(0)
+        _check(lib().phem_server_submit(self.h, ctypes.byref(r), ctypes.byref(t)),
+               "phem_server_submit")
+        self.keep[t.value] = (u, v, w, forest_ids, labels)
+        return t.value
+
+    def wait(self, ticket):
+        res = Result()
+        _check(lib().phem_server_wait(self.h, ticket, ctypes.byref(res)), "phem_server_wait")
+        self.keep.pop(ticket, None)
+        return res.as_dict()
+
+    def poll(self, ticket):
+        res = Result()
+        done = ctypes.c_int(0)
+        _check(lib().phem_server_poll(self.h, ticket, ctypes.byref(res), ctypes.byref(done)),
+               "phem_server_poll")
+        if done.value:
+            self.keep.pop(ticket, None)
+            return res.as_dict()
+        return None
+
+    def drain(self):
+        _check(lib().phem_server_drain(self.h), "phem_server_drain")
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().phem_server_get_stats(self.h, ctypes.byref(st)), "phem_server_get_stats")
+        return st.as_dict()
+
+    def reset_stats(self):
+        _check(lib().phem_server_reset_stats(self.h), "phem_server_reset_stats")
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib().phem_server_destroy(self.h)
+        self.h = None
+        self.keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,171 @@
+"""GPU tests of the serving loop (phem_server_*): every request's result in a
+mixed, concurrently batched stream equals the result of the same request
+served alone and the oracle's Kruskal forest, labels and exact weight; a
+request larger than a slot is refused; statistics add up."""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import synth as S
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def P():
+    import phem_native
+    phem_native.require_device()
+    return phem_native
+
+
+def host_graph(g, seed=3):
+    """Small graphs of the serving mix's shapes (sizes scaled down)."""
+    rng = np.random.default_rng(seed * 1000 + g)
+    kind = g % 3
+    if kind == 0:
+        sc = int(rng.integers(3, 11))
+        u, v, w = S.rmat(sc, int(rng.integers(1, 16)) << (sc - 1), S.stream_seed(seed, g))
+        return 1 << sc, u, v, w
+    if kind == 1:
+        n = int(rng.integers(1, 2000))
+        u, v, w = S.uniform(n, int(rng.integers(0, 8 * n)), S.stream_seed(seed, g))
+        return n, u, v, w
+    side = int(rng.integers(2, 30))
+    u, v, w = S.grid(side, S.stream_seed(seed, g))
+    return side * side, u, v, w
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).cuda()
+
+
+def submit(srv, n, u, v, w, on_device, arrival=-1.0):
+    ids = np.zeros(max(n, 1), np.uint32)
+    lab = np.zeros(max(n, 1), np.uint32)
+    if on_device:
+        t = srv.submit(n, dev(u), dev(v), dev(w), ids, lab, arrival)
+    else:
+        t = srv.submit(n, u, v, w, ids, lab, arrival)
+    return t, ids, lab
+
+
+def test_mix_equals_alone_and_oracle(P):
+    graphs = [host_graph(g) for g in range(120)]
+    # served alone: one request at a time through a one-graph server
+    alone = []
+    solo = P.Server(mem_bytes=256 << 20, slots=1, max_graphs=1)
+    for n, u, v, w in graphs:
+        t, ids, lab = submit(solo, n, u, v, w, on_device=False)
+        r = solo.wait(t)
+        assert r["status"] == 0 and r["batch_graphs"] == 1
+        alone.append((ids[: r["forest_count"]].copy(), lab[:n].copy(), r))
+    solo.close()
+    # the mix: all submitted at once, alternating host and device inputs, from
+    # two submitting threads, several batches in flight
+    srv = P.Server(mem_bytes=512 << 20, slots=3, max_graphs=16, window_us=200)
+    out = [None] * len(graphs)
+
+    def feed(idx):
+        for g in idx:
+            n, u, v, w = graphs[g]
+            out[g] = submit(srv, n, u, v, w, on_device=(g % 2 == 0))
+
+    th = [threading.Thread(target=feed, args=(range(k, len(graphs), 2),)) for k in (0, 1)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    batched = 0
+    for g, (n, u, v, w) in enumerate(graphs):
+        t, ids, lab = out[g]
+        r = srv.wait(t)
+        assert r["status"] == 0
+        batched += r["batch_graphs"] > 1
+        got_ids = ids[: r["forest_count"]]
+        a_ids, a_lab, a_r = alone[g]
+        assert np.array_equal(got_ids, a_ids), f"graph {g}"
+        assert np.array_equal(lab[:n], a_lab), f"graph {g}"
+        assert (r["weight_fx"], r["weight_exact"]) == (a_r["weight_fx"], a_r["weight_exact"])
+        k_ids, k_lab = O.kruskal(n, u, v, w, 1)
+        assert np.array_equal(got_ids, k_ids), f"graph {g}"
+        assert np.array_equal(lab[:n], k_lab), f"graph {g}"
+        ws, exact = O.weight_sum(k_ids, w, 1)
+        assert bool(r["weight_exact"]) == exact
+        if exact:
+            assert r["weight_fx"] == ws
+        assert r["arrival"] <= r["start"] <= r["done"]
+    assert batched > 0, "no request shared a batch"
+    st = srv.stats()
+    assert st["completed"] == st["submitted"] == len(graphs)
+    assert st["edges"] == sum(len(gr[1]) for gr in graphs)
+    assert 0 < st["lat_p50"] <= st["lat_p99"] <= st["lat_max"]
+    assert st["mean_batch_graphs"] > 1
+    srv.close()
+
+
+def test_oversized_request_refused(P):
+    srv = P.Server(mem_bytes=64 << 20, slots=1, max_graphs=8)
+    n, m = 1 << 16, 1 << 22
+    u, v, w = S.uniform(n, m, 1)
+    with pytest.raises(P.PhemError) as e:
+        srv.submit(n, u, v, w)
+    assert e.value.code == P.EBUSY
+    assert srv.stats()["refused"] == 1
+    # a small one still goes through
+    n2, u2, v2, w2 = host_graph(1)
+    t, ids, lab = submit(srv, n2, u2, v2, w2, on_device=False)
+    r = srv.wait(t)
+    k_ids, _ = O.kruskal(n2, u2, v2, w2, 1)
+    assert np.array_equal(ids[: r["forest_count"]], k_ids)
+    srv.close()
+
+
+def test_admission_by_footprint_splits_batches(P):
+    """With room for only a few graphs per slot, a burst is served in many
+    batches, each within the slot's memory, all correct."""
+    graphs = [host_graph(g, seed=5) for g in range(40)]
+    srv = P.Server(mem_bytes=36 << 20, slots=2, max_graphs=4096)
+    out = [submit(srv, n, u, v, w, on_device=False) for n, u, v, w in graphs]
+    srv.drain()
+    for g, (n, u, v, w) in enumerate(graphs):
+        t, ids, lab = out[g]
+        r = srv.poll(t)
+        assert r is not None and r["status"] == 0
+        k_ids, k_lab = O.kruskal(n, u, v, w, 1)
+        assert np.array_equal(ids[: r["forest_count"]], k_ids)
+        assert np.array_equal(lab[:n], k_lab)
+    st = srv.stats()
+    assert st["batches"] > 2
+    srv.close()
+
+
+def test_unknown_ticket(P):
+    srv = P.Server(mem_bytes=16 << 20, slots=1)
+    with pytest.raises(P.PhemError) as e:
+        srv.wait(12345)
+    assert e.value.code == P.ENOTFOUND
+    srv.close()
+
+
+def test_u32_weights_server(P):
+    srv = P.Server(mem_bytes=64 << 20, slots=2, max_graphs=64, wmode=P.W_U32)
+    rng = np.random.default_rng(2)
+    reqs = []
+    for g in range(20):
+        n = int(rng.integers(2, 500))
+        m = int(rng.integers(0, 3000))
+        u = rng.integers(0, n, m).astype(np.uint32)
+        v = rng.integers(0, n, m).astype(np.uint32)
+        w = rng.integers(0, 8, m).astype(np.uint32)  # heavy ties
+        reqs.append((n, u, v, w) + submit(srv, n, u, v, w, on_device=False))
+    for n, u, v, w, t, ids, lab in reqs:
+        r = srv.wait(t)
+        k_ids, k_lab = O.kruskal(n, u, v, w, 0)
+        assert np.array_equal(ids[: r["forest_count"]], k_ids)
+        assert np.array_equal(lab[:n], k_lab)
+        assert r["weight_fx"] == int(w[k_ids].astype(np.uint64).sum())
+    srv.close()
