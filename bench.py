@@ -90,6 +90,111 @@ def cpu_baseline(u_np, v_np, w_np, n, threads, runs):
     return len(u_np) / (mst_s + cc_s), mst_s, cc_s
 
 
+def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load):
+    """The serving mix (graphs_n independent graphs of the mix, device
+    resident): one-at-a-time plain solves, then the serving loop saturated
+    (all requests at once), then Poisson arrivals at `load` x the saturated
+    request rate.  Every request returns forest ids, labels and the exact
+    weight to host memory; latency = done - arrival."""
+    import numpy as np
+    import phem_workloads as W
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    graphs = [W.mix_graph(seed, g, device=dev) for g in range(graphs_n)]
+    torch.cuda.synchronize()
+    edges = sum(g[1].numel() for g in graphs)
+    nmax = max(g[0] for g in graphs)
+    mmax = max(g[1].numel() for g in graphs)
+    print(f"# serving mix: {graphs_n} graphs, {edges} edges, generated in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    out_ids = [np.zeros(g[0], np.uint32) for g in graphs]
+    out_lab = [np.zeros(g[0], np.uint32) for g in graphs]
+
+    # plain: one graph at a time through phem_mst_solve, results to host
+    solver = P.MstSolver(nmax, mmax, device=dev)
+    h_ids = torch.empty(nmax, dtype=torch.int32, pin_memory=True)
+    h_lab = torch.empty(nmax, dtype=torch.int32, pin_memory=True)
+    lat_plain = []
+    plain_w = []
+
+    def plain(n, u, v, w):
+        a = time.perf_counter()
+        ids, lab, st = solver.solve(u, v, w, P.W_F32, n=n)
+        k = st["mst_count"]
+        h_ids[:k].copy_(ids[:k], non_blocking=True)
+        h_lab[:n].copy_(lab[:n], non_blocking=True)
+        torch.cuda.synchronize()
+        lat_plain.append(time.perf_counter() - a)
+        return st
+
+    for g in graphs[:4]:
+        plain(*g)
+    lat_plain.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for g in graphs:
+        plain_w.append(plain(*g)["weight_fx"])
+    t_plain = time.perf_counter() - t0
+    print(f"# plain: {t_plain:.2f} s", file=sys.stderr, flush=True)
+    del solver
+
+    srv = P.Server(mem_bytes=0, slots=slots, max_graphs=4096,
+                   max_batch_edges=max_batch_edges, wmode=P.W_F32)
+
+    def run_served(arrivals):
+        srv.reset_stats()
+        tick = []
+        base = P.Server.clock()
+        for i, (n, u, v, w) in enumerate(graphs):
+            if arrivals is not None:
+                at = base + arrivals[i]
+                while P.Server.clock() < at:
+                    pass
+            else:
+                at = -1.0
+            tick.append(srv.submit(n, u, v, w, out_ids[i], out_lab[i], at))
+        res = [srv.wait(t) for t in tick]
+        st = srv.stats()
+        wall = st["last_done"] - st["first_arrival"]
+        return res, st, wall
+
+    run_served(None)  # warm-up: slot memory and pinned buffers
+    res, st, wall = run_served(None)
+    bad = sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res))
+    cap_eps = edges / wall
+    rate = load * graphs_n / wall  # requests per second
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, graphs_n))
+    res_p, st_p, wall_p = run_served(arrivals)
+    bad += sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res_p))
+    srv.close()
+    lp = sorted(lat_plain)
+    pct = lambda l, p: l[max(0, int(np.ceil(p * len(l))) - 1)]
+    return {
+        "workload": f"serve_mix_{graphs_n}", "graphs": graphs_n, "edges": edges,
+        "inputs": "device resident; results copied to host memory",
+        "plain": {"edges_per_s": edges / t_plain, "wall_s": round(t_plain, 4),
+                  "lat_p50_ms": round(pct(lp, .5) * 1e3, 3),
+                  "lat_p99_ms": round(pct(lp, .99) * 1e3, 3)},
+        "served_saturated": {"edges_per_s": cap_eps, "wall_s": round(wall, 4),
+                             "batches": st["batches"],
+                             "mean_batch_graphs": round(st["mean_batch_graphs"], 1),
+                             "lat_p50_ms": round(st["lat_p50"] * 1e3, 3),
+                             "lat_p99_ms": round(st["lat_p99"] * 1e3, 3)},
+        "served_poisson": {"load": load, "rate_req_per_s": round(rate, 1),
+                           "edges_per_s": edges / wall_p, "wall_s": round(wall_p, 4),
+                           "batches": st_p["batches"],
+                           "mean_batch_graphs": round(st_p["mean_batch_graphs"], 1),
+                           "lat_p50_ms": round(st_p["lat_p50"] * 1e3, 3),
+                           "lat_p99_ms": round(st_p["lat_p99"] * 1e3, 3)},
+        "speedup_saturated_vs_plain": round(cap_eps * t_plain / edges, 2),
+        "speedup_poisson_vs_plain": round(edges / wall_p * t_plain / edges, 2),
+        "slots": slots, "max_batch_edges": max_batch_edges,
+        "weights_mismatch_vs_plain": bad,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +206,13 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=5,
                     help="timed CPU baseline runs after one warm-up (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix run")
+    ap.add_argument("--serve-graphs", type=int, default=4096)
+    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-batch-log2", type=int, default=27,
+                    help="edges per served batch (log2)")
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="Poisson arrival rate as a fraction of the saturated rate")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this size before the timed region "
                          "(a known-bytes reference for hardware counters)")
@@ -248,6 +360,12 @@ def main():
                                          f"median of {args.cpu_runs} runs after a warm-up: "
                                          f"OpenMP PBBS filter-Kruskal MST {t_mst:.2f} s + "
                                          f"HAST union-find CC {t_cc:.2f} s"}
+    if not args.no_serve:
+        del du, dv, dw, de, solver, cc_par, cc_lab, cc_forest
+        torch.cuda.empty_cache()
+        out["serving"] = serve_bench(P, torch, args.serve_graphs, 5, args.serve_slots,
+                                     1 << args.serve_batch_log2, args.serve_load)
+
     print(json.dumps(out), flush=True)
 
 

@@ -2,7 +2,8 @@
 //
 // Requests wait in one arrival-ordered queue.  Each slot owns a worker thread,
 // a HIP stream, one block of device memory and pinned host staging buffers
-// sized at creation, so serving allocates nothing per request.  An idle worker
+// (the input one on the slot's first request from host memory), so serving
+// allocates nothing per request.  An idle worker
 // takes the queued requests in order while the batch's device footprint (the
 // batched solve's workspace, staged host inputs and outputs) fits its memory
 // and the graph and edge caps hold, runs them as one batched solve and copies
@@ -161,6 +162,13 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
   // host inputs: into pinned staging (several threads for large batches),
   // then one copy per field
   std::vector<const uint32_t *> pu(k), pv(k), pw(k);
+  if (rc == PHEM_OK && mh && !S.hin) {
+    hipError_t e = hipHostMalloc((void **)&S.hin, 12 * o.max_batch_edges);
+    if (e != hipSuccess) {
+      S.hin = nullptr;
+      rc = set_err(PHEM_ENOMEM, "pinned input staging: %s", hipGetErrorString(e));
+    }
+  }
   if (rc == PHEM_OK && mh) {
     uint32_t *hu = S.hin, *hv = S.hin + mh, *hw = S.hin + 2 * mh;
     auto stage = [&](uint32_t g0, uint32_t g1) {
@@ -321,8 +329,6 @@ extern "C" int phem_server_create(const phem_server_opts *opts,
   for (auto &sl : S->slots) {
     hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&sl.dev, S->slot_bytes);
-    if (e == hipSuccess)
-      e = hipHostMalloc((void **)&sl.hin, 12 * o.max_batch_edges);
     sl.hout_bytes = a256(4ull * o.max_graphs) + 2 * a256(8ull * o.max_graphs) +
                     2 * a256(4 * o.max_batch_edges);
     if (e == hipSuccess) e = hipHostMalloc((void **)&sl.hout, sl.hout_bytes);

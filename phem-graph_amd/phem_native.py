@@ -215,19 +215,22 @@ class MstSolver:
         self.ids = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
         self.labels = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
 
-    def solve(self, u, v, w, wmode=W_F32, labels=True, stream=None, light_factor=None):
-        """light_factor: None = library default; <= 0 = canonical Boruvka."""
+    def solve(self, u, v, w, wmode=W_F32, labels=True, stream=None, light_factor=None,
+              n=None):
+        """light_factor: None = library default; <= 0 = canonical Boruvka.
+        n: vertex count of this graph (default and maximum: the solver's)."""
         m = u.numel()
-        if m > self.m:
-            raise PhemError(f"m={m} exceeds solver capacity {self.m}")
+        n = self.n if n is None else n
+        if m > self.m or n > self.n:
+            raise PhemError(f"n={n} m={m} exceed solver capacity {self.n}, {self.m}")
         st = MstStats()
         opts = None if light_factor is None else ctypes.byref(MstOpts(light_factor, 0, 0))
-        _check(lib().phem_mst_solve(self.n, m, _p(u), _p(v), _p(w), wmode, _p(self.ws),
+        _check(lib().phem_mst_solve(n, m, _p(u), _p(v), _p(w), wmode, _p(self.ws),
                                     self.ws_bytes, _p(self.ids),
                                     _p(self.labels) if labels else None, opts,
                                     ctypes.byref(st), _stream(stream)), "phem_mst_solve")
         s = st.as_dict()
-        return self.ids[: s["mst_count"]], (self.labels[: self.n] if labels else None), s
+        return self.ids[: s["mst_count"]], (self.labels[:n] if labels else None), s
 
     def solve_aos(self, edges, wmode=W_F32, labels=True, stream=None, light_factor=None):
         """edges: device tensor of m x 3 32-bit words {u, v, w}."""
