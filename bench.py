@@ -195,6 +195,54 @@ def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load):
     }
 
 
+def stream_bench(P, torch, scale, m0_log2, batches, batch_log2, seed):
+    """Streaming forest: an RMAT graph (n = 2^scale, 2^m0_log2 edges) applied
+    as the first batch, then `batches` batches of 2^batch_log2 uniform
+    insertions (all generated on the device beforehand).  Per-batch latency is
+    the host time of phem_stream_apply (which returns with the forest and the
+    union-find labels updated).  At the end the resident forest is compared
+    with a from-scratch solve over every edge inserted."""
+    import numpy as np
+    import phem_workloads as W
+
+    dev = torch.device("cuda:0")
+    n, m0, mb = 1 << scale, 1 << m0_log2, 1 << batch_log2
+    base = P.gen_rmat(scale, m0, seed, device=dev)
+    bat = [P.gen_uniform(n, mb, W.stream_seed(seed, b), device=dev) for b in range(batches)]
+    torch.cuda.synchronize()
+    st = P.Stream(n, m0, wmode=P.W_F32, device=dev)
+    t0 = time.perf_counter()
+    st.apply(*base)
+    t_base = time.perf_counter() - t0
+    lat = []
+    t0 = time.perf_counter()
+    for b in range(batches):
+        a = time.perf_counter()
+        st.apply(*bat[b])
+        lat.append(time.perf_counter() - a)
+    t_all = time.perf_counter() - t0
+    s = st.stats()
+    fu, fv, fw, fg = st.forest()
+    gid = fg.cpu().numpy()
+    # every edge inserted, in insertion order -> one solve from scratch
+    cat = [torch.cat([base[i]] + [x[i] for x in bat]) for i in range(3)]
+    del bat
+    ids, lab, rs = P.mst_solve(n, cat[0], cat[1], cat[2], P.W_F32)
+    same = bool(np.array_equal(ids.cpu().numpy().view(np.uint32).astype(np.int64), gid)
+                and rs["weight_fx"] == s["weight_fx"]
+                and torch.equal(lab, st.labels()))
+    st.close()
+    lat.sort()
+    pct = lambda p: lat[max(0, int(np.ceil(p * len(lat))) - 1)]
+    return {"workload": f"stream_rmat_n2^{scale}_m2^{m0_log2}+{batches}x2^{batch_log2}",
+            "base_ms": round(t_base * 1e3, 3),
+            "batch_edges_per_s": batches * mb / t_all,
+            "batch_lat_p50_ms": round(pct(.5) * 1e3, 3),
+            "batch_lat_p99_ms": round(pct(.99) * 1e3, 3),
+            "forest_edges": len(gid), "edges_inserted": s["edges_inserted"],
+            "forest_equals_recompute": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +255,7 @@ def main():
                     help="timed CPU baseline runs after one warm-up (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix run")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streaming run")
     ap.add_argument("--serve-graphs", type=int, default=4096)
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-batch-log2", type=int, default=27,
@@ -365,6 +414,12 @@ def main():
         torch.cuda.empty_cache()
         out["serving"] = serve_bench(P, torch, args.serve_graphs, 5, args.serve_slots,
                                      1 << args.serve_batch_log2, args.serve_load)
+        torch.cuda.empty_cache()
+    if not args.no_stream:
+        if args.no_serve:
+            del du, dv, dw, de, solver, cc_par, cc_lab, cc_forest
+            torch.cuda.empty_cache()
+        out["streaming"] = stream_bench(P, torch, 22, 24, 256, 20, 4)
 
     print(json.dumps(out), flush=True)
 

@@ -64,3 +64,36 @@ def test_stream_rejects_oversized_batch(P):
     with pytest.raises(P.PhemError):
         st.apply(u, u, u)
     st.close()
+
+
+@pytest.mark.slow
+def test_stream_rmat_2e22_256_batches_vs_recompute(P):
+    """RMAT n=2^22, m=2^24 (weights [0,1)), then 256 batches of 2^20 uniform
+    insertions: at checkpoints the resident forest equals the OpenMP PBBS
+    restatement's forest of every edge so far (ids = insertion order), with
+    the same exact weight, and the labels equal the CPU union-find's."""
+    scale, m0, nb, mb = 22, 1 << 24, 256, 1 << 20
+    n = 1 << scale
+    print("generating rmat 2^22 / 2^24 and 256 batches", flush=True)
+    base = P.gen_rmat(scale, m0, 4)
+    st = P.Stream(n, m0, wmode=1)
+    st.apply(*base)
+    U, V, W = [host32(t).copy() for t in base]
+    U, V, W = [U], [V], [W]
+    checks = {1, 128, nb}
+    for b in range(1, nb + 1):
+        bu, bv, bw = P.gen_uniform(n, mb, S.stream_seed(4, b - 1))
+        st.apply(bu, bv, bw)
+        U.append(host32(bu).copy()), V.append(host32(bv).copy()), W.append(host32(bw).copy())
+        if b in checks:
+            au, av, aw = np.concatenate(U), np.concatenate(V), np.concatenate(W)
+            print(f"batch {b}: checking {len(au)} edges", flush=True)
+            p_ids = O.par_mst(n, au, av, aw, 1)
+            fu, fv, fw, fg = st.forest()
+            gid = fg.cpu().numpy().astype(np.uint64)
+            assert np.array_equal(gid, p_ids.astype(np.uint64)), f"batch {b}"
+            s = st.stats()
+            ws, exact = O.weight_sum(p_ids, aw, 1)
+            assert exact and s["weight_fx"] == ws
+            assert np.array_equal(host32(st.labels()), O.par_cc(n, au, av)), f"batch {b}"
+    st.close()
