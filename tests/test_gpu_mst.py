@@ -147,3 +147,46 @@ def test_mst_aos_layout_matches_soa(P, name):
     k_ids, k_lab = O.kruskal(n, u, v, w, wmode)
     assert np.array_equal(host(ids), k_ids)
     assert np.array_equal(host(lab), k_lab)
+
+
+def _full_check(P, n, u, v, w, canonical_log=False):
+    du, dv, dw = dev(u), dev(v), dev(w)
+    ids, lab, st = P.mst_solve(n, du, dv, dw, P.W_F32)
+    torch.cuda.synchronize()
+    d_ids, d_lab = host(ids).copy(), host(lab).copy()
+    p_ids = O.par_mst(n, u, v, w, 1)
+    assert np.array_equal(d_ids, p_ids)
+    ws, exact = O.weight_sum(p_ids, w, 1)
+    assert exact and st["weight_fx"] == ws
+    assert np.array_equal(d_lab, O.par_cc(n, u, v))
+    clab, _, _ = P.cc_solve(n, P.pack_edges(du, dv), forest=False)
+    assert np.array_equal(host(clab), d_lab)
+    if canonical_log:
+        c_ids, _, cst = P.mst_solve(n, du, dv, dw, P.W_F32, light_factor=0.0)
+        assert np.array_equal(host(c_ids), p_ids)
+        b = O.boruvka(n, u, v, w, 1)
+        assert cst["m_log"] == b["log_m"].tolist()
+        assert cst["c_log"] == b["log_c"].tolist()
+    return st
+
+
+@pytest.mark.slow
+def test_mst_grid_4096_tie_heavy(P):
+    """4096 x 4096 grid, 20% of edges deleted, integer weights 1..16 as
+    float32: massive ties resolved by edge id, many components."""
+    print("generating grid 4096^2", flush=True)
+    u, v, w = S.grid(4096, 3)
+    st = _full_check(P, 4096 * 4096, u, v, w)
+    print(f"grid: m={len(u)} forest={st['mst_count']} rounds={st['rounds']}", flush=True)
+
+
+@pytest.mark.slow
+def test_mst_path_2e22_chords_deep_rounds(P):
+    """Random-permuted path on 2^22 vertices plus 2^20 uniform chords: long
+    hook chains and many rounds; the canonical schedule's per-round log
+    equals the oracle's restatement of the reference round loop."""
+    n = 1 << 22
+    print("generating path 2^22 + 2^20 chords", flush=True)
+    u, v, w = S.path_chords(n, 1 << 20, 4)
+    st = _full_check(P, n, u, v, w, canonical_log=True)
+    print(f"path: m={len(u)} forest={st['mst_count']} rounds={st['rounds']}", flush=True)
