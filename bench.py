@@ -90,11 +90,12 @@ def cpu_baseline(u_np, v_np, w_np, n, threads, runs):
     return len(u_np) / (mst_s + cc_s), mst_s, cc_s
 
 
-def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load):
+def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_graphs=256):
     """The serving mix (graphs_n independent graphs of the mix, device
     resident): one-at-a-time plain solves, then the serving loop saturated
     (all requests at once), then Poisson arrivals at `load` x the saturated
-    request rate.  Every request returns forest ids, labels and the exact
+    request rate, then the first host_graphs graphs again from host memory
+    (all at once; the rate includes the host-to-device copies).  Every request returns forest ids, labels and the exact
     weight to host memory; latency = done - arrival."""
     import numpy as np
     import phem_workloads as W
@@ -168,6 +169,16 @@ def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, graphs_n))
     res_p, st_p, wall_p = run_served(arrivals)
     bad += sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res_p))
+    # host-resident inputs (the PCIe-inclusive rate) on a prefix of the mix
+    hk = min(host_graphs, graphs_n)
+    hg = [(n, u.cpu().numpy().view(np.uint32), v.cpu().numpy().view(np.uint32),
+           w.cpu().numpy().view(np.uint32)) for n, u, v, w in graphs[:hk]]
+    h_edges = sum(len(g[1]) for g in hg)
+    srv.reset_stats()
+    tick = [srv.submit(n, u, v, w, out_ids[i], out_lab[i]) for i, (n, u, v, w) in enumerate(hg)]
+    res_h = [srv.wait(t) for t in tick]
+    st_h = srv.stats()
+    bad += sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res_h))
     srv.close()
     lp = sorted(lat_plain)
     pct = lambda l, p: l[max(0, int(np.ceil(p * len(l))) - 1)]
@@ -188,6 +199,11 @@ def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load):
                            "mean_batch_graphs": round(st_p["mean_batch_graphs"], 1),
                            "lat_p50_ms": round(st_p["lat_p50"] * 1e3, 3),
                            "lat_p99_ms": round(st_p["lat_p99"] * 1e3, 3)},
+        "served_host_inputs": {"graphs": hk, "edges": h_edges,
+                               "edges_per_s": h_edges / (st_h["last_done"] - st_h["first_arrival"]),
+                               "batches": st_h["batches"],
+                               "lat_p50_ms": round(st_h["lat_p50"] * 1e3, 3),
+                               "lat_p99_ms": round(st_h["lat_p99"] * 1e3, 3)},
         "speedup_saturated_vs_plain": round(cap_eps * t_plain / edges, 2),
         "speedup_poisson_vs_plain": round(edges / wall_p * t_plain / edges, 2),
         "slots": slots, "max_batch_edges": max_batch_edges,
