@@ -31,7 +31,8 @@ def mst_kernel_bytes(st, m, n):
     Input passes read 12 B per input edge (u, v, w; the id is the index) and
     write 16 B (u', v', key) per edge they keep; a compaction round reads 16 B
     per alive edge and writes 16 B per survivor; flatten and labels read and
-    write the 4 B parent / label of every vertex."""
+    write the 4 B parent / label of every vertex; the giant-edge emission
+    reads the 8 B per-vertex minimum and writes 16 B per emitted edge."""
     ml, lr = st["m_log"], st["light_rounds"]
     R = len(ml)
     # survivors written by round r: the next round's alive count within a phase
@@ -44,7 +45,10 @@ def mst_kernel_bytes(st, m, n):
         b["k_compact"] -= 16 * ml[0] + 16 * out[0]
     else:
         b["k_light"] = 12 * m + 16 * st["light_edges"]
-        b["k_heavy"] = 12 * m + 16 * st["heavy_edges"]
+        g = st.get("giant_edges", 0)
+        b["k_heavy"] = 12 * m + 16 * (st["heavy_edges"] - g)
+        b["k_giant_fold"] = 12 * n
+        b["k_giant_emit"] = 8 * n + 16 * g
     return b
 
 
@@ -396,7 +400,7 @@ def main():
         "mst": {"ms": round(mst_ms, 4), "edges_per_s": m / (mst_ms * 1e-3),
                 "rounds": st["rounds"], "light_rounds": st["light_rounds"],
                 "forest": st["mst_count"], "light_edges": st["light_edges"],
-                "heavy_edges": st["heavy_edges"],
+                "heavy_edges": st["heavy_edges"], "giant_edges": st["giant_edges"],
                 "B_canonical_bytes": bm,
                 "roofline_frac_canonical_bytes": round(bm / (mst_ms * 1e-3) / 8e12, 4),
                 "B_actual_bytes": ba,

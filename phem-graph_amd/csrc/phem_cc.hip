@@ -45,7 +45,11 @@ template <bool FILTER>
 __global__ __launch_bounds__(CBLK) void k_cc_union(const ull *__restrict__ edges,
                                                    uint64_t m, uint32_t n,
                                                    uint32_t *par, ull *forest,
-                                                   const uint32_t *__restrict__ giant) {
+                                                   const uint32_t *__restrict__ giant,
+                                                   const uint32_t *gsum) {
+  __shared__ uint32_t s_sum[FILTER ? GSUM_WORDS : 1];
+  uint32_t covered = 0;
+  if (FILTER) covered = gsum_load<CBLK>(s_sum, gsum, n);
   for (uint64_t tile = blockIdx.x; tile * CTILE < m; tile += gridDim.x) {
     const uint64_t base = tile * CTILE + threadIdx.x;
     ull e[CEPT];
@@ -66,8 +70,8 @@ __global__ __launch_bounds__(CBLK) void k_cc_union(const ull *__restrict__ edges
       const uint32_t a = (uint32_t)(e[j] >> 32), b = (uint32_t)e[j];
       ok[j] = a < b && b < n;
       if (FILTER && ok[j])
-        ok[j] = !(((giant[a >> 5] >> (a & 31)) & 1u) &&
-                  ((giant[b >> 5] >> (b & 31)) & 1u));
+        ok[j] = !(giant_test(a, s_sum, covered, giant) &&
+                  giant_test(b, s_sum, covered, giant));
     }
 #pragma unroll
     for (int j = 0; j < CEPT; j++) {
@@ -132,7 +136,8 @@ extern "C" int phem_cc_union(uint32_t n, const uint64_t *d_edges, uint64_t m,
     PHEM_PROF("k_cc_union", (hipStream_t)stream);
     hipLaunchKernelGGL(k_cc_union<false>, dim3(grid_for(m, CTILE)), dim3(CBLK),
                        0, (hipStream_t)stream, (const ull *)d_edges, m, n,
-                       d_parent, (ull *)d_forest, (const uint32_t *)nullptr);
+                       d_parent, (ull *)d_forest, (const uint32_t *)nullptr,
+                       (const uint32_t *)nullptr);
   }
   PHEM_LAUNCHED("k_cc_union");
   return PHEM_OK;
@@ -181,7 +186,9 @@ extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
   }
   hipStream_t s = (hipStream_t)stream;
   uint32_t *giant_root = d_labels;  // 1 word
-  uint32_t *bits = d_labels + 2;    // ceil(n / 32) words <= n - 2
+  uint32_t *bits = d_labels + 2;    // ceil(n / 32) words
+  // summary: gsum_words(n) <= ceil(n / 1024) words; all fit in n for n >= 64
+  uint32_t *gsum = bits + (n + 31) / 32;
   rc = phem_cc_union(n, d_edges, m1, d_parent, d_forest, stream);
   if (rc) return rc;
   {
@@ -204,10 +211,18 @@ extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
   }
   PHEM_LAUNCHED("k_giant_bits");
   {
+    PHEM_PROF("k_giant_summary", s);
+    hipLaunchKernelGGL(k_giant_summary,
+                       dim3(grid_for(32ull * gsum_words(n), SPLIT_BLK)),
+                       dim3(SPLIT_BLK), 0, s, n, (const uint32_t *)bits, gsum);
+  }
+  PHEM_LAUNCHED("k_giant_summary");
+  {
     PHEM_PROF("k_cc_union_filtered", s);
     hipLaunchKernelGGL(k_cc_union<true>, dim3(grid_for(m - m1, CTILE)),
                        dim3(CBLK), 0, s, (const ull *)d_edges + m1, m - m1, n,
-                       d_parent, (ull *)d_forest, (const uint32_t *)bits);
+                       d_parent, (ull *)d_forest, (const uint32_t *)bits,
+                       (const uint32_t *)gsum);
   }
   PHEM_LAUNCHED("k_cc_union_filtered");
   return phem_cc_labels(n, d_parent, d_labels, stream);

@@ -58,8 +58,8 @@ constexpr int IPT = 4;         // components per thread in hook / compress
 constexpr int LTILE = BLK * IPT;
 
 // Counter area (ull), every counter on its own 128-byte line (CSTR ull):
-// scalars, then three blocks of KSEG partial sums (valid edges, weight sum,
-// inexact weights), then per round three segmented counters: E(r) alive edges
+// scalars, then four blocks of KSEG partial sums (valid edges, weight sum,
+// inexact weights, edges emitted by k_giant_emit), then per round three segmented counters: E(r) alive edges
 // at the start of round r, A(r) components active in round r, L(r) component
 // list for round r.
 constexpr int C_MSTCNT = 0 * CSTR;  // forest size
@@ -67,14 +67,14 @@ constexpr int C_TAU = 1 * CSTR;     // light / heavy key threshold
 constexpr int C_GIANT = 2 * CSTR;   // giant-component root (32-bit)
 constexpr int C_SUMS = 3 * CSTR;    // k_seg_sums output (up to 8 values)
 constexpr int C_PARTS = 4 * CSTR;
-constexpr int C_ROUND = C_PARTS + 3 * KSEG * CSTR;
+constexpr int C_ROUND = C_PARTS + 4 * KSEG * CSTR;
 constexpr int RSTRIDE = 3 * KSEG * CSTR;
 constexpr int NCTR = C_ROUND + RSTRIDE * (PHEM_MAX_ROUNDS + 1);
 
 __host__ __device__ constexpr int cE(int r) { return C_ROUND + RSTRIDE * r; }
 __host__ __device__ constexpr int cA(int r) { return cE(r) + KSEG * CSTR; }
 __host__ __device__ constexpr int cL(int r) { return cE(r) + 2 * KSEG * CSTR; }
-// which: 0 valid edges, 1 weight sum, 2 inexact weights
+// which: 0 valid edges, 1 weight sum, 2 inexact weights, 3 giant edges
 __host__ __device__ constexpr int cPart(int which) {
   return C_PARTS + which * KSEG * CSTR;
 }
@@ -161,7 +161,8 @@ struct SrcAoS {
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(BLK) void k_mst_init(uint32_t n, uint32_t *par,
-                                                  ull *best, uint64_t words,
+                                                  ull *best, ull *gmin,
+                                                  uint64_t words,
                                                   uint32_t *bitmap, ull *ctr) {
   uint64_t lim = n;
   if (words > lim) lim = words;
@@ -171,6 +172,7 @@ __global__ __launch_bounds__(BLK) void k_mst_init(uint32_t n, uint32_t *par,
     if (i < n) {
       par[i] = (uint32_t)i;
       best[i] = KEY_NONE;
+      if (gmin) gmin[i] = KEY_NONE;
     }
     if (i < words) bitmap[i] = 0u;
     if (i < (uint64_t)NCTR) ctr[i] = 0ull;
@@ -302,10 +304,13 @@ __global__ __launch_bounds__(BLK) void k_pass(
     const ull *tau_ptr, const uint32_t *__restrict__ giant,
     uint32_t *__restrict__ oa, uint32_t *__restrict__ ob,
     ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *valid_parts,
-    ull *best) {
+    ull *best, ull *gmin, const uint32_t *gsum) {
   __shared__ SegView sv;
+  __shared__ uint32_t s_sum[MODE == P_HEAVY ? GSUM_WORDS : 1];
   constexpr bool INPUT = MODE != P_EDGES;
   if (!INPUT) seg_load<TILE>(sv, in_cnt);
+  uint32_t covered = 0;
+  if (MODE == P_HEAVY) covered = gsum_load<BLK>(s_sum, gsum, n);
   const ull tau = (MODE == P_LIGHT || MODE == P_HEAVY) ? *tau_ptr : 0ull;
   const uint32_t ntiles_in = INPUT ? (uint32_t)ceil_div_dev(m0, TILE) : 0u;
   uint32_t nval = 0;  // valid input edges seen by this thread
@@ -366,13 +371,20 @@ __global__ __launch_bounds__(BLK) void k_pass(
       for (int j = 0; j < EPT; j++) val[j] = val[j] && key[j] < tau;
     }
     if (MODE == P_HEAVY) {
+      // both endpoints in the giant component: inside one component.  One
+      // endpoint x in it: of the parallel edges between x's component and
+      // the giant only the lightest can be in the forest (cycle property),
+      // so the edge is only offered to gmin[x]; k_giant_fold moves each
+      // vertex's minimum to its root and k_giant_emit turns each
+      // component's minimum into one alive edge.
 #pragma unroll
       for (int j = 0; j < EPT; j++) {
         val[j] = val[j] && key[j] >= tau;
         if (val[j]) {
-          const bool ga = (giant[a[j] >> 5] >> (a[j] & 31)) & 1u;
-          const bool gb = (giant[b[j] >> 5] >> (b[j] & 31)) & 1u;
-          val[j] = !(ga && gb);
+          const bool ga = giant_test(a[j], s_sum, covered, giant);
+          const bool gb = giant_test(b[j], s_sum, covered, giant);
+          if (ga != gb) min_store(gmin, ga ? b[j] : a[j], key[j]);
+          val[j] = !(ga || gb);
         }
       }
     }
@@ -421,6 +433,75 @@ __global__ __launch_bounds__(BLK) void k_pass(
       for (int w = 0; w < BLK / WAVE; w++) tot += s_w[w];
       if (tot) atomicAdd(valid_parts + (blockIdx.x % KSEG) * CSTR, tot);
     }
+  }
+}
+
+// After the heavy pass: the minimum of every non-root vertex goes to its
+// root (par is flat), and the vertex's own entry is cleared.  Only root
+// entries are written by other threads, only non-root entries are read.
+__global__ __launch_bounds__(BLK) void k_giant_fold(
+    uint32_t n, const uint32_t *__restrict__ par, ull *__restrict__ gmin) {
+  for (uint64_t x = (uint64_t)blockIdx.x * BLK + threadIdx.x; x < n;
+       x += (uint64_t)gridDim.x * BLK) {
+    const ull k = gmin[x];
+    if (k == KEY_NONE) continue;
+    const uint32_t r = par[x];
+    if (r != x) {
+      min_store(gmin, r, k);
+      gmin[x] = KEY_NONE;
+    }
+  }
+}
+
+// After the heavy pass: every component x with an edge to the giant
+// component G becomes one alive edge (x, G, gmin[x]) offered to best[] of
+// both sides.  Tile v appends to segment v mod KSEG of the alive-edge list.
+__global__ __launch_bounds__(BLK) void k_giant_emit(
+    uint32_t n, ull *__restrict__ gmin, const uint32_t *giant_root,
+    uint32_t *__restrict__ oa, uint32_t *__restrict__ ob,
+    ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *best,
+    ull *parts) {
+  const uint32_t G = *giant_root;
+  const uint32_t ntiles = (uint32_t)ceil_div_dev(n, LTILE);
+  uint32_t emitted = 0;  // by this thread
+  for (uint32_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    bool keep[IPT];
+    ull key[IPT];
+    uint64_t slot[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+      key[j] = x < n ? gmin[x] : KEY_NONE;
+      keep[j] = key[j] != KEY_NONE;
+      if (keep[j]) {
+        emitted++;
+        min_store(best, (uint32_t)x, key[j]);
+        min_store(best, G, key[j]);
+      }
+    }
+    const uint32_t seg = v % KSEG;
+    block_append<BLK, IPT>(keep, slot, out_cnt + seg * CSTR,
+                           (uint64_t)seg * out_cap);
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      if (keep[j]) {
+        const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+        oa[slot[j]] = (uint32_t)x;
+        ob[slot[j]] = G;
+        ok[slot[j]] = key[j];
+      }
+    }
+  }
+  __shared__ ull s_w[BLK / WAVE];
+  ull x = emitted;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
+  if (lane_id() == 0) s_w[threadIdx.x / WAVE] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ull tot = 0;
+    for (int w = 0; w < BLK / WAVE; w++) tot += s_w[w];
+    if (tot) atomicAdd(parts + (blockIdx.x % KSEG) * CSTR, tot);
   }
 }
 
@@ -578,7 +659,7 @@ __global__ __launch_bounds__(WAVE) void k_seg_sums(const ull *ctr, SumReq q,
 // workspace layout
 
 struct MstLayout {
-  uint64_t par, best, listL, listA, ea_a, ea_b, ea_k, eb_a, eb_b, eb_k, bitmap,
+  uint64_t par, best, gmin, gsum, listL, listA, ea_a, ea_b, ea_k, eb_a, eb_b, eb_k, bitmap,
       gbits, ctr, blk, total;
   uint64_t words, nb, ecap, lcap;
 };
@@ -596,11 +677,16 @@ static MstLayout mst_layout(uint32_t n, uint64_t m) {
   L.nb = ceil_div(L.words, WPB);
   // Segment capacities: a producer pass over at most ceil(x / T) + KSEG tiles
   // sends at most ceil(ceil(x / T) / KSEG) + 1 tiles to one segment.
-  L.ecap = (ceil_div(ceil_div(m ? m : 1, TILE), KSEG) + 2) * TILE;
+  // The heavy pass and k_giant_emit (LTILE vertices per tile) append to the
+  // same list, so its segments hold both producers' worst case.
+  L.ecap = (ceil_div(ceil_div(m ? m : 1, TILE), KSEG) + 2) * TILE +
+           ceil_div(ceil_div(n ? n : 1, LTILE), KSEG) * LTILE;
   L.lcap = (ceil_div(ceil_div(n ? n : 1, LTILE), KSEG) + 2) * LTILE;
   const uint64_t E = L.ecap * KSEG, LN = L.lcap * KSEG;
   L.par = take(4ull * n);
   L.best = take(8ull * n);
+  L.gmin = take(8ull * n);
+  L.gsum = take(4ull * GSUM_WORDS);
   L.listL = take(4ull * LN);
   L.listA = take(4ull * LN);
   L.ea_a = take(4ull * E);
@@ -665,6 +751,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   char *ws = (char *)d_ws;
   uint32_t *par = (uint32_t *)(ws + L.par);
   ull *best = (ull *)(ws + L.best);
+  ull *gmin = (ull *)(ws + L.gmin);
+  uint32_t *gsum = (uint32_t *)(ws + L.gsum);
   uint32_t *listL = (uint32_t *)(ws + L.listL);
   uint32_t *listA = (uint32_t *)(ws + L.listA);
   uint32_t *Ea[2] = {(uint32_t *)(ws + L.ea_a), (uint32_t *)(ws + L.eb_a)};
@@ -685,7 +773,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   {
     PHEM_PROF("k_mst_init", s);
     hipLaunchKernelGGL(k_mst_init, dim3(grid_for(initw, BLK)), dim3(BLK), 0, s,
-                       n, par, best, L.words, bitmap, ctr);
+                       n, par, best, split ? gmin : (ull *)nullptr, L.words,
+                       bitmap, ctr);
   }
   PHEM_LAUNCHED("k_mst_init");
 
@@ -747,7 +836,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                            dim3(BLK), 0, s, src, Ea[cur], Eb[cur], Ek[cur],
                            ctr + cE(r), L.ecap, 0ull, n, wmode, par, noull, nou,
                            Ea[cur ^ 1], Eb[cur ^ 1], Ek[cur ^ 1],
-                           ctr + cE(r + 1), L.ecap, (ull *)nullptr, best);
+                           ctr + cE(r + 1), L.ecap, (ull *)nullptr, best,
+                           (ull *)nullptr, nou);
       }
       PHEM_LAUNCHED("k_compact");
       int rc = read_sums({{cE(r), KSEG}, {cA(r), KSEG}, {cE(r + 1), KSEG},
@@ -794,7 +884,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       hipLaunchKernelGGL((k_pass<P_ALL, Src>), dim3(grid_for(m, TILE)),
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
                          n, wmode, par, noull, nou, Ea[0], Eb[0], Ek[0],
-                         ctr + cE(1), L.ecap, ctr + cPart(0), best);
+                         ctr + cE(1), L.ecap, ctr + cPart(0), best,
+                         (ull *)nullptr, nou);
     }
     PHEM_LAUNCHED("k_compact<R0>");
     int rc = read_sums({{cPart(0), KSEG}, {cA(0), KSEG}, {cE(1), KSEG},
@@ -827,7 +918,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       hipLaunchKernelGGL((k_pass<P_LIGHT, Src>), dim3(grid_for(m, TILE)),
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
                          n, wmode, par, ctr + C_TAU, nou, Ea[0], Eb[0], Ek[0],
-                         ctr + cE(0), L.ecap, ctr + cPart(0), best);
+                         ctr + cE(0), L.ecap, ctr + cPart(0), best,
+                         (ull *)nullptr, nou);
     }
     PHEM_LAUNCHED("k_light");
     int rc = read_sums({{C_TAU, 1}, {cE(0), KSEG}}, hv);
@@ -861,17 +953,42 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     }
     PHEM_LAUNCHED("k_giant_bits");
     {
+      PHEM_PROF("k_giant_summary", s);
+      hipLaunchKernelGGL(k_giant_summary,
+                         dim3(grid_for(32ull * gsum_words(n), SPLIT_BLK)),
+                         dim3(SPLIT_BLK), 0, s, n, (const uint32_t *)gbits, gsum);
+    }
+    PHEM_LAUNCHED("k_giant_summary");
+    {
       PHEM_PROF("k_heavy", s);
       hipLaunchKernelGGL((k_pass<P_HEAVY, Src>), dim3(grid_for(m, TILE)),
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
                          n, wmode, par, ctr + C_TAU, gbits, Ea[cur], Eb[cur],
-                         Ek[cur], ctr + cE(r), L.ecap, (ull *)nullptr, best);
+                         Ek[cur], ctr + cE(r), L.ecap, (ull *)nullptr, best,
+                         gmin, gsum);
     }
     PHEM_LAUNCHED("k_heavy");
-    rc = read_sums({{cE(r), KSEG}}, hv);
+    {
+      PHEM_PROF("k_giant_fold", s);
+      hipLaunchKernelGGL(k_giant_fold, dim3(grid_for(n, BLK)), dim3(BLK), 0, s,
+                         n, par, gmin);
+    }
+    PHEM_LAUNCHED("k_giant_fold");
+    {
+      PHEM_PROF("k_giant_emit", s);
+      hipLaunchKernelGGL(k_giant_emit, dim3(grid_for(ceil_div(n, LTILE), 1)),
+                         dim3(BLK), 0, s, n, gmin,
+                         (const uint32_t *)(ctr + C_GIANT), Ea[cur], Eb[cur],
+                         Ek[cur], ctr + cE(r), L.ecap, best, ctr + cPart(3));
+    }
+    PHEM_LAUNCHED("k_giant_emit");
+    rc = read_sums({{cE(r), KSEG}, {cPart(3), KSEG}}, hv);
     if (rc) return rc;
     m_r = hv[0];
-    if (stats) stats->heavy_edges = m_r;
+    if (stats) {
+      stats->heavy_edges = m_r;
+      stats->giant_edges = hv[1];
+    }
     rc = run_rounds(0, true);
     if (rc) return rc;
   }

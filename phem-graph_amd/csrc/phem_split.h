@@ -109,5 +109,53 @@ __global__ __launch_bounds__(SPLIT_BLK) void k_giant_bits(uint32_t n,
   }
 }
 
+// Summary of the giant bitmap kept in LDS by the kernels that test it: bit
+// w of the summary = bitmap word w is all ones (its 32 vertices are all in
+// the giant component).  It covers the first GSUM_WORDS * 1024 vertices
+// (16 KB of LDS); a set bit answers the test without an L2 request, a clear
+// bit or a vertex beyond it falls back to the bitmap.
+constexpr uint32_t GSUM_WORDS = 4096;
+
+__host__ __device__ constexpr uint32_t gsum_words(uint32_t n) {
+  return ((uint64_t)n + 1023) / 1024 < GSUM_WORDS
+             ? (uint32_t)(((uint64_t)n + 1023) / 1024)
+             : GSUM_WORDS;
+}
+
+__global__ __launch_bounds__(SPLIT_BLK) void k_giant_summary(
+    uint32_t n, const uint32_t *__restrict__ bits, uint32_t *__restrict__ sum) {
+  const uint64_t words = ((uint64_t)n + 31) / 32;
+  const uint64_t cover = 32ull * gsum_words(n);  // bitmap words covered
+  for (uint64_t base = (uint64_t)blockIdx.x * SPLIT_BLK; base < cover;
+       base += (uint64_t)gridDim.x * SPLIT_BLK) {
+    const uint64_t w = base + threadIdx.x;
+    // a partial last word has clear tail bits, so it never reads as full
+    const uint64_t mask = __ballot(w < words && bits[w] == 0xFFFFFFFFu);
+    if (lane_id() == 0) {
+      const uint64_t k = w >> 5;  // w is a multiple of 64 for lane 0
+      if (k < cover / 32) sum[k] = (uint32_t)mask;
+      if (k + 1 < cover / 32) sum[k + 1] = (uint32_t)(mask >> 32);
+    }
+  }
+}
+
+// Loads the summary into LDS (every thread of the block calls it).
+template <int NT>
+__device__ __forceinline__ uint32_t gsum_load(uint32_t *s_sum,
+                                              const uint32_t *sum, uint32_t n) {
+  const uint32_t nw = gsum_words(n);
+  for (uint32_t i = threadIdx.x; i < nw; i += NT) s_sum[i] = sum[i];
+  __syncthreads();
+  return nw * 1024u;  // vertices covered
+}
+
+__device__ __forceinline__ bool giant_test(uint32_t x, const uint32_t *s_sum,
+                                           uint32_t covered,
+                                           const uint32_t *__restrict__ bits) {
+  const uint32_t w = x >> 5;
+  if (x < covered && ((s_sum[w >> 5] >> (w & 31)) & 1u)) return true;
+  return (bits[w] >> (x & 31)) & 1u;
+}
+
 }  // namespace
 }  // namespace phem

@@ -48,6 +48,7 @@ class MstStats(ctypes.Structure):
         ("tau", ctypes.c_uint64),
         ("light_edges", ctypes.c_uint64),
         ("heavy_edges", ctypes.c_uint64),
+        ("giant_edges", ctypes.c_uint64),
         ("m_log", ctypes.c_uint64 * MAX_ROUNDS),
         ("c_log", ctypes.c_uint64 * MAX_ROUNDS),
     ]
@@ -58,6 +59,7 @@ class MstStats(ctypes.Structure):
                     weight_exact=bool(self.weight_exact), sum_m=self.sum_m,
                     sum_c=self.sum_c, valid_edges=self.valid_edges, tau=self.tau,
                     light_edges=self.light_edges, heavy_edges=self.heavy_edges,
+                    giant_edges=self.giant_edges,
                     light_rounds=self.light_rounds,
                     m_log=list(self.m_log[:r]),
                     c_log=list(self.c_log[:r]))
