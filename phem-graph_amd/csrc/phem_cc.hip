@@ -20,6 +20,7 @@ namespace phem {
 constexpr int CBLK = 256;
 constexpr int CEPT = 8;
 constexpr int CTILE = CBLK * CEPT;
+constexpr uint32_t GSUM_CC = 4096;  // giant-summary words in LDS (16 KB)
 
 __global__ __launch_bounds__(CBLK) void k_cc_init(uint32_t n, uint32_t *par,
                                                   ull *forest) {
@@ -47,9 +48,9 @@ __global__ __launch_bounds__(CBLK) void k_cc_union(const ull *__restrict__ edges
                                                    uint32_t *par, ull *forest,
                                                    const uint32_t *__restrict__ giant,
                                                    const uint32_t *gsum) {
-  __shared__ uint32_t s_sum[FILTER ? GSUM_WORDS : 1];
+  __shared__ uint32_t s_sum[FILTER ? GSUM_CC : 1];
   uint32_t covered = 0;
-  if (FILTER) covered = gsum_load<CBLK>(s_sum, gsum, n);
+  if (FILTER) covered = gsum_load<CBLK, GSUM_CC>(s_sum, gsum, n);
   for (uint64_t tile = blockIdx.x; tile * CTILE < m; tile += gridDim.x) {
     const uint64_t base = tile * CTILE + threadIdx.x;
     ull e[CEPT];
@@ -187,7 +188,7 @@ extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
   hipStream_t s = (hipStream_t)stream;
   uint32_t *giant_root = d_labels;  // 1 word
   uint32_t *bits = d_labels + 2;    // ceil(n / 32) words
-  // summary: gsum_words(n) <= ceil(n / 1024) words; all fit in n for n >= 64
+  // summary: gsum_words <= ceil(n / 1024) words; all fit in n for n >= 64
   uint32_t *gsum = bits + (n + 31) / 32;
   rc = phem_cc_union(n, d_edges, m1, d_parent, d_forest, stream);
   if (rc) return rc;
@@ -213,7 +214,7 @@ extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
   {
     PHEM_PROF("k_giant_summary", s);
     hipLaunchKernelGGL(k_giant_summary,
-                       dim3(grid_for(32ull * gsum_words(n), SPLIT_BLK)),
+                       dim3(grid_for(32ull * gsum_words(n, GSUM_MAX), SPLIT_BLK)),
                        dim3(SPLIT_BLK), 0, s, n, (const uint32_t *)bits, gsum);
   }
   PHEM_LAUNCHED("k_giant_summary");

@@ -56,6 +56,7 @@ constexpr int WPT = 8;         // bitmap words per thread
 constexpr int WPB = BLK * WPT; // bitmap words per block
 constexpr int IPT = 4;         // components per thread in hook / compress
 constexpr int LTILE = BLK * IPT;
+constexpr uint32_t GSUM_HEAVY = 8192;  // giant-summary words in LDS (32 KB)
 
 // Counter area (ull), every counter on its own 128-byte line (CSTR ull):
 // scalars, then four blocks of KSEG partial sums (valid edges, weight sum,
@@ -306,11 +307,11 @@ __global__ __launch_bounds__(BLK) void k_pass(
     ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *valid_parts,
     ull *best, ull *gmin, const uint32_t *gsum) {
   __shared__ SegView sv;
-  __shared__ uint32_t s_sum[MODE == P_HEAVY ? GSUM_WORDS : 1];
+  __shared__ uint32_t s_sum[MODE == P_HEAVY ? GSUM_HEAVY : 1];
   constexpr bool INPUT = MODE != P_EDGES;
   if (!INPUT) seg_load<TILE>(sv, in_cnt);
   uint32_t covered = 0;
-  if (MODE == P_HEAVY) covered = gsum_load<BLK>(s_sum, gsum, n);
+  if (MODE == P_HEAVY) covered = gsum_load<BLK, GSUM_HEAVY>(s_sum, gsum, n);
   const ull tau = (MODE == P_LIGHT || MODE == P_HEAVY) ? *tau_ptr : 0ull;
   const uint32_t ntiles_in = INPUT ? (uint32_t)ceil_div_dev(m0, TILE) : 0u;
   uint32_t nval = 0;  // valid input edges seen by this thread
@@ -686,7 +687,7 @@ static MstLayout mst_layout(uint32_t n, uint64_t m) {
   L.par = take(4ull * n);
   L.best = take(8ull * n);
   L.gmin = take(8ull * n);
-  L.gsum = take(4ull * GSUM_WORDS);
+  L.gsum = take(4ull * GSUM_MAX);
   L.listL = take(4ull * LN);
   L.listA = take(4ull * LN);
   L.ea_a = take(4ull * E);
@@ -955,7 +956,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     {
       PHEM_PROF("k_giant_summary", s);
       hipLaunchKernelGGL(k_giant_summary,
-                         dim3(grid_for(32ull * gsum_words(n), SPLIT_BLK)),
+                         dim3(grid_for(32ull * gsum_words(n, GSUM_MAX), SPLIT_BLK)),
                          dim3(SPLIT_BLK), 0, s, n, (const uint32_t *)gbits, gsum);
     }
     PHEM_LAUNCHED("k_giant_summary");

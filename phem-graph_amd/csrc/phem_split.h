@@ -111,21 +111,22 @@ __global__ __launch_bounds__(SPLIT_BLK) void k_giant_bits(uint32_t n,
 
 // Summary of the giant bitmap kept in LDS by the kernels that test it: bit
 // w of the summary = bitmap word w is all ones (its 32 vertices are all in
-// the giant component).  It covers the first GSUM_WORDS * 1024 vertices
-// (16 KB of LDS); a set bit answers the test without an L2 request, a clear
-// bit or a vertex beyond it falls back to the bitmap.
-constexpr uint32_t GSUM_WORDS = 4096;
+// the giant component).  Summary word k covers vertices [1024 k, 1024 k +
+// 1024); the array holds up to GSUM_MAX words and a kernel keeps the first W
+// of them in LDS (W * 4 bytes).  A set bit answers the test without an L2
+// request; a clear bit, or a vertex beyond the kernel's W words, falls back
+// to the bitmap.
+constexpr uint32_t GSUM_MAX = 8192;
 
-__host__ __device__ constexpr uint32_t gsum_words(uint32_t n) {
-  return ((uint64_t)n + 1023) / 1024 < GSUM_WORDS
-             ? (uint32_t)(((uint64_t)n + 1023) / 1024)
-             : GSUM_WORDS;
+__host__ __device__ constexpr uint32_t gsum_words(uint32_t n, uint32_t W) {
+  return ((uint64_t)n + 1023) / 1024 < W ? (uint32_t)(((uint64_t)n + 1023) / 1024)
+                                         : W;
 }
 
 __global__ __launch_bounds__(SPLIT_BLK) void k_giant_summary(
     uint32_t n, const uint32_t *__restrict__ bits, uint32_t *__restrict__ sum) {
   const uint64_t words = ((uint64_t)n + 31) / 32;
-  const uint64_t cover = 32ull * gsum_words(n);  // bitmap words covered
+  const uint64_t cover = 32ull * gsum_words(n, GSUM_MAX);  // bitmap words covered
   for (uint64_t base = (uint64_t)blockIdx.x * SPLIT_BLK; base < cover;
        base += (uint64_t)gridDim.x * SPLIT_BLK) {
     const uint64_t w = base + threadIdx.x;
@@ -139,14 +140,15 @@ __global__ __launch_bounds__(SPLIT_BLK) void k_giant_summary(
   }
 }
 
-// Loads the summary into LDS (every thread of the block calls it).
-template <int NT>
+// Loads the first W summary words into LDS (every thread of the block
+// calls it); returns the number of vertices they cover.
+template <int NT, uint32_t W>
 __device__ __forceinline__ uint32_t gsum_load(uint32_t *s_sum,
                                               const uint32_t *sum, uint32_t n) {
-  const uint32_t nw = gsum_words(n);
+  const uint32_t nw = gsum_words(n, W);
   for (uint32_t i = threadIdx.x; i < nw; i += NT) s_sum[i] = sum[i];
   __syncthreads();
-  return nw * 1024u;  // vertices covered
+  return nw * 1024u;
 }
 
 __device__ __forceinline__ bool giant_test(uint32_t x, const uint32_t *s_sum,
