@@ -101,12 +101,19 @@ typedef struct phem_mst_stats {
  *   the rest (filter-Kruskal, as the CPU path does, MST/src/utils.cu:794-841).
  *   <= 0 runs the canonical Boruvka round loop over all edges
  *   (MST/src/kernels.cu:507-689); its per-round log is then the reference's.
- *   Default 1.0, or the environment variable PHEM_MST_LIGHT_FACTOR. */
+ *   NaN, like NULL options: 1.0, or the environment variable
+ *   PHEM_MST_LIGHT_FACTOR.
+ * flags: PHEM_MST_MAXIMUM or 0. */
 typedef struct phem_mst_opts {
   double light_factor;
-  uint32_t flags; /* reserved, 0 */
+  uint32_t flags; /* PHEM_MST_* bits */
   uint32_t reserved;
 } phem_mst_opts;
+
+/* Maximum spanning forest: edges ordered by (weight, id) descending, the
+ * order of the reference's populate_max_weight / populate_eid_max
+ * (MST/src/kernels.cu:437-502: atomicMax on the weight, then on the id). */
+#define PHEM_MST_MAXIMUM 1u
 
 /* Bytes of device workspace phem_mst_solve needs for (n, m). */
 int phem_mst_workspace_bytes(uint32_t n, uint64_t m, uint64_t *bytes);

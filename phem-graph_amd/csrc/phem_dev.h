@@ -21,6 +21,24 @@ __device__ __forceinline__ uint32_t wkey_inv(uint32_t k, int mode) {
   return mode ? ((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k) : k;
 }
 
+// Order of the edges: bit 0 of `ord` = float32 weights, bit 1 = maximum
+// spanning forest.  The minimum order is (wkey(w), id) ascending; the maximum
+// order is (w, id) descending, encoded as (~wkey(w), 0xFFFFFFFE - id)
+// ascending so that one minimum search serves both.  Ids are < 2^32 - 2, so
+// no key of either order equals the all-ones "no edge" value.
+constexpr int ORD_F32 = 1;
+constexpr int ORD_MAX = 2;
+
+__device__ __forceinline__ uint64_t make_key(uint32_t w, uint64_t id, int ord) {
+  const uint32_t k = wkey(w, ord & ORD_F32);
+  if (ord & ORD_MAX)
+    return ((uint64_t)(~k) << 32) | (uint32_t)(0xFFFFFFFEu - (uint32_t)id);
+  return ((uint64_t)k << 32) | (uint32_t)id;
+}
+__device__ __forceinline__ uint32_t key_id(uint64_t key, int ord) {
+  return (ord & ORD_MAX) ? 0xFFFFFFFEu - (uint32_t)key : (uint32_t)key;
+}
+
 // ---- wave64 helpers --------------------------------------------------------
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }

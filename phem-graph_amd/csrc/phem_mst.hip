@@ -37,6 +37,7 @@
 // Every appended list (alive edges, component lists) is segmented: tile v of
 // a producer appends to segment v mod KSEG, whose counter has its own cache
 // line, so no global atomic is shared by all tiles.
+#include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(BLK) void k_mst_init(uint32_t n, uint32_t *par,
 // Round-0 sweep over the input edges (canonical schedule).
 template <class Src>
 __global__ __launch_bounds__(BLK) void k_sweep0(Src src, uint64_t m, uint32_t n,
-                                                int wmode, ull *best) {
+                                                int ord, ull *best) {
   for (uint64_t tile = blockIdx.x; tile * TILE < m; tile += gridDim.x) {
     uint32_t a[EPT], b[EPT], ww[EPT];
 #pragma unroll
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(BLK) void k_sweep0(Src src, uint64_t m, uint32_t n,
       const uint64_t i =
           tile * TILE + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
       if (a[j] < n && b[j] < n && a[j] != b[j]) {
-        uint64_t key = ((uint64_t)wkey(ww[j], wmode) << 32) | i;
+        uint64_t key = make_key(ww[j], i, ord);
         min_store(best, a[j], key);
         min_store(best, b[j], key);
       }
@@ -208,8 +209,8 @@ __global__ __launch_bounds__(BLK) void k_sweep0(Src src, uint64_t m, uint32_t n,
 template <class Src>
 __global__ __launch_bounds__(BLK) void k_hook(
     const uint32_t *__restrict__ list, const ull *list_cnt, uint64_t list_cap,
-    uint32_t n_ident, ull *best, Src src, uint32_t *par, uint32_t *bitmap,
-    uint32_t *act, ull *act_cnt, uint64_t act_cap) {
+    uint32_t n_ident, ull *best, Src src, int ord, uint32_t *par,
+    uint32_t *bitmap, uint32_t *act, ull *act_cnt, uint64_t act_cap) {
   __shared__ SegView sv;
   if (list_cnt) seg_load<LTILE>(sv, list_cnt);
   const uint32_t ntiles_ident = (uint32_t)ceil_div_dev(n_ident, LTILE);
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(BLK) void k_hook(
         if (key != KEY_NONE) {
           active[j] = true;
           best[c[j]] = KEY_NONE;  // reset for the next round's sweep
-          uint32_t e = (uint32_t)key;
+          uint32_t e = key_id(key, ord);
           atomicOr(&bitmap[e >> 5], 1u << (e & 31));
           uf_union(par, src.U(e), src.V(e));
         }
@@ -301,7 +302,7 @@ template <int MODE, class Src>
 __global__ __launch_bounds__(BLK) void k_pass(
     Src src, const uint32_t *__restrict__ ia, const uint32_t *__restrict__ ib,
     const ull *__restrict__ ik, const ull *in_cnt, uint64_t in_cap,
-    uint64_t m0, uint32_t n, int wmode, const uint32_t *__restrict__ par,
+    uint64_t m0, uint32_t n, int ord, const uint32_t *__restrict__ par,
     const ull *tau_ptr, const uint32_t *__restrict__ giant,
     uint32_t *__restrict__ oa, uint32_t *__restrict__ ob,
     ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *valid_parts,
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
         src.load4(i4, end, a + 4 * g, b + 4 * g, wv);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          key[4 * g + q] = ((ull)wkey(wv[q], wmode) << 32) | (i4 + q);
+          key[4 * g + q] = make_key(wv[q], i4 + q, ord);
           val[4 * g + q] = a[4 * g + q] < n && b[4 * g + q] < n &&
                            a[4 * g + q] != b[4 * g + q];
         }
@@ -510,7 +511,7 @@ __global__ __launch_bounds__(BLK) void k_giant_emit(
 // (KEY_NONE: every edge is light).
 template <class Src>
 __global__ __launch_bounds__(SBLK) void k_threshold(Src src, uint64_t m,
-                                                    uint32_t n, int wmode,
+                                                    uint32_t n, int ord,
                                                     uint64_t light_edges,
                                                     ull *tau_out) {
   __shared__ ull s[NSAMP];
@@ -518,7 +519,7 @@ __global__ __launch_bounds__(SBLK) void k_threshold(Src src, uint64_t m,
     const uint64_t i = (uint64_t)j * m / NSAMP;
     const uint32_t a = src.U(i), b = src.V(i);
     s[j] = (a < n && b < n && a != b)
-               ? (((ull)wkey(src.W(i), wmode) << 32) | i)
+               ? make_key(src.W(i), i, ord)
                : KEY_NONE;
   }
   __syncthreads();
@@ -738,13 +739,19 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                    (unsigned long long)m);
   if (wmode != PHEM_W_U32 && wmode != PHEM_W_F32)
     return set_err(PHEM_EINVAL, "unknown weight mode %d", wmode);
+  const uint32_t flags = opts ? opts->flags : 0u;
+  if (flags & ~(uint32_t)PHEM_MST_MAXIMUM)
+    return set_err(PHEM_EINVAL, "unknown option flags 0x%x", flags);
+  // key order passed to the kernels that build keys
+  const int ord = wmode | ((flags & PHEM_MST_MAXIMUM) ? ORD_MAX : 0);
   if (n && !d_mst_ids) return set_err(PHEM_EINVAL, "d_mst_ids is NULL");
   const MstLayout L = mst_layout(n, m);
   if (!d_ws || ws_bytes < L.total)
     return set_err(PHEM_ENOMEM, "workspace %llu B < required %llu B",
                    (unsigned long long)ws_bytes, (unsigned long long)L.total);
-  const double light_factor =
-      opts ? opts->light_factor : mst_default_light_factor();
+  const double light_factor = opts && !std::isnan(opts->light_factor)
+                                  ? opts->light_factor
+                                  : mst_default_light_factor();
   const uint64_t light_edges =
       light_factor > 0 ? (uint64_t)(light_factor * (double)n) : m;
   const bool split = light_edges < m;
@@ -819,7 +826,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
         PHEM_PROF("k_hook", s);
         hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(hook_tiles, 1)),
                            dim3(BLK), 0, s, listL, all ? noull : ctr + cL(r),
-                           L.lcap, all ? n : 0u, best, src, par, bitmap, listA,
+                           L.lcap, all ? n : 0u, best, src, ord, par, bitmap,
+                           listA,
                            ctr + cA(r), L.lcap);
       }
       PHEM_LAUNCHED("k_hook");
@@ -835,7 +843,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
         hipLaunchKernelGGL((k_pass<P_EDGES, Src>),
                            dim3(grid_for(ceil_div(m_r, TILE) + KSEG, 1)),
                            dim3(BLK), 0, s, src, Ea[cur], Eb[cur], Ek[cur],
-                           ctr + cE(r), L.ecap, 0ull, n, wmode, par, noull, nou,
+                           ctr + cE(r), L.ecap, 0ull, n, ord, par, noull, nou,
                            Ea[cur ^ 1], Eb[cur ^ 1], Ek[cur ^ 1],
                            ctr + cE(r + 1), L.ecap, (ull *)nullptr, best,
                            (ull *)nullptr, nou);
@@ -862,14 +870,15 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     {
       PHEM_PROF("k_sweep0", s);
       hipLaunchKernelGGL(k_sweep0<Src>, dim3(grid_for(m, TILE)), dim3(BLK), 0,
-                         s, src, m, n, wmode, best);
+                         s, src, m, n, ord, best);
     }
     PHEM_LAUNCHED("k_sweep0");
     const uint64_t t0 = ceil_div(n, LTILE);
     {
       PHEM_PROF("k_hook", s);
       hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(t0, 1)), dim3(BLK), 0, s,
-                         listL, noull, L.lcap, n, best, src, par, bitmap, listA,
+                         listL, noull, L.lcap, n, best, src, ord, par, bitmap,
+                         listA,
                          ctr + cA(0), L.lcap);
     }
     PHEM_LAUNCHED("k_hook");
@@ -884,7 +893,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       PHEM_PROF("k_compact<R0>", s);
       hipLaunchKernelGGL((k_pass<P_ALL, Src>), dim3(grid_for(m, TILE)),
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
-                         n, wmode, par, noull, nou, Ea[0], Eb[0], Ek[0],
+                         n, ord, par, noull, nou, Ea[0], Eb[0], Ek[0],
                          ctr + cE(1), L.ecap, ctr + cPart(0), best,
                          (ull *)nullptr, nou);
     }
@@ -911,14 +920,14 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     {
       PHEM_PROF("k_threshold", s);
       hipLaunchKernelGGL(k_threshold<Src>, dim3(1), dim3(SBLK), 0, s, src, m,
-                         n, wmode, light_edges, ctr + C_TAU);
+                         n, ord, light_edges, ctr + C_TAU);
     }
     PHEM_LAUNCHED("k_threshold");
     {
       PHEM_PROF("k_light", s);
       hipLaunchKernelGGL((k_pass<P_LIGHT, Src>), dim3(grid_for(m, TILE)),
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
-                         n, wmode, par, ctr + C_TAU, nou, Ea[0], Eb[0], Ek[0],
+                         n, ord, par, ctr + C_TAU, nou, Ea[0], Eb[0], Ek[0],
                          ctr + cE(0), L.ecap, ctr + cPart(0), best,
                          (ull *)nullptr, nou);
     }
@@ -964,7 +973,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       PHEM_PROF("k_heavy", s);
       hipLaunchKernelGGL((k_pass<P_HEAVY, Src>), dim3(grid_for(m, TILE)),
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
-                         n, wmode, par, ctr + C_TAU, gbits, Ea[cur], Eb[cur],
+                         n, ord, par, ctr + C_TAU, gbits, Ea[cur], Eb[cur],
                          Ek[cur], ctr + cE(r), L.ecap, (ull *)nullptr, best,
                          gmin, gsum);
     }

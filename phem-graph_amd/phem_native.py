@@ -203,6 +203,16 @@ def mst_workspace_bytes(n, m):
     return b.value
 
 
+MST_MAXIMUM = 1
+
+
+def _opts(light_factor, maximum):
+    if light_factor is None and not maximum:
+        return None
+    lf = float("nan") if light_factor is None else light_factor
+    return ctypes.byref(MstOpts(lf, MST_MAXIMUM if maximum else 0, 0))
+
+
 class MstSolver:
     """Reusable workspace + outputs for phem_mst_solve on one device.
 
@@ -218,15 +228,16 @@ class MstSolver:
         self.labels = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
 
     def solve(self, u, v, w, wmode=W_F32, labels=True, stream=None, light_factor=None,
-              n=None):
+              n=None, maximum=False):
         """light_factor: None = library default; <= 0 = canonical Boruvka.
-        n: vertex count of this graph (default and maximum: the solver's)."""
+        n: vertex count of this graph (default and maximum: the solver's).
+        maximum: maximum spanning forest ((weight, id) descending)."""
         m = u.numel()
         n = self.n if n is None else n
         if m > self.m or n > self.n:
             raise PhemError(f"n={n} m={m} exceed solver capacity {self.n}, {self.m}")
         st = MstStats()
-        opts = None if light_factor is None else ctypes.byref(MstOpts(light_factor, 0, 0))
+        opts = _opts(light_factor, maximum)
         _check(lib().phem_mst_solve(n, m, _p(u), _p(v), _p(w), wmode, _p(self.ws),
                                     self.ws_bytes, _p(self.ids),
                                     _p(self.labels) if labels else None, opts,
@@ -234,13 +245,14 @@ class MstSolver:
         s = st.as_dict()
         return self.ids[: s["mst_count"]], (self.labels[:n] if labels else None), s
 
-    def solve_aos(self, edges, wmode=W_F32, labels=True, stream=None, light_factor=None):
+    def solve_aos(self, edges, wmode=W_F32, labels=True, stream=None, light_factor=None,
+                  maximum=False):
         """edges: device tensor of m x 3 32-bit words {u, v, w}."""
         m = edges.numel() // 3
         if m > self.m:
             raise PhemError(f"m={m} exceeds solver capacity {self.m}")
         st = MstStats()
-        opts = None if light_factor is None else ctypes.byref(MstOpts(light_factor, 0, 0))
+        opts = _opts(light_factor, maximum)
         _check(lib().phem_mst_solve_aos(self.n, m, _p(edges), wmode, _p(self.ws),
                                         self.ws_bytes, _p(self.ids),
                                         _p(self.labels) if labels else None, opts,
@@ -250,9 +262,10 @@ class MstSolver:
         return self.ids[: s["mst_count"]], (self.labels[: self.n] if labels else None), s
 
 
-def mst_solve(n, u, v, w, wmode=W_F32, labels=True, light_factor=None):
+def mst_solve(n, u, v, w, wmode=W_F32, labels=True, light_factor=None, maximum=False):
     return MstSolver(n, u.numel(), device=u.device).solve(u, v, w, wmode, labels,
-                                                          light_factor=light_factor)
+                                                          light_factor=light_factor,
+                                                          maximum=maximum)
 
 
 def cc_solve(n, edges, forest=True, stream=None):
@@ -356,7 +369,7 @@ def batch_solve(voff, eoff, u, v, w, wmode=W_F32, labels=True, weights=True,
     lab = torch.empty(max(n_total, 1), dtype=torch.int32, device=dev) if labels else None
     wfx = torch.empty(k, dtype=torch.int64, device=dev) if weights else None
     bad = torch.empty(k, dtype=torch.int64, device=dev) if weights else None
-    opts = None if light_factor is None else ctypes.byref(MstOpts(light_factor, 0, 0))
+    opts = _opts(light_factor, False)
     st = MstStats()
     _check(lib().phem_batch_solve(k, hv, he, _p(u), _p(v), _p(w), wmode, _p(ws),
                                   ws.numel(), _p(ids), _p(cnt), _p(lab), _p(wfx), _p(bad),

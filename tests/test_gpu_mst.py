@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import cases
+import max_forest as MF
 import oracle as O
 import synth as S
 
@@ -190,3 +191,20 @@ def test_mst_path_2e22_chords_deep_rounds(P):
     u, v, w = S.path_chords(n, 1 << 20, 4)
     st = _full_check(P, n, u, v, w, canonical_log=True)
     print(f"path: m={len(u)} forest={st['mst_count']} rounds={st['rounds']}", flush=True)
+
+
+@pytest.mark.parametrize("lf", [None, 0.0])
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_maximum_spanning_forest(P, name, lf):
+    n, u, v, w, wmode = cases.make(name)
+    ids, lab, st = P.mst_solve(n, dev(u), dev(v), dev(w), wmode, light_factor=lf,
+                               maximum=True)
+    torch.cuda.synchronize()
+    x_ids, x_lab = MF.max_forest_oracle(n, u, v, w, wmode)
+    assert np.array_equal(host(ids), x_ids)
+    assert np.array_equal(host(lab), x_lab)
+    ws, exact = O.weight_sum(x_ids, w, wmode)
+    assert st["weight_exact"] == exact
+    if exact:
+        assert st["weight_fx"] == ws
+

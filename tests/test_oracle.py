@@ -14,6 +14,7 @@ import scipy.sparse as sp
 import scipy.sparse.csgraph as cg
 
 import cases
+import max_forest as MF
 import oracle as O
 import synth as S
 
@@ -224,3 +225,46 @@ def test_checker_catches_planted_mistakes():
     # spanning forest that the invariants accept; only the comparison with
     # Kruskal can reject it, which is why parity tests compare id sets
     assert vv == v[ids[0]]
+
+
+def test_maximum_reduction_pinned_by_brute_force():
+    """The reduction used above agrees with exhaustive search on tiny graphs
+    with heavy ties (every spanning forest enumerated)."""
+    import itertools
+    rng = np.random.default_rng(11)
+    for t in range(30):
+        n = int(rng.integers(2, 6))
+        m = int(rng.integers(1, 7))
+        u = rng.integers(0, n, m).astype(np.uint32)
+        v = rng.integers(0, n, m).astype(np.uint32)
+        w = rng.integers(0, 3, m).astype(np.uint32)
+        x_ids, _ = MF.max_forest_oracle(n, u, v, w, 0)
+        # brute force: among edge subsets forming a spanning forest of the
+        # graph with the maximum size, the lexicographically greatest by the
+        # descending (w, id) keys of its edges
+        valid = [i for i in range(m) if u[i] != v[i]]
+        best = None
+        for r in range(len(valid), -1, -1):
+            for sub in itertools.combinations(valid, r):
+                par = list(range(n))
+
+                def f(x):
+                    while par[x] != x:
+                        x = par[x]
+                    return x
+                ok = True
+                for i in sub:
+                    a, b = f(u[i]), f(v[i])
+                    if a == b:
+                        ok = False
+                        break
+                    par[a] = b
+                if not ok:
+                    continue
+                key = sorted(((int(w[i]), i) for i in sub), reverse=True)
+                if best is None or len(sub) > len(best[1]) or (
+                        len(sub) == len(best[1]) and key > best[0]):
+                    best = (key, sub)
+            if best is not None and len(best[1]) == r:
+                break
+        assert sorted(best[1]) == x_ids.tolist(), (n, u, v, w)
