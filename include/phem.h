@@ -330,6 +330,41 @@ int phem_server_get_stats(phem_server *srv, phem_server_stats *st);
 int phem_server_reset_stats(phem_server *srv);
 
 /* ---------------------------------------------------------------------------
+ * Graph files (host memory).  The readers fill a phem_graph_file whose arrays
+ * the library allocates; phem_graph_file_free releases them.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  uint32_t n;           /* vertices */
+  int32_t wmode;        /* order of w: PHEM_W_U32 or PHEM_W_F32 */
+  uint64_t m;           /* edges (u, v, w forms) or records (edges form) */
+  uint32_t *u, *v, *w;  /* MST forms: m edges, 0-based */
+  uint64_t *edges;      /* CC forms: m records (u << 32) | v with u < v */
+} phem_graph_file;
+
+/* MatrixMarket coordinate file (MST/src/utils.cu:160-179, 896-1036): comment
+ * lines, "rows cols entries", then "u v [w]" lines with 1-based ids; n = rows,
+ * at most `entries` edges, stored 0-based with u <= v.  Integer weights are
+ * read (a missing weight reads as 0), or with generate_weights set to
+ * w = ((u-1)(v-1) mod n) + 1 in 32-bit unsigned arithmetic (utils.cu:1002).
+ * Fills u, v, w (PHEM_W_U32). */
+int phem_read_mtx(const char *path, int generate_weights, phem_graph_file *g);
+/* HAST edge list (.edges / .eg2 / .txt, CC/graph.cu:45-90): "V E" then E
+ * directed pairs "u v"; keeps the pairs with u < v.  Fills edges. */
+int phem_read_edge_list(const char *path, phem_graph_file *g);
+/* ECL binary CSR (.egr, CC/graph.cu:92-121, CC/README.md:56-66): int64 V,
+ * int64 E, int64 offsets[V+1], int32 neighbours[E]; keeps u < neighbour.
+ * Fills edges. */
+int phem_read_ecl(const char *path, phem_graph_file *g);
+/* PBBS weighted edge array (MST/src/common/graphIO.h:169-188): the word
+ * "WeightedEdgeArray" then triples "u v w", w read as a double and stored as
+ * float32; n = largest id + 1.  Fills u, v, w (PHEM_W_F32). */
+int phem_read_pbbs_weighted(const char *path, phem_graph_file *g);
+/* Writes u, v, w as the reference's weighted MatrixMarket dump
+ * (MST/src/new_boruvka.cu:399-408): both directions, 1-based. */
+int phem_write_mtx_weighted(const char *path, const phem_graph_file *g);
+void phem_graph_file_free(phem_graph_file *g);
+
+/* ---------------------------------------------------------------------------
  * Synthetic inputs on the device (counter-based splitmix64 streams; the same
  * bytes as the host generator on every host).  Edges are stored with u < v.
  * ------------------------------------------------------------------------- */

@@ -5,7 +5,10 @@
  * library in place of its CUDA sources:
  *
  *   MST/src/kernels.cu, MST/src/new_boruvka.cu   -> boruvka_for_edge_array,
- *                                                   device_boruvka
+ *                                                   device_boruvka,
+ *                                                   new_boruvka
+ *   MST/src/utils.cu (file reading)               -> getGraphInfo,
+ *                                                   mmapPopulateEdgeArray
  *   CC/ExternalSpanningTree.cu                    -> SpanningTree
  *   CC/GPUSpanningTree.cu                         -> gpu_spanning_tree
  *
@@ -22,6 +25,7 @@
 #include <thrust/device_ptr.h>
 
 #include <cstdint>
+#include <fstream>
 #include <limits>
 #include <string>
 
@@ -50,6 +54,40 @@ struct outgoing_edge_id {
 void boruvka_for_edge_array(edge *edges, vertex num_nodes, vertex num_edges,
                             edge *final_msf, vertex &number_of_filled_edges,
                             std::string result = "res.txt");
+
+/* Vertex count and entry count from a MatrixMarket header
+ * (MST/src/utils.cu:160-179); leaves the stream after the header. */
+void getGraphInfo(std::ifstream &file, vertex &num_nodes,
+                  large_vertex &num_edges_large);
+
+/* Reads num_edges edges of a MatrixMarket file into edges (MST/src/utils.cu:
+ * 896-1036): 0-based, u <= v, integer weights or the generated weight
+ * ((u-1)(v-1) mod num_nodes) + 1.  Prints and returns on an unreadable file,
+ * as the reference does. */
+void mmapPopulateEdgeArray(const std::string &filename, edge *edges,
+                           vertex num_edges, vertex num_nodes,
+                           bool generate_random_weights = false);
+
+/* The reference's MST driver (MST/src/new_boruvka.cu:330-510): reads a
+ * MatrixMarket file and computes its minimum spanning forest.
+ *  - default: one solve of the whole graph (boruvka_for_edge_array);
+ *  - use_streamline / use_cpu_gpu_streamline: chunks of chunk_size edges
+ *    (or num_chunks_ideal chunks) folded into a forest kept on the device,
+ *    F = MSF(F u chunk) (MST/src/kernels.cu:849-1154); the CPU pre-filter of
+ *    the heterogeneous variant is not part of this library, its chunks run
+ *    on the GPU;
+ *  - use_malloc_managed: runs the default path (device memory);
+ *  - use_cpu_only: the ParlayLib CPU path stays the caller's
+ *    (cpu_compute_pbbs); this library prints a notice and returns;
+ *  - debug && save_weighted_graph: writes weighted.mtx and returns.
+ * Prints "Total weight W" (the exact forest weight).  When result_filename is
+ * not "res.txt" the forest is written there as "u+1 v+1 w" lines. */
+void new_boruvka(std::string filename, std::string result_filename, bool debug,
+                 vertex chunk_size, int num_chunks_ideal,
+                 bool generate_random_weights = false,
+                 bool use_malloc_managed = false, bool use_streamline = false,
+                 bool use_cpu_only = false, bool use_cpu_gpu_streamline = false,
+                 bool save_weighted_graph = false);
 
 /* Minimum spanning forest of device edges (MST/src/kernels.cu:507-689).
  * Reads d_edges[0 .. num_edges); appends the forest's edges to

@@ -2,7 +2,10 @@
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <mutex>
+#include <string>
+#include <vector>
 
 #include "phem.h"
 #include "phem_dev.h"
@@ -165,6 +168,139 @@ void boruvka_for_edge_array(edge *edges, vertex num_nodes, vertex num_edges,
   die_hip(hipFree(d_edges), "hipFree");
   die_hip(hipFree(d_out), "hipFree");
   die_hip(hipFree(d_rep), "hipFree");
+}
+
+// ---- MST driver: MatrixMarket file -> forest --------------------------------
+
+void getGraphInfo(std::ifstream &file, vertex &num_nodes,
+                  large_vertex &num_edges_large) {
+  std::string line;
+  while (std::getline(file, line))
+    if (line.empty() || line[0] != '%') break;
+  unsigned long long rows = 0, cols = 0, entries = 0;
+  if (std::sscanf(line.c_str(), "%llu %llu %llu", &rows, &cols, &entries) != 3)
+    rows = entries = 0;
+  num_nodes = (vertex)rows;
+  num_edges_large = (large_vertex)entries;
+}
+
+void mmapPopulateEdgeArray(const std::string &filename, edge *edges,
+                           vertex num_edges, vertex /*num_nodes*/,
+                           bool generate_random_weights) {
+  phem_graph_file g;
+  int rc = phem_read_mtx(filename.c_str(), generate_random_weights ? 1 : 0, &g);
+  if (rc != PHEM_OK) {
+    std::fprintf(stderr, "Error reading %s: %s\n", filename.c_str(),
+                 phem_last_error());
+    return;
+  }
+  const uint64_t k = g.m < num_edges ? g.m : num_edges;
+  for (uint64_t i = 0; i < k; i++) edges[i] = edge{g.u[i], g.v[i], g.w[i]};
+  phem_graph_file_free(&g);
+}
+
+namespace {
+
+// Forest as "u+1 v+1 w" lines (the streamline path's result file,
+// MST/src/kernels.cu:1104-1117) and its exact weight.
+uint64_t report_forest(const edge *f, uint64_t k, const std::string &result) {
+  uint64_t w = 0;
+  for (uint64_t i = 0; i < k; i++) w += f[i].w;
+  if (result != "res.txt") {
+    FILE *o = std::fopen(result.c_str(), "w");
+    if (!o) {
+      std::fprintf(stderr, "cannot write %s\n", result.c_str());
+      std::exit(EXIT_FAILURE);
+    }
+    for (uint64_t i = 0; i < k; i++)
+      std::fprintf(o, "%llu %llu %u\n", (unsigned long long)f[i].u + 1,
+                   (unsigned long long)f[i].v + 1, f[i].w);
+    std::fclose(o);
+  }
+  return w;
+}
+
+}  // namespace
+
+void new_boruvka(std::string filename, std::string result_filename, bool debug,
+                 vertex chunk_size, int num_chunks_ideal,
+                 bool generate_random_weights, bool use_malloc_managed,
+                 bool use_streamline, bool use_cpu_only,
+                 bool use_cpu_gpu_streamline, bool save_weighted_graph) {
+  std::printf("new boruvka %s\n", filename.c_str());
+  phem_graph_file g;
+  die_phem(phem_read_mtx(filename.c_str(), generate_random_weights ? 1 : 0, &g),
+           "phem_read_mtx");
+  const uint32_t n = g.n;
+  const uint64_t m = g.m;
+  std::printf("Total num edges %llu\nTotal num nodes %u\n", (unsigned long long)m, n);
+  if (num_chunks_ideal > 0)  // MST/src/new_boruvka.cu:326-354
+    chunk_size = (vertex)((float)m / ((float)num_chunks_ideal + 0.4f));
+  if (debug && save_weighted_graph) {
+    die_phem(phem_write_mtx_weighted("weighted.mtx", &g), "phem_write_mtx_weighted");
+    phem_graph_file_free(&g);
+    return;
+  }
+  if (use_cpu_only) {
+    std::printf("cpu only: the ParlayLib path (cpu_compute_pbbs) is the "
+                "caller's; nothing to do here\n");
+    phem_graph_file_free(&g);
+    return;
+  }
+  std::vector<edge> forest;
+  if (use_streamline || use_cpu_gpu_streamline) {
+    if (use_cpu_gpu_streamline)
+      std::printf("streamline: heterogeneous variant runs every chunk on the GPU\n");
+    const uint64_t cs = chunk_size ? chunk_size : (m ? m : 1);
+    std::printf("streamline chunk boruvka, chunk size %llu\n", (unsigned long long)cs);
+    phem_stream_t *st = nullptr;
+    die_phem(phem_stream_create(n, cs, PHEM_W_U32, &st), "phem_stream_create");
+    uint32_t *d = nullptr;
+    die_hip(hipMalloc(&d, 12 * cs), "hipMalloc (chunk)");
+    for (uint64_t off = 0; off < m; off += cs) {
+      const uint64_t k = m - off < cs ? m - off : cs;
+      die_hip(hipMemcpy(d, g.u + off, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
+      die_hip(hipMemcpy(d + cs, g.v + off, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
+      die_hip(hipMemcpy(d + 2 * cs, g.w + off, 4 * k, hipMemcpyHostToDevice),
+              "hipMemcpy");
+      die_phem(phem_stream_apply(st, d, d + cs, d + 2 * cs, k, nullptr),
+               "phem_stream_apply");
+    }
+    uint64_t fc = 0;
+    const uint32_t *fu, *fv, *fw;
+    const uint64_t *fg;
+    die_phem(phem_stream_forest(st, &fc, &fu, &fv, &fw, &fg), "phem_stream_forest");
+    std::vector<uint32_t> hu(fc), hv(fc), hw(fc);
+    if (fc) {
+      die_hip(hipMemcpy(hu.data(), fu, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
+      die_hip(hipMemcpy(hv.data(), fv, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
+      die_hip(hipMemcpy(hw.data(), fw, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    forest.resize(fc);
+    for (uint64_t i = 0; i < fc; i++) forest[i] = edge{hu[i], hv[i], hw[i]};
+    die_hip(hipFree(d), "hipFree");
+    die_phem(phem_stream_destroy(st), "phem_stream_destroy");
+    const uint64_t w = report_forest(forest.data(), fc, result_filename);
+    std::printf("final weight by streamline approach %llu\n", (unsigned long long)w);
+  } else {
+    if (use_malloc_managed)
+      std::printf("malloc managed: runs in device memory\n");
+    if (m > PHEM_MAX_EDGES) {
+      std::fprintf(stderr, "new_boruvka: %llu edges beyond the 32-bit id range\n",
+                   (unsigned long long)m);
+      std::exit(EXIT_FAILURE);
+    }
+    std::vector<edge> edges(m ? m : 1);
+    for (uint64_t i = 0; i < m; i++) edges[i] = edge{g.u[i], g.v[i], g.w[i]};
+    forest.resize(n ? n : 1);
+    vertex filled = 0;
+    std::printf("monolithic GPU approach\n");
+    boruvka_for_edge_array(edges.data(), n, (vertex)m, forest.data(), filled,
+                           result_filename);
+    const uint64_t w = report_forest(forest.data(), filled, result_filename);
+    std::printf("Total weight %llu\n", (unsigned long long)w);
+  }
+  phem_graph_file_free(&g);
 }
 
 // ---- CC ---------------------------------------------------------------------

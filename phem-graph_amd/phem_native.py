@@ -100,6 +100,12 @@ class ServerStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class GraphFile(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("wmode", ctypes.c_int32), ("m", ctypes.c_uint64),
+                ("u", ctypes.c_void_p), ("v", ctypes.c_void_p), ("w", ctypes.c_void_p),
+                ("edges", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -157,6 +163,12 @@ def lib():
         "phem_server_drain": (i32, [vp]),
         "phem_server_get_stats": (i32, [vp, c.POINTER(ServerStats)]),
         "phem_server_reset_stats": (i32, [vp]),
+        "phem_read_mtx": (i32, [c.c_char_p, i32, c.POINTER(GraphFile)]),
+        "phem_read_edge_list": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
+        "phem_read_ecl": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
+        "phem_read_pbbs_weighted": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
+        "phem_write_mtx_weighted": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
+        "phem_graph_file_free": (None, [c.POINTER(GraphFile)]),
         "phem_prof_enable": (i32, [i32]),
         "phem_prof_reset": (i32, []),
         "phem_prof_count": (i32, []),
@@ -522,3 +534,56 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def _take_graph(g):
+    """Copies a GraphFile's arrays into numpy and frees the library's."""
+    import numpy as np
+
+    def arr(ptr, dt, k):
+        if not ptr or k == 0:
+            return np.zeros(0, dt)
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(
+            ctypes.c_uint32 if dt == np.uint32 else ctypes.c_uint64)), shape=(k,)).copy()
+    out = {"n": g.n, "m": g.m, "wmode": g.wmode}
+    if g.u:
+        out["u"], out["v"] = arr(g.u, np.uint32, g.m), arr(g.v, np.uint32, g.m)
+        out["w"] = arr(g.w, np.uint32, g.m)
+    if g.edges:
+        out["edges"] = arr(g.edges, np.uint64, g.m)
+    lib().phem_graph_file_free(ctypes.byref(g))
+    return out
+
+
+def read_mtx(path, generate_weights=False):
+    g = GraphFile()
+    _check(lib().phem_read_mtx(os.fsencode(path), int(generate_weights), ctypes.byref(g)),
+           "phem_read_mtx")
+    return _take_graph(g)
+
+
+def read_edge_list(path):
+    g = GraphFile()
+    _check(lib().phem_read_edge_list(os.fsencode(path), ctypes.byref(g)), "phem_read_edge_list")
+    return _take_graph(g)
+
+
+def read_ecl(path):
+    g = GraphFile()
+    _check(lib().phem_read_ecl(os.fsencode(path), ctypes.byref(g)), "phem_read_ecl")
+    return _take_graph(g)
+
+
+def read_pbbs_weighted(path):
+    g = GraphFile()
+    _check(lib().phem_read_pbbs_weighted(os.fsencode(path), ctypes.byref(g)),
+           "phem_read_pbbs_weighted")
+    return _take_graph(g)
+
+
+def write_mtx_weighted(path, n, u, v, w):
+    import numpy as np
+    u, v, w = (np.ascontiguousarray(x, dtype=np.uint32) for x in (u, v, w))
+    g = GraphFile(n, W_U32, len(u), u.ctypes.data, v.ctypes.data, w.ctypes.data, None)
+    _check(lib().phem_write_mtx_weighted(os.fsencode(path), ctypes.byref(g)),
+           "phem_write_mtx_weighted")

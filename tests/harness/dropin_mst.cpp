@@ -43,3 +43,29 @@ extern "C" int harness_device_boruvka(const uint32_t *h_aos, uint32_t n,
   (void)hipFree(d_rep); (void)hipFree(d_new); (void)hipFree(d_it); (void)hipFree(d_out);
   return 0;
 }
+
+// new_boruvka on a MatrixMarket file; mode 0 plain, 1 streamline,
+// 2 heterogeneous streamline, 3 managed.
+extern "C" int harness_new_boruvka(const char *file, const char *result,
+                                   uint32_t chunk_size, int num_chunks,
+                                   int generate_weights, int mode) {
+  new_boruvka(file, result, false, chunk_size, num_chunks, generate_weights != 0,
+              mode == 3, mode == 1, false, mode == 2, false);
+  return 0;
+}
+
+extern "C" int harness_graph_info(const char *file, uint32_t *n, uint64_t *m) {
+  std::ifstream f(file);
+  vertex nn = 0;
+  large_vertex mm = 0;
+  getGraphInfo(f, nn, mm);
+  *n = nn;
+  *m = mm;
+  return 0;
+}
+
+extern "C" int harness_mmap_populate(const char *file, uint32_t *out_aos, uint32_t m,
+                                     uint32_t n, int generate_weights) {
+  mmapPopulateEdgeArray(file, (edge *)out_aos, m, n, generate_weights != 0);
+  return 0;
+}

@@ -121,3 +121,56 @@ def test_reference_cc_driver_over_dropin(HAST_LIB, name, share, batch):
     fv = (f & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     assert np.array_equal(O.bfs_labels(n, fu, fv), want)
     assert np.isin(f, rec).all()
+
+
+def _new_boruvka_lib(H):
+    H.harness_new_boruvka.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint32,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    H.harness_graph_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32),
+                                     ctypes.POINTER(ctypes.c_uint64)]
+    H.harness_mmap_populate.argtypes = [ctypes.c_char_p, _u32p, ctypes.c_uint32,
+                                        ctypes.c_uint32, ctypes.c_int]
+    return H
+
+
+def _mtx(tmp_path, n, a, b, w=None):
+    """A MatrixMarket file with 1-based pairs (a, b) and optional weights."""
+    p = tmp_path / "g.mtx"
+    lines = [f"{x} {y}" + ("" if w is None else f" {z}")
+             for x, y, z in zip(a, b, w if w is not None else a)]
+    p.write_text("%%MatrixMarket matrix coordinate integer general\n% test\n"
+                 f"{n} {n} {len(a)}\n" + "\n".join(lines) + "\n")
+    return str(p)
+
+
+@pytest.mark.parametrize("mode,chunk,nchunks", [(0, 0, 0), (1, 7000, 0), (1, 0, 5),
+                                                (2, 0, 3), (3, 0, 0)])
+@pytest.mark.parametrize("gen", [0, 1])
+def test_new_boruvka_matrix_market(H, tmp_path, mode, chunk, nchunks, gen):
+    """The reference's MST driver on a MatrixMarket file: the forest written
+    to the result file (u+1 v+1 w, ascending input order) equals the oracle's
+    Kruskal forest of the graph the reader loaded, with integer weights (many
+    ties) or the generated-weight rule, whole or in chunks."""
+    import phem_native as P
+    L = _new_boruvka_lib(H)
+    rng = np.random.default_rng(3 + gen)
+    n, m = 3000, 40000
+    a = rng.integers(1, n + 1, m)
+    b = rng.integers(1, n + 1, m)
+    w = rng.integers(0, 50, m)
+    path = _mtx(tmp_path, n, a, b, None if gen else w)
+    g = P.read_mtx(path, generate_weights=bool(gen))
+    ids, lab = O.kruskal(g["n"], g["u"], g["v"], g["w"], 0)
+    res = str(tmp_path / "forest.txt")
+    print(f"new_boruvka mode={mode} chunk={chunk} nchunks={nchunks} gen={gen}", flush=True)
+    assert L.harness_new_boruvka(path.encode(), res.encode(), chunk, nchunks, gen, mode) == 0
+    got = np.loadtxt(res, dtype=np.uint64, ndmin=2)
+    want = np.stack([g["u"][ids] + 1, g["v"][ids] + 1, g["w"][ids]], axis=1).astype(np.uint64)
+    assert got.shape == want.shape and np.array_equal(got, want)
+    # the header reader and the array reader agree with the library's reader
+    nn, mm = ctypes.c_uint32(0), ctypes.c_uint64(0)
+    L.harness_graph_info(path.encode(), ctypes.byref(nn), ctypes.byref(mm))
+    assert (nn.value, mm.value) == (n, m)
+    aos = np.zeros(3 * m, np.uint32)
+    L.harness_mmap_populate(path.encode(), aos, m, n, gen)
+    assert np.array_equal(aos.reshape(-1, 3), np.stack([g["u"], g["v"], g["w"]], axis=1))
