@@ -329,6 +329,15 @@ int phem_server_drain(phem_server *srv);
 int phem_server_get_stats(phem_server *srv, phem_server_stats *st);
 int phem_server_reset_stats(phem_server *srv);
 
+/* Minimum spanning forest on the host: Kruskal over (weight, id) with a
+ * multithreaded stable radix sort.  The CPU pre-filter of the heterogeneous
+ * chunked solve (MST/src/kernels.cu:1503-1788, where the reference calls its
+ * ParlayLib mst() on the last chunk).  ids_out (n slots) receives the forest's
+ * edge ids ascending, *count their number.  threads <= 0: all host threads. */
+int phem_cpu_msf(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
+                 const uint32_t *w, int wmode, uint32_t *ids_out,
+                 uint64_t *count, int threads);
+
 /* ---------------------------------------------------------------------------
  * Graph files (host memory).  The readers fill a phem_graph_file whose arrays
  * the library allocates; phem_graph_file_free releases them.

@@ -71,11 +71,12 @@ void mmapPopulateEdgeArray(const std::string &filename, edge *edges,
 /* The reference's MST driver (MST/src/new_boruvka.cu:330-510): reads a
  * MatrixMarket file and computes its minimum spanning forest.
  *  - default: one solve of the whole graph (boruvka_for_edge_array);
- *  - use_streamline / use_cpu_gpu_streamline: chunks of chunk_size edges
- *    (or num_chunks_ideal chunks) folded into a forest kept on the device,
- *    F = MSF(F u chunk) (MST/src/kernels.cu:849-1154); the CPU pre-filter of
- *    the heterogeneous variant is not part of this library, its chunks run
- *    on the GPU;
+ *  - use_streamline: chunks of chunk_size edges (or num_chunks_ideal
+ *    chunks) folded into a forest kept on the device, F = MSF(F u chunk)
+ *    (MST/src/kernels.cu:849-1154);
+ *  - use_cpu_gpu_streamline: the same, with the last chunk first reduced to
+ *    its own forest on the CPU (phem_cpu_msf) while the GPU folds in the
+ *    others (MST/src/kernels.cu:1155-1826);
  *  - use_malloc_managed: runs the default path (device memory);
  *  - use_cpu_only: the ParlayLib CPU path stays the caller's
  *    (cpu_compute_pbbs); this library prints a notice and returns;

@@ -1,0 +1,122 @@
+// phem_cpu.cpp -- the host side's own minimum spanning forest, used to
+// pre-filter a chunk on the CPU while the GPU streams the other chunks (the
+// reference's heterogeneous streamline, MST/src/kernels.cu:1503-1788: the
+// last chunk is replaced by its own forest before the GPU folds it in, which
+// keeps the result exact because an edge outside the forest of a subgraph is
+// the heaviest edge of a cycle and stays outside every larger forest).
+//
+// Kruskal: edge positions sorted stably by the 32-bit order key of the weight
+// (four 8-bit LSD passes; each pass histograms, scans and scatters with
+// several threads over contiguous blocks, so equal keys keep position order),
+// then one union-find sweep in that order.
+#include <algorithm>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "phem.h"
+#include "phem_internal.h"
+
+namespace phem {
+namespace {
+
+uint32_t order_key(uint32_t w, int wmode) {
+  if (wmode == PHEM_W_U32) return w;
+  return (w & 0x80000000u) ? ~w : (w | 0x80000000u);
+}
+
+// Stable LSD radix sort of positions 0..m-1 by key[pos].
+void radix_positions(const std::vector<uint32_t> &key, std::vector<uint32_t> &pos,
+                     unsigned T) {
+  const uint64_t m = key.size();
+  std::vector<uint32_t> tmp(m);
+  pos.resize(m);
+  for (uint64_t i = 0; i < m; i++) pos[i] = (uint32_t)i;
+  std::vector<uint64_t> hist((size_t)T * 256);
+  for (int shift = 0; shift < 32; shift += 8) {
+    std::fill(hist.begin(), hist.end(), 0);
+    auto block = [&](unsigned t, uint64_t *b, uint64_t *e) {
+      *b = m * t / T;
+      *e = m * (t + 1) / T;
+    };
+    auto count = [&](unsigned t) {
+      uint64_t b, e;
+      block(t, &b, &e);
+      uint64_t *h = &hist[(size_t)t * 256];
+      for (uint64_t i = b; i < e; i++) h[(key[pos[i]] >> shift) & 255u]++;
+    };
+    std::vector<std::thread> ts;
+    for (unsigned t = 1; t < T; t++) ts.emplace_back(count, t);
+    count(0);
+    for (auto &t : ts) t.join();
+    ts.clear();
+    // exclusive offsets: digit-major, then block order (stability)
+    uint64_t run = 0;
+    for (int d = 0; d < 256; d++)
+      for (unsigned t = 0; t < T; t++) {
+        const uint64_t c = hist[(size_t)t * 256 + d];
+        hist[(size_t)t * 256 + d] = run;
+        run += c;
+      }
+    auto scatter = [&](unsigned t) {
+      uint64_t b, e;
+      block(t, &b, &e);
+      uint64_t *h = &hist[(size_t)t * 256];
+      for (uint64_t i = b; i < e; i++) {
+        const uint32_t p = pos[i];
+        tmp[h[(key[p] >> shift) & 255u]++] = p;
+      }
+    };
+    for (unsigned t = 1; t < T; t++) ts.emplace_back(scatter, t);
+    scatter(0);
+    for (auto &t : ts) t.join();
+    pos.swap(tmp);
+  }
+}
+
+uint32_t find_root(std::vector<uint32_t> &par, uint32_t x) {
+  while (par[x] != x) {
+    par[x] = par[par[x]];
+    x = par[x];
+  }
+  return x;
+}
+
+}  // namespace
+}  // namespace phem
+
+using namespace phem;
+
+extern "C" int phem_cpu_msf(uint32_t n, uint64_t m, const uint32_t *u,
+                            const uint32_t *v, const uint32_t *w, int wmode,
+                            uint32_t *ids_out, uint64_t *count, int threads) {
+  if (!count) return set_err(PHEM_EINVAL, "count is NULL");
+  if (m > PHEM_MAX_EDGES) return set_err(PHEM_EINVAL, "m=%llu beyond limits",
+                                         (unsigned long long)m);
+  if (wmode != PHEM_W_U32 && wmode != PHEM_W_F32)
+    return set_err(PHEM_EINVAL, "unknown weight mode %d", wmode);
+  if (m && (!u || !v || !w)) return set_err(PHEM_EINVAL, "NULL edge array");
+  if (n && !ids_out) return set_err(PHEM_EINVAL, "ids_out is NULL");
+  unsigned T = threads > 0 ? (unsigned)threads : std::thread::hardware_concurrency();
+  T = std::max(1u, std::min(T, 64u));
+  if (m < (1u << 16)) T = 1;
+  std::vector<uint32_t> key(m), pos;
+  for (uint64_t i = 0; i < m; i++) key[i] = order_key(w[i], wmode);
+  radix_positions(key, pos, T);
+  std::vector<uint32_t> par(n);
+  for (uint32_t x = 0; x < n; x++) par[x] = x;
+  std::vector<uint32_t> chosen;
+  chosen.reserve(n ? n - 1 : 0);
+  for (uint64_t k = 0; k < m && chosen.size() + 1 < (uint64_t)n; k++) {
+    const uint32_t i = pos[k], a = u[i], b = v[i];
+    if (a == b || a >= n || b >= n) continue;
+    const uint32_t ra = find_root(par, a), rb = find_root(par, b);
+    if (ra == rb) continue;
+    par[ra > rb ? ra : rb] = ra > rb ? rb : ra;
+    chosen.push_back(i);
+  }
+  std::sort(chosen.begin(), chosen.end());
+  std::memcpy(ids_out, chosen.data(), 4 * chosen.size());
+  *count = chosen.size();
+  return PHEM_OK;
+}

@@ -4,7 +4,9 @@
 #include <cstdlib>
 #include <fstream>
 #include <mutex>
+#include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "phem.h"
@@ -249,22 +251,61 @@ void new_boruvka(std::string filename, std::string result_filename, bool debug,
   }
   std::vector<edge> forest;
   if (use_streamline || use_cpu_gpu_streamline) {
-    if (use_cpu_gpu_streamline)
-      std::printf("streamline: heterogeneous variant runs every chunk on the GPU\n");
     const uint64_t cs = chunk_size ? chunk_size : (m ? m : 1);
     std::printf("streamline chunk boruvka, chunk size %llu\n", (unsigned long long)cs);
     phem_stream_t *st = nullptr;
     die_phem(phem_stream_create(n, cs, PHEM_W_U32, &st), "phem_stream_create");
     uint32_t *d = nullptr;
     die_hip(hipMalloc(&d, 12 * cs), "hipMalloc (chunk)");
-    for (uint64_t off = 0; off < m; off += cs) {
-      const uint64_t k = m - off < cs ? m - off : cs;
-      die_hip(hipMemcpy(d, g.u + off, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
-      die_hip(hipMemcpy(d + cs, g.v + off, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
-      die_hip(hipMemcpy(d + 2 * cs, g.w + off, 4 * k, hipMemcpyHostToDevice),
-              "hipMemcpy");
+    auto apply = [&](const uint32_t *hu, const uint32_t *hv, const uint32_t *hw,
+                     uint64_t k) {
+      die_hip(hipMemcpy(d, hu, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
+      die_hip(hipMemcpy(d + cs, hv, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
+      die_hip(hipMemcpy(d + 2 * cs, hw, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
       die_phem(phem_stream_apply(st, d, d + cs, d + 2 * cs, k, nullptr),
                "phem_stream_apply");
+    };
+    const uint64_t nchunks = m ? (m + cs - 1) / cs : 0;
+    const uint64_t last = nchunks ? (nchunks - 1) * cs : 0;  // first edge of the last chunk
+    // heterogeneous: the CPU reduces the last chunk to its own forest while
+    // the GPU folds in the others (the first 10^8 edges of the chunk, as the
+    // reference caps its CPU share; the rest stays unfiltered)
+    std::vector<uint32_t> cpu_ids;
+    uint64_t cpu_cnt = 0, cpu_span = 0;
+    std::thread cpu;
+    if (use_cpu_gpu_streamline && nchunks > 1) {
+      cpu_span = std::min<uint64_t>(m - last, 100000000ull);
+      cpu_ids.resize(n ? n : 1);
+      cpu = std::thread([&] {
+        const unsigned hw = std::thread::hardware_concurrency();
+        die_phem(phem_cpu_msf(n, cpu_span, g.u + last, g.v + last, g.w + last,
+                              PHEM_W_U32, cpu_ids.data(), &cpu_cnt,
+                              (int)std::min(hw ? hw : 1u, 16u)),
+                 "phem_cpu_msf");
+      });
+    }
+    for (uint64_t c = 0; c < nchunks; c++) {
+      const uint64_t off = c * cs, k = m - off < cs ? m - off : cs;
+      if (c + 1 < nchunks || !cpu.joinable()) {
+        apply(g.u + off, g.v + off, g.w + off, k);
+        continue;
+      }
+      cpu.join();
+      std::vector<uint32_t> fu, fv, fw;  // filtered chunk, input order kept
+      for (uint64_t i = 0; i < cpu_cnt; i++) {
+        const uint64_t e = last + cpu_ids[i];
+        fu.push_back(g.u[e]);
+        fv.push_back(g.v[e]);
+        fw.push_back(g.w[e]);
+      }
+      for (uint64_t e = last + cpu_span; e < m; e++) {
+        fu.push_back(g.u[e]);
+        fv.push_back(g.v[e]);
+        fw.push_back(g.w[e]);
+      }
+      std::printf("%llu cut down to %llu\n", (unsigned long long)cpu_span,
+                  (unsigned long long)cpu_cnt);
+      apply(fu.data(), fv.data(), fw.data(), fu.size());
     }
     uint64_t fc = 0;
     const uint32_t *fu, *fv, *fw;

@@ -163,6 +163,7 @@ def lib():
         "phem_server_drain": (i32, [vp]),
         "phem_server_get_stats": (i32, [vp, c.POINTER(ServerStats)]),
         "phem_server_reset_stats": (i32, [vp]),
+        "phem_cpu_msf": (i32, [u32, u64, vp, vp, vp, i32, vp, c.POINTER(c.c_uint64), i32]),
         "phem_read_mtx": (i32, [c.c_char_p, i32, c.POINTER(GraphFile)]),
         "phem_read_edge_list": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
         "phem_read_ecl": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
@@ -587,3 +588,14 @@ def write_mtx_weighted(path, n, u, v, w):
     g = GraphFile(n, W_U32, len(u), u.ctypes.data, v.ctypes.data, w.ctypes.data, None)
     _check(lib().phem_write_mtx_weighted(os.fsencode(path), ctypes.byref(g)),
            "phem_write_mtx_weighted")
+
+
+def cpu_msf(n, u, v, w, wmode=W_F32, threads=0):
+    """Host minimum spanning forest (phem_cpu_msf): ascending edge ids."""
+    import numpy as np
+    u, v, w = (np.ascontiguousarray(x, dtype=np.uint32) for x in (u, v, w))
+    ids = np.zeros(max(n, 1), np.uint32)
+    cnt = ctypes.c_uint64(0)
+    _check(lib().phem_cpu_msf(n, len(u), u.ctypes.data, v.ctypes.data, w.ctypes.data, wmode,
+                              ids.ctypes.data, ctypes.byref(cnt), threads), "phem_cpu_msf")
+    return ids[:cnt.value].copy()

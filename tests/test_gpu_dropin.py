@@ -144,7 +144,8 @@ def _mtx(tmp_path, n, a, b, w=None):
 
 
 @pytest.mark.parametrize("mode,chunk,nchunks", [(0, 0, 0), (1, 7000, 0), (1, 0, 5),
-                                                (2, 0, 3), (3, 0, 0)])
+                                                (2, 0, 3), (2, 0, 1), (2, 9000, 0),
+                                                (3, 0, 0)])
 @pytest.mark.parametrize("gen", [0, 1])
 def test_new_boruvka_matrix_market(H, tmp_path, mode, chunk, nchunks, gen):
     """The reference's MST driver on a MatrixMarket file: the forest written
