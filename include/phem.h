@@ -52,6 +52,22 @@ extern "C" {
 #define PHEM_NO_EDGE 0xFFFFFFFFFFFFFFFFull /* empty spanning-forest slot */
 #define PHEM_MAX_ROUNDS 64
 
+/* Process-wide tunables (the autotuner's knobs).  Built-in defaults, the
+ * environment variables PHEM_BLOCKS_PER_CU, PHEM_MST_LIGHT_FACTOR and
+ * PHEM_CC_SAMPLE_FACTOR at first use, then phem_set_tuning.  Set them between
+ * solves, not while one runs.  Results do not depend on them: the forest,
+ * labels and weights are the same for every setting. */
+typedef struct {
+  uint32_t blocks_per_cu;  /* resident blocks per CU of the looping kernels
+                              (1..64; 0 in phem_set_tuning: unchanged) */
+  uint32_t reserved;
+  double mst_light_factor; /* default light_factor (NaN: unchanged) */
+  double cc_sample_factor; /* CC sampling phase: the first f*n records
+                              (<= 0: one phase; NaN: unchanged) */
+} phem_tuning;
+int phem_get_tuning(phem_tuning *t);
+int phem_set_tuning(const phem_tuning *t);
+
 /* Library / device information. */
 const char *phem_version(void);
 const char *phem_last_error(void);

@@ -159,13 +159,6 @@ extern "C" int phem_cc_labels(uint32_t n, const uint32_t *d_parent,
 }
 
 namespace phem {
-double cc_default_sample_factor() {
-  static double f = [] {
-    const char *e = getenv("PHEM_CC_SAMPLE_FACTOR");
-    return e ? atof(e) : 2.0;
-  }();
-  return f;
-}
 }  // namespace phem
 
 // Two phases when m is well above n (sampling idea of Afforest, Sutton et al.,

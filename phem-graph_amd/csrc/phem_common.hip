@@ -4,6 +4,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <cmath>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -23,24 +25,58 @@ int set_err(int code, const char *fmt, ...) {
   return code;
 }
 
+// Process-wide tunables: built-in defaults, overridden by the environment at
+// first use, then by phem_set_tuning.
+namespace {
+std::atomic<unsigned> g_blocks_per_cu{0};  // 0: not initialised
+std::atomic<double> g_light_factor{std::numeric_limits<double>::quiet_NaN()};
+std::atomic<double> g_sample_factor{std::numeric_limits<double>::quiet_NaN()};
+std::once_flag g_tune_once;
+
+void tuning_init() {
+  std::call_once(g_tune_once, [] {
+    unsigned per_cu = 16;
+    if (const char *e = getenv("PHEM_BLOCKS_PER_CU")) {
+      const int x = atoi(e);
+      if (x > 0 && x <= 64) per_cu = (unsigned)x;
+    }
+    unsigned z = 0;
+    g_blocks_per_cu.compare_exchange_strong(z, per_cu);
+    double lf = 1.0, sf = 2.0;
+    if (const char *e = getenv("PHEM_MST_LIGHT_FACTOR")) lf = atof(e);
+    if (const char *e = getenv("PHEM_CC_SAMPLE_FACTOR")) sf = atof(e);
+    // a value set before first use wins over the environment
+    double cur = g_light_factor.load();
+    if (std::isnan(cur)) g_light_factor.compare_exchange_strong(cur, lf);
+    cur = g_sample_factor.load();
+    if (std::isnan(cur)) g_sample_factor.compare_exchange_strong(cur, sf);
+  });
+}
+}  // namespace
+
 unsigned resident_blocks() {
-  static std::atomic<unsigned> cache[64];
+  static std::atomic<int> cus_cache[64];
+  tuning_init();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-  unsigned v = cache[dev].load(std::memory_order_relaxed);
-  if (v) return v;
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
-          hipSuccess || cus <= 0)
-    cus = 256;
-  unsigned per_cu = 16;
-  if (const char *e = getenv("PHEM_BLOCKS_PER_CU")) {
-    const int x = atoi(e);
-    if (x > 0 && x <= 64) per_cu = (unsigned)x;
+  int cus = cus_cache[dev].load(std::memory_order_relaxed);
+  if (!cus) {
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess || cus <= 0)
+      cus = 256;
+    cus_cache[dev].store(cus, std::memory_order_relaxed);
   }
-  v = (unsigned)cus * per_cu;
-  cache[dev].store(v, std::memory_order_relaxed);
-  return v;
+  return (unsigned)cus * g_blocks_per_cu.load(std::memory_order_relaxed);
+}
+
+double mst_default_light_factor() {
+  tuning_init();
+  return g_light_factor.load(std::memory_order_relaxed);
+}
+
+double cc_default_sample_factor() {
+  tuning_init();
+  return g_sample_factor.load(std::memory_order_relaxed);
 }
 
 // ---- per-kernel event profiler --------------------------------------------
@@ -156,5 +192,26 @@ extern "C" int phem_device_check(int *device_count) {
   if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
     return phem::set_err(PHEM_ENODEV, "device %d is %s, library built for gfx950",
                          dev, p.gcnArchName);
+  return PHEM_OK;
+}
+
+extern "C" int phem_get_tuning(phem_tuning *t) {
+  if (!t) return phem::set_err(PHEM_EINVAL, "t is NULL");
+  phem::tuning_init();
+  t->blocks_per_cu = phem::g_blocks_per_cu.load();
+  t->reserved = 0;
+  t->mst_light_factor = phem::g_light_factor.load();
+  t->cc_sample_factor = phem::g_sample_factor.load();
+  return PHEM_OK;
+}
+
+extern "C" int phem_set_tuning(const phem_tuning *t) {
+  if (!t) return phem::set_err(PHEM_EINVAL, "t is NULL");
+  if (t->blocks_per_cu > 64)
+    return phem::set_err(PHEM_EINVAL, "blocks_per_cu %u beyond 64", t->blocks_per_cu);
+  phem::tuning_init();
+  if (t->blocks_per_cu) phem::g_blocks_per_cu.store(t->blocks_per_cu);
+  if (!std::isnan(t->mst_light_factor)) phem::g_light_factor.store(t->mst_light_factor);
+  if (!std::isnan(t->cc_sample_factor)) phem::g_sample_factor.store(t->cc_sample_factor);
   return PHEM_OK;
 }

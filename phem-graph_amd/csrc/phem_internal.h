@@ -19,6 +19,10 @@ inline uint64_t align_up(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
 // environment variable PHEM_BLOCKS_PER_CU.
 unsigned resident_blocks();
 
+// Tunable defaults (phem_set_tuning, else the environment, else built-in).
+double mst_default_light_factor();
+double cc_default_sample_factor();
+
 // Grid size for `work` items at `per_block` items per block, capped at the
 // resident block count: kernels loop over their work, so blocks stay alive
 // across many tiles instead of the dispatcher launching one block per tile.

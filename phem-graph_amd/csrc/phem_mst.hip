@@ -721,13 +721,6 @@ extern "C" int phem_mst_workspace_bytes(uint32_t n, uint64_t m,
 
 namespace phem {
 
-double mst_default_light_factor() {
-  static double f = [] {
-    const char *e = getenv("PHEM_MST_LIGHT_FACTOR");
-    return e ? atof(e) : 1.0;
-  }();
-  return f;
-}
 
 template <class Src>
 static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,

@@ -106,6 +106,11 @@ class GraphFile(ctypes.Structure):
                 ("edges", ctypes.c_void_p)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("mst_light_factor", ctypes.c_double), ("cc_sample_factor", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -163,6 +168,8 @@ def lib():
         "phem_server_drain": (i32, [vp]),
         "phem_server_get_stats": (i32, [vp, c.POINTER(ServerStats)]),
         "phem_server_reset_stats": (i32, [vp]),
+        "phem_get_tuning": (i32, [c.POINTER(Tuning)]),
+        "phem_set_tuning": (i32, [c.POINTER(Tuning)]),
         "phem_cpu_msf": (i32, [u32, u64, vp, vp, vp, i32, vp, c.POINTER(c.c_uint64), i32]),
         "phem_read_mtx": (i32, [c.c_char_p, i32, c.POINTER(GraphFile)]),
         "phem_read_edge_list": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
@@ -599,3 +606,18 @@ def cpu_msf(n, u, v, w, wmode=W_F32, threads=0):
     _check(lib().phem_cpu_msf(n, len(u), u.ctypes.data, v.ctypes.data, w.ctypes.data, wmode,
                               ids.ctypes.data, ctypes.byref(cnt), threads), "phem_cpu_msf")
     return ids[:cnt.value].copy()
+
+
+def get_tuning():
+    t = Tuning()
+    _check(lib().phem_get_tuning(ctypes.byref(t)), "phem_get_tuning")
+    return {"blocks_per_cu": t.blocks_per_cu, "mst_light_factor": t.mst_light_factor,
+            "cc_sample_factor": t.cc_sample_factor}
+
+
+def set_tuning(blocks_per_cu=0, mst_light_factor=None, cc_sample_factor=None):
+    """Sets the process-wide knobs (None / 0: unchanged)."""
+    nan = float("nan")
+    t = Tuning(blocks_per_cu or 0, 0, nan if mst_light_factor is None else mst_light_factor,
+               nan if cc_sample_factor is None else cc_sample_factor)
+    _check(lib().phem_set_tuning(ctypes.byref(t)), "phem_set_tuning")

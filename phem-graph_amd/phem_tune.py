@@ -1,0 +1,176 @@
+"""Autotuner for the library's runtime knobs (phem_set_tuning):
+
+- blocks_per_cu: resident 256-thread blocks per compute unit for the looping
+  kernels (grid size of every edge and vertex pass);
+- mst_light_factor: size of the light phase of the minimum spanning forest
+  solve (edges below the filter threshold ~ factor * n), i.e. where the
+  filter-Kruskal split falls;
+- cc_sample_factor: size of the sampling phase of the connected-components
+  solve (first factor * n records).
+
+tune() builds a synthetic graph of the caller's shape on the device (RMAT or
+uniform, n vertices, m edges), then improves one knob at a time (coordinate
+descent over a small grid), timing each setting with HIP events on the
+current stream over a few solves.  Every setting must give the same forest,
+weight and labels as the first (the results do not depend on the knobs); a
+difference raises.  The winner is stored per device and size class in a JSON
+cache; load() applies the entry nearest to a graph's (log2 n, log2 m).
+
+    python -m phem_tune --n-log2 24 --m-log2 28      # tune, store, print
+"""
+import argparse
+import json
+import math
+import os
+
+import phem_native as P
+
+CACHE = os.environ.get("PHEM_TUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "phem", "tune.json"))
+
+GRID = {
+    "blocks_per_cu": [4, 8, 16, 32],
+    "mst_light_factor": [0.25, 0.5, 1.0, 2.0],
+    "cc_sample_factor": [1.0, 1.5, 2.0, 3.0],
+}
+
+
+def _device_name():
+    import torch
+    return torch.cuda.get_device_name(0)
+
+
+def size_class(n, m):
+    return [round(math.log2(max(n, 1)), 1), round(math.log2(max(m, 1)), 1)]
+
+
+def nearest(entries, device, n, m):
+    """The stored entry for `device` whose size class is closest to (n, m)."""
+    sc = size_class(n, m)
+    best, bd = None, None
+    for e in entries:
+        if e.get("device") != device:
+            continue
+        d = abs(e["size_class"][0] - sc[0]) + abs(e["size_class"][1] - sc[1])
+        if bd is None or d < bd:
+            best, bd = e, d
+    return best
+
+
+def _read(path):
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data if isinstance(data, list) else []
+    except (OSError, ValueError):
+        return []
+
+
+def save(result, path=CACHE):
+    entries = [e for e in _read(path)
+               if not (e.get("device") == result["device"]
+                       and e.get("size_class") == result["size_class"])]
+    entries.append(result)
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(entries, f, indent=1)
+    os.replace(tmp, path)
+
+
+def load(n, m, path=CACHE, apply=True, device=None):
+    """Applies (and returns) the stored configuration nearest to (n, m)."""
+    e = nearest(_read(path), device or _device_name(), n, m)
+    if e is not None and apply:
+        P.set_tuning(**e["config"])
+    return None if e is None else e["config"]
+
+
+def tune(n, m, graph="rmat", seed=1, reps=3, grid=None, device="cuda", verbose=False):
+    import torch
+    grid = grid or GRID
+    dev = torch.device(device)
+    if graph == "rmat":
+        scale = max(1, int(round(math.log2(n))))
+        n = 1 << scale
+        u, v, w = P.gen_rmat(scale, m, seed, positive=True, device=dev)
+    else:
+        u, v, w = P.gen_uniform(n, m, seed, device=dev)
+    e = P.pack_edges(u, v)
+    solver = P.MstSolver(n, m, device=dev)
+    stream = torch.cuda.current_stream()
+    ref = {}
+
+    def run_mst():
+        ids, lab, st = solver.solve(u, v, w, P.W_F32)
+        return (st["mst_count"], st["weight_fx"], int(lab.sum().item()))
+
+    def run_cc():
+        lab, _, _ = P.cc_solve(n, e, forest=False)
+        return int(lab.sum().item()) + int((lab == torch.arange(n, device=dev,
+                                                               dtype=torch.int32)).sum())
+
+    def timed(fn, what):
+        out = fn()  # warm-up, and the result check
+        if what in ref and ref[what] != out:
+            raise P.PhemError(f"tuning changed the {what} result: {out} != {ref[what]}")
+        ref.setdefault(what, out)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        ts = []
+        for _ in range(reps):
+            a.record(stream)
+            fn()
+            b.record(stream)
+            b.synchronize()
+            ts.append(a.elapsed_time(b))
+        return sorted(ts)[len(ts) // 2]
+
+    start = P.get_tuning()
+    best = dict(start)
+    log = []
+
+    def cost(knob):
+        if knob == "mst_light_factor":
+            return timed(run_mst, "mst")
+        if knob == "cc_sample_factor":
+            return timed(run_cc, "cc")
+        return timed(run_mst, "mst") + timed(run_cc, "cc")
+
+    try:
+        for knob in ("blocks_per_cu", "mst_light_factor", "cc_sample_factor"):
+            scores = {}
+            for val in grid[knob]:
+                P.set_tuning(**{**best, knob: val})
+                scores[val] = cost(knob)
+                log.append({"knob": knob, "value": val, "ms": round(scores[val], 4)})
+                if verbose:
+                    print(f"# {knob}={val}: {scores[val]:.3f} ms", flush=True)
+            best[knob] = min(scores, key=scores.get)
+        P.set_tuning(**best)
+        total = timed(run_mst, "mst") + timed(run_cc, "cc")
+    except Exception:
+        P.set_tuning(**start)
+        raise
+    return {"device": _device_name(), "size_class": size_class(n, m), "n": n, "m": m,
+            "graph": graph, "config": best, "mst_cc_ms": round(total, 4), "log": log}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--n-log2", type=int, default=24)
+    ap.add_argument("--m-log2", type=int, default=28)
+    ap.add_argument("--graph", choices=["rmat", "uniform"], default="rmat")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--cache", default=CACHE)
+    ap.add_argument("--no-save", action="store_true")
+    a = ap.parse_args()
+    P.require_device()
+    r = tune(1 << a.n_log2, 1 << a.m_log2, a.graph, reps=a.reps, verbose=True)
+    if not a.no_save:
+        save(r, a.cache)
+    print(json.dumps(r))
+
+
+if __name__ == "__main__":
+    main()
