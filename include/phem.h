@@ -117,8 +117,9 @@ typedef struct phem_mst_stats {
  *   the rest (filter-Kruskal, as the CPU path does, MST/src/utils.cu:794-841).
  *   <= 0 runs the canonical Boruvka round loop over all edges
  *   (MST/src/kernels.cu:507-689); its per-round log is then the reference's.
- *   NaN, like NULL options: 1.0, or the environment variable
- *   PHEM_MST_LIGHT_FACTOR.
+ *   NaN, like NULL options: the mst_light_factor knob (phem_set_tuning;
+ *   1.0 unless set or given by the environment variable
+ *   PHEM_MST_LIGHT_FACTOR).
  * flags: PHEM_MST_MAXIMUM or 0. */
 typedef struct phem_mst_opts {
   double light_factor;

@@ -35,7 +35,7 @@ std::once_flag g_tune_once;
 
 void tuning_init() {
   std::call_once(g_tune_once, [] {
-    unsigned per_cu = 16;
+    unsigned per_cu = 32;  // measured best at RMAT 2^24 / 2^28 (phem_tune)
     if (const char *e = getenv("PHEM_BLOCKS_PER_CU")) {
       const int x = atoi(e);
       if (x > 0 && x <= 64) per_cu = (unsigned)x;

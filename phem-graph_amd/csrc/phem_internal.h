@@ -14,9 +14,9 @@ int set_err(int code, const char *fmt, ...);
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 inline uint64_t align_up(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
 
-// Blocks launched for the library's looping kernels: compute units x 16
-// (256-thread blocks; the dispatcher backfills as blocks finish), or x the
-// environment variable PHEM_BLOCKS_PER_CU.
+// Blocks launched for the library's looping kernels: compute units x the
+// blocks_per_cu knob (32 by default; 256-thread blocks, the dispatcher
+// backfills as blocks finish).
 unsigned resident_blocks();
 
 // Tunable defaults (phem_set_tuning, else the environment, else built-in).
