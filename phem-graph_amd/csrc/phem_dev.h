@@ -38,6 +38,11 @@ __device__ __forceinline__ uint64_t make_key(uint32_t w, uint64_t id, int ord) {
 __device__ __forceinline__ uint32_t key_id(uint64_t key, int ord) {
   return (ord & ORD_MAX) ? 0xFFFFFFFEu - (uint32_t)key : (uint32_t)key;
 }
+// the weight bit pattern a key was made from
+__device__ __forceinline__ uint32_t key_weight(uint64_t key, int ord) {
+  const uint32_t k = (uint32_t)(key >> 32);
+  return wkey_inv((ord & ORD_MAX) ? ~k : k, ord & ORD_F32);
+}
 
 // ---- wave64 helpers --------------------------------------------------------
 

@@ -57,6 +57,7 @@ constexpr int WPT = 8;         // bitmap words per thread
 constexpr int WPB = BLK * WPT; // bitmap words per block
 constexpr int IPT = 4;         // components per thread in hook / compress
 constexpr int LTILE = BLK * IPT;
+constexpr int SPEC = 3;  // Boruvka rounds launched per counter read
 constexpr uint32_t GSUM_HEAVY = 8192;  // giant-summary words in LDS (32 KB)
 
 // Counter area (ull), every counter on its own 128-byte line (CSTR ull):
@@ -210,9 +211,13 @@ template <class Src>
 __global__ __launch_bounds__(BLK) void k_hook(
     const uint32_t *__restrict__ list, const ull *list_cnt, uint64_t list_cap,
     uint32_t n_ident, ull *best, Src src, int ord, uint32_t *par,
-    uint32_t *bitmap, uint32_t *act, ull *act_cnt, uint64_t act_cap) {
+    uint32_t *bitmap, uint32_t *act, ull *act_cnt, uint64_t act_cap,
+    ull *wsum_parts, ull *inexact_parts) {
   __shared__ SegView sv;
   if (list_cnt) seg_load<LTILE>(sv, list_cnt);
+  // exact weight of the forest edges this thread hooks (each forest edge is
+  // hooked exactly once: keys are unique and the chosen edges form a forest)
+  ull wsum = 0, wbad = 0;
   const uint32_t ntiles_ident = (uint32_t)ceil_div_dev(n_ident, LTILE);
   for (uint32_t v = blockIdx.x;; v += gridDim.x) {
     uint64_t start;
@@ -239,7 +244,14 @@ __global__ __launch_bounds__(BLK) void k_hook(
           active[j] = true;
           best[c[j]] = KEY_NONE;  // reset for the next round's sweep
           uint32_t e = key_id(key, ord);
-          atomicOr(&bitmap[e >> 5], 1u << (e & 31));
+          const uint32_t bit = 1u << (e & 31);
+          // a component whose edge was already marked (both sides chose the
+          // same edge) adds no weight: only the atomicOr that sets the bit does
+          if (!(atomicOr(&bitmap[e >> 5], bit) & bit)) {
+            ull fx;
+            if (weight_fx(key_weight(key, ord), ord & ORD_F32, &fx)) wsum += fx;
+            else wbad++;
+          }
           uf_union(par, src.U(e), src.V(e));
         }
       }
@@ -250,6 +262,27 @@ __global__ __launch_bounds__(BLK) void k_hook(
 #pragma unroll
     for (int j = 0; j < IPT; j++)
       if (active[j]) act[slot[j]] = c[j];
+  }
+  __shared__ ull s_w[2][BLK / WAVE];
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    wsum += __shfl_xor(wsum, d);
+    wbad += __shfl_xor(wbad, d);
+  }
+  if (lane_id() == 0) {
+    s_w[0][threadIdx.x / WAVE] = wsum;
+    s_w[1][threadIdx.x / WAVE] = wbad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ull a = 0, b = 0;
+    for (int w = 0; w < BLK / WAVE; w++) {
+      a += s_w[0][w];
+      b += s_w[1][w];
+    }
+    const uint32_t seg = blockIdx.x % KSEG;
+    if (a) atomicAdd(wsum_parts + seg * CSTR, a);
+    if (b) atomicAdd(inexact_parts + seg * CSTR, b);
   }
 }
 
@@ -466,6 +499,7 @@ __global__ __launch_bounds__(BLK) void k_giant_emit(
   const uint32_t G = *giant_root;
   const uint32_t ntiles = (uint32_t)ceil_div_dev(n, LTILE);
   uint32_t emitted = 0;  // by this thread
+  ull gbest = KEY_NONE;  // lightest emitted key seen by this thread (for G)
   for (uint32_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
     bool keep[IPT];
     ull key[IPT];
@@ -478,7 +512,7 @@ __global__ __launch_bounds__(BLK) void k_giant_emit(
       if (keep[j]) {
         emitted++;
         min_store(best, (uint32_t)x, key[j]);
-        min_store(best, G, key[j]);
+        gbest = key[j] < gbest ? key[j] : gbest;
       }
     }
     const uint32_t seg = v % KSEG;
@@ -494,16 +528,28 @@ __global__ __launch_bounds__(BLK) void k_giant_emit(
       }
     }
   }
-  __shared__ ull s_w[BLK / WAVE];
+  // one count and one giant-side offer per block
+  __shared__ ull s_w[BLK / WAVE], s_g[BLK / WAVE];
   ull x = emitted;
 #pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
-  if (lane_id() == 0) s_w[threadIdx.x / WAVE] = x;
+  for (int d = 1; d < WAVE; d <<= 1) {
+    x += __shfl_xor(x, d);
+    const ull o = __shfl_xor(gbest, d);
+    gbest = o < gbest ? o : gbest;
+  }
+  if (lane_id() == 0) {
+    s_w[threadIdx.x / WAVE] = x;
+    s_g[threadIdx.x / WAVE] = gbest;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    ull tot = 0;
-    for (int w = 0; w < BLK / WAVE; w++) tot += s_w[w];
+    ull tot = 0, g = KEY_NONE;
+    for (int w = 0; w < BLK / WAVE; w++) {
+      tot += s_w[w];
+      g = s_g[w] < g ? s_g[w] : g;
+    }
     if (tot) atomicAdd(parts + (blockIdx.x % KSEG) * CSTR, tot);
+    if (g != KEY_NONE) min_store(best, G, g);
   }
 }
 
@@ -595,16 +641,11 @@ __global__ __launch_bounds__(BLK) void k_scan_blocks(ull *blk, uint64_t nb,
   if (threadIdx.x == 0) *sum = carry;
 }
 
-// Bitmap -> ascending ids; exact weight sum of the forest (partial sums
-// spread over KSEG counters).
-template <class Src>
+// Bitmap -> ascending ids (the exact weight is summed by k_hook).
 __global__ __launch_bounds__(BLK) void k_bm_emit(const uint32_t *__restrict__ bm,
                                                  uint64_t words,
                                                  const ull *__restrict__ blk,
-                                                 uint32_t *__restrict__ ids,
-                                                 Src src, int wmode,
-                                                 ull *wsum_parts,
-                                                 ull *inexact_parts) {
+                                                 uint32_t *__restrict__ ids) {
   const uint64_t w0 = (uint64_t)blockIdx.x * WPB + (uint64_t)threadIdx.x * WPT;
   uint32_t word[WPT];
   ull c = 0;
@@ -615,35 +656,24 @@ __global__ __launch_bounds__(BLK) void k_bm_emit(const uint32_t *__restrict__ bm
   }
   ull tot;
   ull pos = blk[blockIdx.x] + block_excl_scan(c, &tot);
-  ull s = 0, bad = 0;
 #pragma unroll
   for (int k = 0; k < WPT; k++) {
     uint32_t x = word[k];
     while (x) {
       uint32_t bit = __ffs(x) - 1;
       x &= x - 1;
-      uint32_t e = (uint32_t)((w0 + k) * 32 + bit);
-      ids[pos++] = e;
-      ull fx;
-      if (weight_fx(src.W(e), wmode, &fx)) s += fx; else bad++;
+      ids[pos++] = (uint32_t)((w0 + k) * 32 + bit);
     }
-  }
-  ull ts, tb;
-  block_excl_scan(s, &ts);
-  block_excl_scan(bad, &tb);
-  if (threadIdx.x == 0) {
-    const uint32_t seg = blockIdx.x % KSEG;
-    if (ts) atomicAdd(wsum_parts + seg * CSTR, ts);
-    if (tb) atomicAdd(inexact_parts + seg * CSTR, tb);
   }
 }
 
 // Sums of counters for the host: value i is ctr[off[i]] (segs[i] == 1) or
 // the sum of the KSEG segment counters at off[i].  One wave.
+constexpr int MAX_SUMS = 16;  // values per k_seg_sums call (the C_SUMS slot)
 struct SumReq {
   int n;
-  int off[8];
-  int segs[8];
+  int off[MAX_SUMS];
+  int segs[MAX_SUMS];
 };
 
 __global__ __launch_bounds__(WAVE) void k_seg_sums(const ull *ctr, SumReq q,
@@ -782,15 +812,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   // Counter values for the host: summed on the device, copied to a pinned
   // buffer (one per host thread, kept for the life of the process).
   static thread_local ull *hpin = nullptr;
-  if (!hpin) PHEM_HIP(hipHostMalloc((void **)&hpin, 8 * sizeof(ull)));
-  auto read_sums = [&](std::initializer_list<std::pair<int, int>> req,
-                       ull *out) -> int {
-    SumReq q{};
-    for (auto &p : req) {
-      q.off[q.n] = p.first;
-      q.segs[q.n] = p.second;
-      q.n++;
-    }
+  if (!hpin) PHEM_HIP(hipHostMalloc((void **)&hpin, MAX_SUMS * sizeof(ull)));
+  auto read_sum_req = [&](const SumReq &q, ull *out) -> int {
     hipLaunchKernelGGL(k_seg_sums, dim3(1), dim3(WAVE), 0, s, ctr, q,
                        ctr + C_SUMS);
     PHEM_LAUNCHED("k_seg_sums");
@@ -799,6 +822,16 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     PHEM_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < q.n; i++) out[i] = hpin[i];
     return PHEM_OK;
+  };
+  auto read_sums = [&](std::initializer_list<std::pair<int, int>> req,
+                       ull *out) -> int {
+    SumReq q{};
+    for (auto &p : req) {
+      q.off[q.n] = p.first;
+      q.segs[q.n] = p.second;
+      q.n++;
+    }
+    return read_sum_req(q, out);
   };
   ull hv[8];
   int r = 0;    // round index (a round = hook + compress + compact)
@@ -811,49 +844,79 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   auto run_rounds = [&](uint64_t list_ub, bool first_all) -> int {
     bool all = first_all;
     while (m_r > 0) {
-      if (r >= PHEM_MAX_ROUNDS)
+      // Up to SPEC rounds per counter read.  A round launched after the
+      // phase has converged finds empty inputs and leaves its counters at
+      // zero; grids come from upper bounds (alive edges and component lists
+      // only shrink within a phase), and the kernels loop over the tiles the
+      // device counters hold.
+      const int batch = PHEM_MAX_ROUNDS - r < SPEC ? PHEM_MAX_ROUNDS - r : SPEC;
+      if (batch <= 0)
         return set_err(PHEM_ENOCONV, "no convergence after %d rounds", r);
-      const uint64_t hook_tiles =
-          all ? ceil_div(n, LTILE) : ceil_div(list_ub, LTILE) + KSEG;
-      {
-        PHEM_PROF("k_hook", s);
-        hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(hook_tiles, 1)),
-                           dim3(BLK), 0, s, listL, all ? noull : ctr + cL(r),
-                           L.lcap, all ? n : 0u, best, src, ord, par, bitmap,
-                           listA,
-                           ctr + cA(r), L.lcap);
+      const uint64_t list_bound = all ? n : list_ub;
+      for (int k = 0; k < batch; k++) {
+        const int rr = r + k;
+        const int c = cur ^ (k & 1);
+        const bool ident = all && k == 0;
+        const uint64_t hook_tiles =
+            ident ? ceil_div(n, LTILE) : ceil_div(list_bound, LTILE) + KSEG;
+        {
+          PHEM_PROF("k_hook", s);
+          hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(hook_tiles, 1)),
+                             dim3(BLK), 0, s, listL,
+                             ident ? noull : ctr + cL(rr), L.lcap,
+                             ident ? n : 0u, best, src, ord, par, bitmap, listA,
+                             ctr + cA(rr), L.lcap, ctr + cPart(1),
+                             ctr + cPart(2));
+        }
+        PHEM_LAUNCHED("k_hook");
+        {
+          PHEM_PROF("k_compress", s);
+          hipLaunchKernelGGL(k_compress, dim3(grid_for(hook_tiles + KSEG, 1)),
+                             dim3(BLK), 0, s, listA, ctr + cA(rr), L.lcap, par,
+                             listL, ctr + cL(rr + 1), L.lcap);
+        }
+        PHEM_LAUNCHED("k_compress");
+        {
+          PHEM_PROF("k_compact", s);
+          hipLaunchKernelGGL((k_pass<P_EDGES, Src>),
+                             dim3(grid_for(ceil_div(m_r, TILE) + KSEG, 1)),
+                             dim3(BLK), 0, s, src, Ea[c], Eb[c], Ek[c],
+                             ctr + cE(rr), L.ecap, 0ull, n, ord, par, noull,
+                             nou, Ea[c ^ 1], Eb[c ^ 1], Ek[c ^ 1],
+                             ctr + cE(rr + 1), L.ecap, (ull *)nullptr, best,
+                             (ull *)nullptr, nou);
+        }
+        PHEM_LAUNCHED("k_compact");
       }
-      PHEM_LAUNCHED("k_hook");
-      {
-        PHEM_PROF("k_compress", s);
-        hipLaunchKernelGGL(k_compress, dim3(grid_for(hook_tiles + KSEG, 1)),
-                           dim3(BLK), 0, s, listA, ctr + cA(r), L.lcap, par,
-                           listL, ctr + cL(r + 1), L.lcap);
+      // E(r .. r+batch), A(r .. r+batch-1), L(r+batch)
+      SumReq q{};
+      for (int k = 0; k <= batch; k++) {
+        q.off[q.n] = cE(r + k);
+        q.segs[q.n++] = KSEG;
       }
-      PHEM_LAUNCHED("k_compress");
-      {
-        PHEM_PROF("k_compact", s);
-        hipLaunchKernelGGL((k_pass<P_EDGES, Src>),
-                           dim3(grid_for(ceil_div(m_r, TILE) + KSEG, 1)),
-                           dim3(BLK), 0, s, src, Ea[cur], Eb[cur], Ek[cur],
-                           ctr + cE(r), L.ecap, 0ull, n, ord, par, noull, nou,
-                           Ea[cur ^ 1], Eb[cur ^ 1], Ek[cur ^ 1],
-                           ctr + cE(r + 1), L.ecap, (ull *)nullptr, best,
-                           (ull *)nullptr, nou);
+      for (int k = 0; k < batch; k++) {
+        q.off[q.n] = cA(r + k);
+        q.segs[q.n++] = KSEG;
       }
-      PHEM_LAUNCHED("k_compact");
-      int rc = read_sums({{cE(r), KSEG}, {cA(r), KSEG}, {cE(r + 1), KSEG},
-                          {cL(r + 1), KSEG}}, hv);
+      q.off[q.n] = cL(r + batch);
+      q.segs[q.n++] = KSEG;
+      ull got[MAX_SUMS];
+      int rc = read_sum_req(q, got);
       if (rc) return rc;
-      if (stats) {
-        stats->m_log[r] = hv[0];
-        stats->c_log[r] = hv[1];
+      int real = 0;  // rounds that started with at least one alive edge
+      while (real < batch && got[real] > 0) {
+        if (stats) {
+          stats->m_log[r + real] = got[real];
+          stats->c_log[r + real] = got[batch + 1 + real];
+        }
+        real++;
       }
-      m_r = hv[2];      // E(r + 1)
-      list_ub = hv[3];  // L(r + 1)
-      cur ^= 1;
-      r++;
+      m_r = got[batch];        // E(r + batch); 0 when the phase converged
+      list_ub = got[2 * batch + 1];
+      cur ^= (real & 1);
+      r += real;
       all = false;
+      if (real < batch) m_r = 0;
     }
     return PHEM_OK;
   };
@@ -871,8 +934,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       PHEM_PROF("k_hook", s);
       hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(t0, 1)), dim3(BLK), 0, s,
                          listL, noull, L.lcap, n, best, src, ord, par, bitmap,
-                         listA,
-                         ctr + cA(0), L.lcap);
+                         listA, ctr + cA(0), L.lcap, ctr + cPart(1),
+                         ctr + cPart(2));
     }
     PHEM_LAUNCHED("k_hook");
     {
@@ -1018,9 +1081,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   PHEM_LAUNCHED("k_scan_blocks");
   {
     PHEM_PROF("k_bm_emit", s);
-    hipLaunchKernelGGL(k_bm_emit<Src>, dim3((unsigned)L.nb), dim3(BLK), 0, s,
-                       bitmap, L.words, blk, d_mst_ids, src, wmode,
-                       ctr + cPart(1), ctr + cPart(2));
+    hipLaunchKernelGGL(k_bm_emit, dim3((unsigned)L.nb), dim3(BLK), 0, s,
+                       bitmap, L.words, blk, d_mst_ids);
   }
   PHEM_LAUNCHED("k_bm_emit");
   int rc = read_sums({{C_MSTCNT, 1}, {cPart(1), KSEG}, {cPart(2), KSEG},

@@ -12,11 +12,14 @@ constexpr int SPLIT_BLK = 256;
 constexpr int NSAMP = 4096;  // threshold sample size
 constexpr int SBLK = 1024;   // one block sorts the sample
 
-// In-LDS bitonic sort of NSAMP keys by SBLK threads.
+constexpr int GSAMP = 1024;  // giant-root sample size
+
+// In-LDS bitonic sort of N keys (a power of two) by SBLK threads.
+template <int N = NSAMP>
 __device__ __forceinline__ void bitonic_sort_lds(ull *s) {
-  for (int k = 2; k <= NSAMP; k <<= 1) {
+  for (int k = 2; k <= N; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < NSAMP; i += SBLK) {
+      for (int i = threadIdx.x; i < N; i += SBLK) {
         int ixj = i ^ j;
         if (ixj > i) {
           ull x = s[i], y = s[ixj];
@@ -41,21 +44,21 @@ __global__ __launch_bounds__(SPLIT_BLK) void k_flatten(uint32_t n, uint32_t *par
   }
 }
 
-// The most frequent root among NSAMP strided vertices (after k_flatten).
+// The most frequent root among GSAMP strided vertices (after k_flatten).
 __global__ __launch_bounds__(SBLK) void k_pick_giant(uint32_t n,
                                                      const uint32_t *par,
                                                      uint32_t *giant_out) {
-  __shared__ ull s[NSAMP];
-  for (int j = threadIdx.x; j < NSAMP; j += SBLK)
-    s[j] = par[(uint64_t)j * n / NSAMP];
+  __shared__ ull s[GSAMP];
+  for (int j = threadIdx.x; j < GSAMP; j += SBLK)
+    s[j] = par[(uint64_t)j * n / GSAMP];
   __syncthreads();
-  bitonic_sort_lds(s);
+  bitonic_sort_lds<GSAMP>(s);
   // every run start finds its run's end by binary search; the longest run
   // (smallest root on ties) wins
   ull best_len = 0, best_val = ~0ull;
-  for (int j = threadIdx.x; j < NSAMP; j += SBLK) {
+  for (int j = threadIdx.x; j < GSAMP; j += SBLK) {
     if (j > 0 && s[j] == s[j - 1]) continue;
-    int lo = j, hi = NSAMP;  // first index > j with a different value
+    int lo = j, hi = GSAMP;  // first index > j with a different value
     while (lo + 1 < hi) {
       const int mid = (lo + hi) >> 1;
       if (s[mid] == s[j]) lo = mid; else hi = mid;
