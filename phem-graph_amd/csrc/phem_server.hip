@@ -259,11 +259,32 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
     if (e != hipSuccess) rc = set_err(PHEM_EHIP, "result copy: %s", hipGetErrorString(e));
   }
   if (rc == PHEM_OK) {
-    for (uint32_t g = 0; g < k; g++) {
-      const phem_request &q = batch[g]->q;
-      if (q.forest_ids && h_cnt[g])
-        std::memcpy(q.forest_ids, h_ids + voff[g], 4ull * h_cnt[g]);
-      if (q.labels && q.n) std::memcpy(q.labels, h_lab + voff[g], 4ull * q.n);
+    auto scatter = [&](uint32_t g0, uint32_t g1) {
+      for (uint32_t g = g0; g < g1; g++) {
+        const phem_request &q = batch[g]->q;
+        if (q.forest_ids && h_cnt[g])
+          std::memcpy(q.forest_ids, h_ids + voff[g], 4ull * h_cnt[g]);
+        if (q.labels && q.n) std::memcpy(q.labels, h_lab + voff[g], 4ull * q.n);
+      }
+    };
+    // several threads for large batches, contiguous runs of graphs with
+    // about the same vertex count each
+    const unsigned nth = n < (2u << 20) ? 1u : 4u;
+    if (nth == 1 || k == 1) {
+      scatter(0, k);
+    } else {
+      std::vector<std::thread> ts;
+      uint32_t g = 0;
+      for (unsigned t = 0; t < nth; t++) {
+        const uint64_t target = n * (t + 1) / nth;
+        uint32_t h = (t + 1 == nth) ? k : g;
+        while (h < k && voff[h] < target) h++;
+        if (h > g) {
+          ts.emplace_back(scatter, g, h);
+          g = h;
+        }
+      }
+      for (auto &t : ts) t.join();
     }
   } else {
     (void)hipStreamSynchronize(S.stream);
