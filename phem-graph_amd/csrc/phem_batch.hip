@@ -38,39 +38,55 @@ __device__ __forceinline__ uint32_t seg_of(const ull *__restrict__ off,
   return lo;
 }
 
+// Graph-of-tile maps: map[t] = the graph holding element t * MT of the
+// concatenated range (one binary search per MT elements).  An element x of
+// tile t then belongs to graph g = map[t] advanced while off[g + 1] <= x.
+constexpr uint32_t MT = 256;
+
+__global__ __launch_bounds__(BBLK) void k_tile_map(const ull *__restrict__ off,
+                                                   uint32_t k, uint64_t total,
+                                                   uint32_t *__restrict__ map) {
+  const uint64_t tiles = ceil_div_u64(total, MT);
+  for (uint64_t t = (uint64_t)blockIdx.x * BBLK + threadIdx.x; t < tiles;
+       t += (uint64_t)gridDim.x * BBLK)
+    map[t] = seg_of(off, k, t * MT);
+}
+
+__device__ __forceinline__ uint32_t graph_of(const uint32_t *__restrict__ map,
+                                             const ull *__restrict__ off,
+                                             uint32_t k, uint64_t x) {
+  uint32_t g = map[x / MT];
+  while (g + 1 < k && off[g + 1] <= x) g++;
+  return g;
+}
+
 // Global endpoints (a local id outside its graph maps to an invalid vertex)
-// and, from pointer tables, the gathered weights.  A thread handles 4
-// consecutive edges; a group that lies inside one graph (the common case)
-// does one graph lookup.
+// and, from pointer tables, the gathered weights.  Consecutive threads take
+// consecutive edges, so the reads of one graph's arrays are coalesced.
 template <bool TABLE>
 __global__ __launch_bounds__(BBLK) void k_batch_rebase(
     const uint32_t *__restrict__ u, const uint32_t *__restrict__ v,
-    const uint32_t *__restrict__ w, const uint64_t *__restrict__ pu,
-    const uint64_t *__restrict__ pv, const uint64_t *__restrict__ pw,
-    uint64_t m, const ull *__restrict__ voff, const ull *__restrict__ eoff,
-    uint32_t k, uint32_t *__restrict__ gu, uint32_t *__restrict__ gv,
+    const uint64_t *__restrict__ pu, const uint64_t *__restrict__ pv,
+    const uint64_t *__restrict__ pw, uint64_t m, const ull *__restrict__ voff,
+    const ull *__restrict__ eoff, const uint32_t *__restrict__ emap, uint32_t k,
+    uint32_t *__restrict__ gu, uint32_t *__restrict__ gv,
     uint32_t *__restrict__ gw) {
-  const uint64_t groups = ceil_div_u64(m, 4);
-  for (uint64_t t = (uint64_t)blockIdx.x * BBLK + threadIdx.x; t < groups;
-       t += (uint64_t)gridDim.x * BBLK) {
-    const uint64_t i0 = t * 4, i1 = i0 + 4 < m ? i0 + 4 : m;
-    uint32_t g = seg_of(eoff, k, i0);
-    for (uint64_t i = i0; i < i1; i++) {
-      while (g + 1 < k && eoff[g + 1] <= i) g++;
-      const ull base = voff[g], ng = voff[g + 1] - base;
-      uint32_t a, b;
-      if (TABLE) {
-        const uint64_t j = i - eoff[g];
-        a = ((const uint32_t *)pu[g])[j];
-        b = ((const uint32_t *)pv[g])[j];
-        gw[i] = ((const uint32_t *)pw[g])[j];
-      } else {
-        a = u[i];
-        b = v[i];
-      }
-      gu[i] = a < ng ? (uint32_t)(base + a) : U32_NONE;
-      gv[i] = b < ng ? (uint32_t)(base + b) : U32_NONE;
+  for (uint64_t i = (uint64_t)blockIdx.x * BBLK + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * BBLK) {
+    const uint32_t g = graph_of(emap, eoff, k, i);
+    const ull base = voff[g], ng = voff[g + 1] - base;
+    uint32_t a, b;
+    if (TABLE) {
+      const uint64_t j = i - eoff[g];
+      a = ((const uint32_t *)pu[g])[j];
+      b = ((const uint32_t *)pv[g])[j];
+      gw[i] = ((const uint32_t *)pw[g])[j];
+    } else {
+      a = u[i];
+      b = v[i];
     }
+    gu[i] = a < ng ? (uint32_t)(base + a) : U32_NONE;
+    gv[i] = b < ng ? (uint32_t)(base + b) : U32_NONE;
   }
 }
 
@@ -98,53 +114,79 @@ __global__ __launch_bounds__(BBLK) void k_batch_counts(
 // forest edge i (global id) -> local id at out[voff[g] + (i - first[g])]
 __global__ __launch_bounds__(BBLK) void k_batch_split_ids(
     const uint32_t *__restrict__ ids, uint64_t count,
-    const ull *__restrict__ voff, const ull *__restrict__ eoff, uint32_t k,
+    const ull *__restrict__ voff, const ull *__restrict__ eoff,
+    const uint32_t *__restrict__ emap, uint32_t k,
     const uint32_t *__restrict__ first, uint32_t *__restrict__ out) {
   for (uint64_t i = (uint64_t)blockIdx.x * BBLK + threadIdx.x; i < count;
        i += (uint64_t)gridDim.x * BBLK) {
     const uint32_t e = ids[i];
-    const uint32_t g = seg_of(eoff, k, e);
+    const uint32_t g = graph_of(emap, eoff, k, e);
     out[voff[g] + (i - first[g])] = (uint32_t)(e - eoff[g]);
   }
 }
 
 __global__ __launch_bounds__(BBLK) void k_batch_split_labels(
     const uint32_t *__restrict__ glab, uint64_t n_total,
-    const ull *__restrict__ voff, uint32_t k, uint32_t *__restrict__ out) {
+    const ull *__restrict__ voff, const uint32_t *__restrict__ vmap, uint32_t k,
+    uint32_t *__restrict__ out) {
   for (uint64_t x = (uint64_t)blockIdx.x * BBLK + threadIdx.x; x < n_total;
        x += (uint64_t)gridDim.x * BBLK) {
-    const uint32_t g = seg_of(voff, k, x);
+    const uint32_t g = graph_of(vmap, voff, k, x);
     out[x] = (uint32_t)(glab[x] - voff[g]);
   }
 }
 
-// per-graph exact forest weight: each thread sums a run of WCH consecutive
-// forest edges (ids ascending, so a run rarely crosses a graph) and flushes
-// one atomic per graph it touched
-constexpr int WCH = 256;
+// Per-graph exact forest weight.  Each thread sums WCH consecutive forest
+// edges (ids ascending), flushing at a graph change; then the wave combines
+// the lanes' last partial sums by graph (the lanes' graphs are non-decreasing
+// across the wave) so that one atomic per graph run of a wave is left.
+constexpr int WCH = 8;
 __global__ __launch_bounds__(BBLK) void k_batch_weights(
     const uint32_t *__restrict__ ids, uint64_t count,
-    const ull *__restrict__ eoff, uint32_t k, const uint32_t *__restrict__ w,
-    int wmode, ull *__restrict__ weights, ull *__restrict__ inexact) {
+    const ull *__restrict__ eoff, const uint32_t *__restrict__ emap, uint32_t k,
+    const uint32_t *__restrict__ w, int wmode, ull *__restrict__ weights,
+    ull *__restrict__ inexact) {
   const uint64_t nthreads = ceil_div_u64(count, WCH);
-  for (uint64_t t = (uint64_t)blockIdx.x * BBLK + threadIdx.x; t < nthreads;
-       t += (uint64_t)gridDim.x * BBLK) {
-    const uint64_t a = t * WCH, b = a + WCH < count ? a + WCH : count;
-    uint32_t g = seg_of(eoff, k, ids[a]);
+  const uint64_t stride = (uint64_t)gridDim.x * BBLK;
+  // every lane of a wave runs the same number of iterations (the shuffles)
+  const uint64_t t0 = (uint64_t)blockIdx.x * BBLK + threadIdx.x;
+  const uint64_t iters = ceil_div_u64(nthreads, stride);
+  for (uint64_t it = 0; it < iters; it++) {
+    const uint64_t t = t0 + it * stride;
+    uint32_t g = 0xFFFFFFFFu;
     ull sum = 0, bad = 0;
-    for (uint64_t i = a; i < b; i++) {
-      const uint32_t e = ids[i];
-      while (g + 1 < k && eoff[g + 1] <= e) {
-        if (sum) atomicAdd(&weights[g], sum);
-        if (bad) atomicAdd(&inexact[g], bad);
-        sum = bad = 0;
-        g++;
+    if (t < nthreads) {
+      const uint64_t a = t * WCH, b = a + WCH < count ? a + WCH : count;
+      g = graph_of(emap, eoff, k, ids[a]);
+      for (uint64_t i = a; i < b; i++) {
+        const uint32_t e = ids[i];
+        while (g + 1 < k && eoff[g + 1] <= e) {
+          if (sum) atomicAdd(&weights[g], sum);
+          if (bad) atomicAdd(&inexact[g], bad);
+          sum = bad = 0;
+          g++;
+        }
+        ull fx;
+        if (weight_fx(w[e], wmode, &fx)) sum += fx; else bad++;
       }
-      ull fx;
-      if (weight_fx(w[e], wmode, &fx)) sum += fx; else bad++;
     }
-    if (sum) atomicAdd(&weights[g], sum);
-    if (bad) atomicAdd(&inexact[g], bad);
+    // segmented inclusive scan over runs of equal g (lane order = id order)
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+      const uint32_t og = __shfl_up(g, d);
+      const ull os = __shfl_up(sum, d), ob = __shfl_up(bad, d);
+      if (lane >= (uint32_t)d && og == g) {
+        sum += os;
+        bad += ob;
+      }
+    }
+    const uint32_t ng = __shfl_down(g, 1);
+    const bool last = lane == WAVE - 1 || ng != g;
+    if (last && g != 0xFFFFFFFFu) {
+      if (sum) atomicAdd(&weights[g], sum);
+      if (bad) atomicAdd(&inexact[g], bad);
+    }
   }
 }
 
@@ -156,7 +198,7 @@ __global__ __launch_bounds__(BBLK) void k_batch_zero(ull *a, uint64_t cnt) {
 
 struct BatchLayout {
   uint64_t mst_ws, mst_bytes, gu, gv, gw, ids, glab, voff, eoff, ptrs, first,
-      total;
+      emap, vmap, total;
 };
 
 static BatchLayout batch_layout(uint64_t n_total, uint64_t m_total, uint32_t k,
@@ -179,6 +221,8 @@ static BatchLayout batch_layout(uint64_t n_total, uint64_t m_total, uint32_t k,
   L.eoff = take(8ull * (k + 1));
   L.ptrs = take(24ull * k);
   L.first = take(4ull * k);
+  L.emap = take(4 * (ceil_div(m_total, MT) + 1));
+  L.vmap = take(4 * (ceil_div(n_total, MT) + 1));
   L.total = o;
   return L;
 }
@@ -238,6 +282,7 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
   ull *voff = (ull *)(ws + L.voff), *eoff = (ull *)(ws + L.eoff);
   uint64_t *ptrs = (uint64_t *)(ws + L.ptrs);
   uint32_t *first = (uint32_t *)(ws + L.first);
+  uint32_t *emap = (uint32_t *)(ws + L.emap), *vmap = (uint32_t *)(ws + L.vmap);
   const uint32_t *gw = table ? (const uint32_t *)(ws + L.gw) : A.d_w;
   // offsets (and pointer tables) go up in one copy from a pinned buffer
   // owned by the calling thread, so the copy is asynchronous and the next
@@ -270,19 +315,27 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
     }
     PHEM_HIP(hipMemcpyAsync(ptrs, t, 24ull * k, hipMemcpyHostToDevice, s));
   }
+  {
+    PHEM_PROF("k_tile_map", s);
+    if (m_total)
+      hipLaunchKernelGGL(k_tile_map, dim3(grid_for(ceil_div(m_total, MT), BBLK)),
+                         dim3(BBLK), 0, s, eoff, k, m_total, emap);
+    if (n_total)
+      hipLaunchKernelGGL(k_tile_map, dim3(grid_for(ceil_div(n_total, MT), BBLK)),
+                         dim3(BBLK), 0, s, voff, k, n_total, vmap);
+  }
+  PHEM_LAUNCHED("k_tile_map");
   if (m_total) {
     PHEM_PROF("k_batch_rebase", s);
     if (table)
-      hipLaunchKernelGGL(k_batch_rebase<true>,
-                         dim3(grid_for(ceil_div(m_total, 4), BBLK)), dim3(BBLK),
-                         0, s, nullptr, nullptr, nullptr, ptrs, ptrs + k,
-                         ptrs + 2ull * k, m_total, voff, eoff, k, gu, gv,
+      hipLaunchKernelGGL(k_batch_rebase<true>, dim3(grid_for(m_total, BBLK)),
+                         dim3(BBLK), 0, s, nullptr, nullptr, ptrs, ptrs + k,
+                         ptrs + 2ull * k, m_total, voff, eoff, emap, k, gu, gv,
                          (uint32_t *)(ws + L.gw));
     else
-      hipLaunchKernelGGL(k_batch_rebase<false>,
-                         dim3(grid_for(ceil_div(m_total, 4), BBLK)), dim3(BBLK),
-                         0, s, A.d_u, A.d_v, A.d_w, nullptr, nullptr, nullptr,
-                         m_total, voff, eoff, k, gu, gv, nullptr);
+      hipLaunchKernelGGL(k_batch_rebase<false>, dim3(grid_for(m_total, BBLK)),
+                         dim3(BBLK), 0, s, A.d_u, A.d_v, nullptr, nullptr,
+                         nullptr, m_total, voff, eoff, emap, k, gu, gv, nullptr);
   }
   PHEM_LAUNCHED("k_batch_rebase");
   phem_mst_stats st;
@@ -299,13 +352,14 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
     PHEM_PROF("k_batch_split_ids", s);
     hipLaunchKernelGGL(k_batch_split_ids, dim3(grid_for(st.mst_count, BBLK)),
                        dim3(BBLK), 0, s, ids, (uint64_t)st.mst_count, voff,
-                       eoff, k, first, A.d_ids);
+                       eoff, emap, k, first, A.d_ids);
   }
   PHEM_LAUNCHED("k_batch_split_ids");
   if (A.d_labels && n_total) {
     PHEM_PROF("k_batch_split_labels", s);
     hipLaunchKernelGGL(k_batch_split_labels, dim3(grid_for(n_total, BBLK)),
-                       dim3(BBLK), 0, s, glab, n_total, voff, k, A.d_labels);
+                       dim3(BBLK), 0, s, glab, n_total, voff, vmap, k,
+                       A.d_labels);
   }
   PHEM_LAUNCHED("k_batch_split_labels");
   if (A.d_weights) {
@@ -321,8 +375,9 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
       PHEM_PROF("k_batch_weights", s);
       hipLaunchKernelGGL(k_batch_weights,
                          dim3(grid_for(ceil_div(st.mst_count, WCH), BBLK)),
-                         dim3(BBLK), 0, s, ids, (uint64_t)st.mst_count, eoff, k,
-                         gw, A.wmode, (ull *)A.d_weights, (ull *)A.d_inexact);
+                         dim3(BBLK), 0, s, ids, (uint64_t)st.mst_count, eoff,
+                         emap, k, gw, A.wmode, (ull *)A.d_weights,
+                         (ull *)A.d_inexact);
     }
     PHEM_LAUNCHED("k_batch_weights");
   }

@@ -553,26 +553,28 @@ __global__ __launch_bounds__(BLK) void k_giant_emit(
   }
 }
 
-// tau = key of rank ~light_edges in a strided sample of the input keys
-// (KEY_NONE: every edge is light).
+// tau = key of rank ~light_edges in a strided sample of TSAMP input keys
+// (KEY_NONE: every edge is light).  The light phase only needs about
+// light_edges edges; the results do not depend on tau.
+constexpr int TSAMP = 1024;
 template <class Src>
 __global__ __launch_bounds__(SBLK) void k_threshold(Src src, uint64_t m,
                                                     uint32_t n, int ord,
                                                     uint64_t light_edges,
                                                     ull *tau_out) {
-  __shared__ ull s[NSAMP];
-  for (int j = threadIdx.x; j < NSAMP; j += SBLK) {
-    const uint64_t i = (uint64_t)j * m / NSAMP;
+  __shared__ ull s[TSAMP];
+  for (int j = threadIdx.x; j < TSAMP; j += SBLK) {
+    const uint64_t i = (uint64_t)j * m / TSAMP;
     const uint32_t a = src.U(i), b = src.V(i);
     s[j] = (a < n && b < n && a != b)
                ? make_key(src.W(i), i, ord)
                : KEY_NONE;
   }
   __syncthreads();
-  bitonic_sort_lds(s);
+  bitonic_sort_lds<TSAMP>(s);
   if (threadIdx.x == 0) {
-    ull r = (ull)((double)light_edges * NSAMP / (double)m);
-    *tau_out = (light_edges >= m || r >= NSAMP) ? KEY_NONE : s[r];
+    ull r = (ull)((double)light_edges * TSAMP / (double)m);
+    *tau_out = (light_edges >= m || r >= TSAMP) ? KEY_NONE : s[r];
   }
 }
 
