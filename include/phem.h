@@ -346,6 +346,10 @@ int phem_server_drain(phem_server *srv);
 int phem_server_get_stats(phem_server *srv, phem_server_stats *st);
 int phem_server_reset_stats(phem_server *srv);
 
+/* ---------------------------------------------------------------------------
+ * Host backend: the same results computed on the CPU.
+ * ------------------------------------------------------------------------- */
+
 /* Minimum spanning forest on the host: Kruskal over (weight, id) with a
  * multithreaded stable radix sort.  The CPU pre-filter of the heterogeneous
  * chunked solve (MST/src/kernels.cu:1503-1788, where the reference calls its
@@ -354,6 +358,13 @@ int phem_server_reset_stats(phem_server *srv);
 int phem_cpu_msf(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
                  const uint32_t *w, int wmode, uint32_t *ids_out,
                  uint64_t *count, int threads);
+
+/* Connected components on the host: lock-free union-find over host threads
+ * (the same hook rule as the device path), labels = minimum vertex id.  Edges
+ * with u == v or an endpoint >= n are ignored.  threads <= 0: all host
+ * threads. */
+int phem_cpu_cc(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
+                uint32_t *labels, int threads);
 
 /* ---------------------------------------------------------------------------
  * Graph files (host memory).  The readers fill a phem_graph_file whose arrays

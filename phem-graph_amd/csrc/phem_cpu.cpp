@@ -1,6 +1,9 @@
-// phem_cpu.cpp -- the host side's own minimum spanning forest, used to
-// pre-filter a chunk on the CPU while the GPU streams the other chunks (the
-// reference's heterogeneous streamline, MST/src/kernels.cu:1503-1788: the
+// phem_cpu.cpp -- the library's host (CPU) backend: minimum spanning forest
+// and connected components over the same argument conventions as the device
+// entry points.
+//
+// The minimum spanning forest also pre-filters a chunk on the CPU while the
+// GPU streams the other chunks (the reference's heterogeneous streamline, MST/src/kernels.cu:1503-1788: the
 // last chunk is replaced by its own forest before the GPU folds it in, which
 // keeps the result exact because an edge outside the forest of a subgraph is
 // the heaviest edge of a cycle and stays outside every larger forest).
@@ -10,6 +13,7 @@
 // several threads over contiguous blocks, so equal keys keep position order),
 // then one union-find sweep in that order.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -118,5 +122,68 @@ extern "C" int phem_cpu_msf(uint32_t n, uint64_t m, const uint32_t *u,
   std::sort(chosen.begin(), chosen.end());
   std::memcpy(ids_out, chosen.data(), 4 * chosen.size());
   *count = chosen.size();
+  return PHEM_OK;
+}
+
+namespace phem {
+namespace {
+
+// Lock-free union-find on host threads: roots are the minimum vertex of
+// their tree (the larger root is hooked under the smaller by CAS), finds
+// halve paths with relaxed stores of ancestors.
+uint32_t cpu_find(std::atomic<uint32_t> *par, uint32_t x) {
+  uint32_t p = par[x].load(std::memory_order_relaxed);
+  while (p != x) {
+    const uint32_t gp = par[p].load(std::memory_order_relaxed);
+    if (gp == p) return p;
+    par[x].store(gp, std::memory_order_relaxed);
+    x = gp;
+    p = par[x].load(std::memory_order_relaxed);
+  }
+  return x;
+}
+
+void cpu_union(std::atomic<uint32_t> *par, uint32_t a, uint32_t b) {
+  a = cpu_find(par, a);
+  b = cpu_find(par, b);
+  while (a != b) {
+    uint32_t hi = a > b ? a : b, lo = a ^ b ^ hi;
+    uint32_t expect = hi;
+    if (par[hi].compare_exchange_strong(expect, lo, std::memory_order_acq_rel))
+      return;
+    const uint32_t r = cpu_find(par, expect);  // hi was hooked meanwhile
+    if (hi == a) a = r; else b = r;
+  }
+}
+
+}  // namespace
+}  // namespace phem
+
+extern "C" int phem_cpu_cc(uint32_t n, uint64_t m, const uint32_t *u,
+                           const uint32_t *v, uint32_t *labels, int threads) {
+  if (m && (!u || !v)) return set_err(PHEM_EINVAL, "NULL edge array");
+  if (n && !labels) return set_err(PHEM_EINVAL, "labels is NULL");
+  unsigned T = threads > 0 ? (unsigned)threads : std::thread::hardware_concurrency();
+  T = std::max(1u, std::min(T, 64u));
+  if (m < (1u << 16)) T = 1;
+  std::vector<std::atomic<uint32_t>> par(n);
+  for (uint32_t x = 0; x < n; x++) par[x].store(x, std::memory_order_relaxed);
+  auto run = [&](unsigned t) {
+    for (uint64_t i = m * t / T; i < m * (t + 1) / T; i++) {
+      const uint32_t a = u[i], b = v[i];
+      if (a != b && a < n && b < n) cpu_union(par.data(), a, b);
+    }
+  };
+  std::vector<std::thread> ts;
+  for (unsigned t = 1; t < T; t++) ts.emplace_back(run, t);
+  run(0);
+  for (auto &t : ts) t.join();
+  // labels: the root of each vertex (the minimum id of its component)
+  for (uint32_t x = 0; x < n; x++) {
+    uint32_t r = x;
+    while (par[r].load(std::memory_order_relaxed) != r)
+      r = par[r].load(std::memory_order_relaxed);
+    labels[x] = r;
+  }
   return PHEM_OK;
 }

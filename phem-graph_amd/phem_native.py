@@ -171,6 +171,7 @@ def lib():
         "phem_get_tuning": (i32, [c.POINTER(Tuning)]),
         "phem_set_tuning": (i32, [c.POINTER(Tuning)]),
         "phem_cpu_msf": (i32, [u32, u64, vp, vp, vp, i32, vp, c.POINTER(c.c_uint64), i32]),
+        "phem_cpu_cc": (i32, [u32, u64, vp, vp, vp, i32]),
         "phem_read_mtx": (i32, [c.c_char_p, i32, c.POINTER(GraphFile)]),
         "phem_read_edge_list": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
         "phem_read_ecl": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
@@ -621,3 +622,13 @@ def set_tuning(blocks_per_cu=0, mst_light_factor=None, cc_sample_factor=None):
     t = Tuning(blocks_per_cu or 0, 0, nan if mst_light_factor is None else mst_light_factor,
                nan if cc_sample_factor is None else cc_sample_factor)
     _check(lib().phem_set_tuning(ctypes.byref(t)), "phem_set_tuning")
+
+
+def cpu_cc(n, u, v, threads=0):
+    """Host connected components (phem_cpu_cc): min-id labels."""
+    import numpy as np
+    u, v = (np.ascontiguousarray(x, dtype=np.uint32) for x in (u, v))
+    lab = np.zeros(max(n, 1), np.uint32)
+    _check(lib().phem_cpu_cc(n, len(u), u.ctypes.data, v.ctypes.data, lab.ctypes.data,
+                             threads), "phem_cpu_cc")
+    return lab[:n]

@@ -1,5 +1,6 @@
-"""The host minimum spanning forest (phem_cpu_msf, the CPU pre-filter of the
-heterogeneous chunked solve) equals the oracle's Kruskal forest."""
+"""The library's host backend: its minimum spanning forest (phem_cpu_msf, also
+the CPU pre-filter of the heterogeneous chunked solve) equals the oracle's
+Kruskal forest, and its connected components (phem_cpu_cc) equal BFS labels."""
 import numpy as np
 import pytest
 
@@ -22,3 +23,17 @@ def test_cpu_msf_tie_heavy_integer_weights():
     w = (w.view(np.float32)).astype(np.uint32)  # integers 1..16
     n = 300 * 300
     assert np.array_equal(P.cpu_msf(n, u, v, w, P.W_U32, 8), O.kruskal(n, u, v, w, 0)[0])
+
+
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_cpu_cc_cases(name):
+    n, u, v, w, wmode = cases.make(name)
+    want = O.bfs_labels(n, u, v)
+    for threads in (1, 8):
+        assert np.array_equal(P.cpu_cc(n, u, v, threads), want)
+
+
+def test_cpu_cc_rmat_threads():
+    u, v, _ = S.rmat(16, 1 << 20, 5)
+    n = 1 << 16
+    assert np.array_equal(P.cpu_cc(n, u, v, 8), O.uf_labels(n, u, v))
