@@ -163,6 +163,46 @@ __device__ __forceinline__ void min_store(ull *best, uint32_t c, uint64_t key) {
   if (key < best[c]) atomicMin(&best[c], (ull)key);
 }
 
+// min_store of key[j] to best[c[j]] for every j with on[j].  All pre-check
+// reads are issued before the first compare, so a thread waits for one
+// memory latency instead of N in a row; they are unconditional (entry 0
+// stands in where on[j] is false) so that no branch splits them.  The
+// pre-check reads only the high word of best[c]: best only decreases, so a
+// stale high word is an upper bound of the current one and key >> 32 <= it
+// whenever key < best[c].  best must have at least one entry.
+template <int N>
+__device__ __forceinline__ void min_store_n(ull *best, const uint32_t (&c)[N],
+                                            const ull (&key)[N],
+                                            const bool (&on)[N]) {
+  uint32_t hi[N];
+#pragma unroll
+  for (int j = 0; j < N; j++) hi[j] = (uint32_t)(best[on[j] ? c[j] : 0u] >> 32);
+#pragma unroll
+  for (int j = 0; j < N; j++)
+    if (on[j] && (uint32_t)(key[j] >> 32) <= hi[j]) atomicMin(&best[c[j]], key[j]);
+}
+
+// min_store_n to both endpoints a[j] and b[j] of every edge j with on[j].
+template <int N>
+__device__ __forceinline__ void min_store_ends(ull *best, const uint32_t (&a)[N],
+                                               const uint32_t (&b)[N],
+                                               const ull (&key)[N],
+                                               const bool (&on)[N]) {
+  uint32_t ha[N], hb[N];
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    ha[j] = (uint32_t)(best[on[j] ? a[j] : 0u] >> 32);
+    hb[j] = (uint32_t)(best[on[j] ? b[j] : 0u] >> 32);
+  }
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    if (!on[j]) continue;
+    const uint32_t kh = (uint32_t)(key[j] >> 32);
+    if (kh <= ha[j]) atomicMin(&best[a[j]], key[j]);
+    if (kh <= hb[j]) atomicMin(&best[b[j]], key[j]);
+  }
+}
+
 // ---- union-find over a parent array where every pointer goes to a smaller
 // id (roots are the minimum vertex of their tree) ---------------------------
 //
@@ -194,6 +234,19 @@ __device__ __forceinline__ uint32_t uf_find(uint32_t *par, uint32_t x) {
   return x;
 }
 
+// find with path halving from x whose parent p = par[x] is already loaded
+__device__ __forceinline__ uint32_t uf_find_from(uint32_t *par, uint32_t x,
+                                                 uint32_t p) {
+  while (p != x) {
+    uint32_t gp = par_load(&par[p]);
+    if (gp == p) return p;
+    par_store(&par[x], gp);
+    x = gp;
+    p = par_load(&par[x]);
+  }
+  return x;
+}
+
 // find without writes (for read-only walks)
 __device__ __forceinline__ uint32_t uf_root(const uint32_t *par, uint32_t x) {
   uint32_t p = par[x];
@@ -204,13 +257,12 @@ __device__ __forceinline__ uint32_t uf_root(const uint32_t *par, uint32_t x) {
   return x;
 }
 
-// Union of the trees of a and b: the larger root is hooked under the smaller
-// by compare-and-swap.  Returns the hooked root, or U32_NONE if a and b were
-// already in one tree.
-__device__ __forceinline__ uint32_t uf_union(uint32_t *par, uint32_t a,
-                                             uint32_t b) {
-  a = uf_find(par, a);
-  b = uf_find(par, b);
+// Union of the trees whose roots were found as a and b: the larger root is
+// hooked under the smaller by compare-and-swap (a failed swap means the root
+// was hooked meanwhile; the walk goes on from its new root).  Returns the
+// hooked root, or U32_NONE if the trees were already one.
+__device__ __forceinline__ uint32_t uf_hook_roots(uint32_t *par, uint32_t a,
+                                                  uint32_t b) {
   while (a != b) {
     uint32_t hi = a > b ? a : b;
     uint32_t lo = a ^ b ^ hi;
@@ -221,6 +273,20 @@ __device__ __forceinline__ uint32_t uf_union(uint32_t *par, uint32_t a,
     if (hi == a) a = r; else b = r;
   }
   return U32_NONE;
+}
+
+// Union of the trees of a and b (uf_hook_roots of their roots).
+__device__ __forceinline__ uint32_t uf_union(uint32_t *par, uint32_t a,
+                                             uint32_t b) {
+  return uf_hook_roots(par, uf_find(par, a), uf_find(par, b));
+}
+
+// uf_union of a and b whose parents pa = par[a], pb = par[b] are already
+// loaded (callers load the parents of several edges together)
+__device__ __forceinline__ uint32_t uf_union_from(uint32_t *par, uint32_t a,
+                                                  uint32_t pa, uint32_t b,
+                                                  uint32_t pb) {
+  return uf_hook_roots(par, uf_find_from(par, a, pa), uf_find_from(par, b, pb));
 }
 
 // Exact weight of one edge as an integer: the unsigned weight itself

@@ -104,6 +104,16 @@ struct SrcSoA {
   __device__ __forceinline__ uint32_t U(uint64_t i) const { return u[i]; }
   __device__ __forceinline__ uint32_t V(uint64_t i) const { return v[i]; }
   __device__ __forceinline__ uint32_t W(uint64_t i) const { return w[i]; }
+  // the four edges i4 .. i4+3, all in range, arrays 16-byte aligned
+  __device__ __forceinline__ void load4_full(uint64_t i4, uint32_t *a,
+                                             uint32_t *b, uint32_t *c) const {
+    u32x4 x = __builtin_nontemporal_load((const u32x4 *)(u + i4));
+    u32x4 y = __builtin_nontemporal_load((const u32x4 *)(v + i4));
+    u32x4 z = __builtin_nontemporal_load((const u32x4 *)(w + i4));
+    a[0] = x.x; a[1] = x.y; a[2] = x.z; a[3] = x.w;
+    b[0] = y.x; b[1] = y.y; b[2] = y.z; b[3] = y.w;
+    c[0] = z.x; c[1] = z.y; c[2] = z.z; c[3] = z.w;
+  }
   __device__ __forceinline__ void load4(uint64_t i4, uint64_t end, uint32_t *a,
                                         uint32_t *b, uint32_t *c) const {
     if (vec && i4 + 4 <= end) {
@@ -135,6 +145,17 @@ struct SrcAoS {
   __device__ __forceinline__ uint32_t U(uint64_t i) const { return e[3 * i]; }
   __device__ __forceinline__ uint32_t V(uint64_t i) const { return e[3 * i + 1]; }
   __device__ __forceinline__ uint32_t W(uint64_t i) const { return e[3 * i + 2]; }
+  // the four edges i4 .. i4+3, all in range, array 16-byte aligned
+  __device__ __forceinline__ void load4_full(uint64_t i4, uint32_t *a,
+                                             uint32_t *b, uint32_t *c) const {
+    const u32x4 *p = (const u32x4 *)(e + 3 * i4);
+    u32x4 x = __builtin_nontemporal_load(p);
+    u32x4 y = __builtin_nontemporal_load(p + 1);
+    u32x4 z = __builtin_nontemporal_load(p + 2);
+    a[0] = x.x; b[0] = x.y; c[0] = x.z; a[1] = x.w;
+    b[1] = y.x; c[1] = y.y; a[2] = y.z; b[2] = y.w;
+    c[2] = z.x; a[3] = z.y; b[3] = z.z; c[3] = z.w;
+  }
   __device__ __forceinline__ void load4(uint64_t i4, uint64_t end, uint32_t *a,
                                         uint32_t *b, uint32_t *c) const {
     if (vec && i4 + 4 <= end) {
@@ -229,33 +250,58 @@ __global__ __launch_bounds__(BLK) void k_hook(
       start = (uint64_t)v * LTILE;
       len = (uint32_t)min_u64(LTILE, (uint64_t)n_ident - start);
     }
+    // Each step's loads (list, best, the chosen edges' endpoints, the bitmap
+    // words, the endpoints' parents) are issued for all IPT components
+    // before any is used.
     bool active[IPT];
     uint32_t c[IPT];
+    ull key[IPT];
     uint64_t slot[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
       const uint32_t idx = (uint32_t)j * BLK + threadIdx.x;
-      active[j] = false;
-      c[j] = 0;
-      if (idx < len) {
-        c[j] = list_cnt ? list[start + idx] : (uint32_t)(start + idx);
-        ull key = best[c[j]];
-        if (key != KEY_NONE) {
-          active[j] = true;
-          best[c[j]] = KEY_NONE;  // reset for the next round's sweep
-          uint32_t e = key_id(key, ord);
-          const uint32_t bit = 1u << (e & 31);
-          // a component whose edge was already marked (both sides chose the
-          // same edge) adds no weight: only the atomicOr that sets the bit does
-          if (!(atomicOr(&bitmap[e >> 5], bit) & bit)) {
-            ull fx;
-            if (weight_fx(key_weight(key, ord), ord & ORD_F32, &fx)) wsum += fx;
-            else wbad++;
-          }
-          uf_union(par, src.U(e), src.V(e));
-        }
+      c[j] = idx < len ? (list_cnt ? list[start + idx] : (uint32_t)(start + idx))
+                       : U32_NONE;
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      key[j] = best[c[j] != U32_NONE ? c[j] : 0u];  // unconditional read
+      if (c[j] == U32_NONE) key[j] = KEY_NONE;
+    }
+    uint32_t e[IPT], eu[IPT], ev[IPT], old[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      active[j] = key[j] != KEY_NONE;
+      e[j] = active[j] ? key_id(key[j], ord) : 0u;
+      if (active[j]) best[c[j]] = KEY_NONE;  // reset for the next round's sweep
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      eu[j] = src.U(e[j]);  // e = 0 for inactive entries
+      ev[j] = src.V(e[j]);
+    }
+    // a component whose edge was already marked (both sides chose the same
+    // edge) adds no weight: only the atomicOr that sets the bit does
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      old[j] = active[j] ? atomicOr(&bitmap[e[j] >> 5], 1u << (e[j] & 31)) : ~0u;
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      if (!((old[j] >> (e[j] & 31)) & 1u)) {
+        ull fx;
+        if (weight_fx(key_weight(key[j], ord), ord & ORD_F32, &fx)) wsum += fx;
+        else wbad++;
       }
     }
+    uint32_t pu[IPT], pv[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      pu[j] = par_load(&par[active[j] ? eu[j] : 0u]);
+      pv[j] = par_load(&par[active[j] ? ev[j] : 0u]);
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      if (active[j]) uf_union_from(par, eu[j], pu[j], ev[j], pv[j]);
     const uint32_t seg = v % KSEG;
     block_append<BLK, IPT>(active, slot, act_cnt + seg * CSTR,
                            (uint64_t)seg * act_cap);
@@ -363,19 +409,32 @@ __global__ __launch_bounds__(BLK) void k_pass(
     uint32_t a[EPT], b[EPT];
     ull key[EPT];
     bool val[EPT];
+    // A full tile (block-uniform) issues all its loads unconditionally, so
+    // they are in flight together; a partial tile takes the guarded path.
+    const bool full = len == (uint32_t)TILE && (!INPUT || src.vec);
+    if (INPUT) {
+      uint32_t wv[EPT];
+      if (full) {
 #pragma unroll
-    for (int g = 0; g < EPT / 4; g++) {
-      const uint64_t i4 = start + ((uint64_t)g * BLK + threadIdx.x) * 4;
-      if (INPUT) {
-        uint32_t wv[4];
-        src.load4(i4, end, a + 4 * g, b + 4 * g, wv);
+        for (int g = 0; g < EPT / 4; g++)
+          src.load4_full(start + ((uint64_t)g * BLK + threadIdx.x) * 4, a + 4 * g,
+                         b + 4 * g, wv + 4 * g);
+      } else {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          key[4 * g + q] = make_key(wv[q], i4 + q, ord);
-          val[4 * g + q] = a[4 * g + q] < n && b[4 * g + q] < n &&
-                           a[4 * g + q] != b[4 * g + q];
-        }
-      } else if (i4 + 4 <= end) {
+        for (int g = 0; g < EPT / 4; g++)
+          src.load4(start + ((uint64_t)g * BLK + threadIdx.x) * 4, end, a + 4 * g,
+                    b + 4 * g, wv + 4 * g);
+      }
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        const uint64_t i = start + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
+        key[j] = make_key(wv[j], i, ord);
+        val[j] = (a[j] < n) & (b[j] < n) & (a[j] != b[j]);
+      }
+    } else if (full) {
+#pragma unroll
+      for (int g = 0; g < EPT / 4; g++) {
+        const uint64_t i4 = start + ((uint64_t)g * BLK + threadIdx.x) * 4;
         u32x4 x = __builtin_nontemporal_load((const u32x4 *)(ia + i4));
         u32x4 y = __builtin_nontemporal_load((const u32x4 *)(ib + i4));
         u64x2 k0 = __builtin_nontemporal_load((const u64x2 *)(ik + i4));
@@ -384,17 +443,18 @@ __global__ __launch_bounds__(BLK) void k_pass(
         b[4 * g] = y.x; b[4 * g + 1] = y.y; b[4 * g + 2] = y.z; b[4 * g + 3] = y.w;
         key[4 * g] = k0.x; key[4 * g + 1] = k0.y;
         key[4 * g + 2] = k1.x; key[4 * g + 3] = k1.y;
+      }
 #pragma unroll
-        for (int q = 0; q < 4; q++) val[4 * g + q] = true;
-      } else {
+      for (int j = 0; j < EPT; j++) val[j] = true;
+    } else {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const bool in = i4 + q < end;
-          a[4 * g + q] = in ? ia[i4 + q] : 0u;
-          b[4 * g + q] = in ? ib[i4 + q] : 0u;
-          key[4 * g + q] = in ? ik[i4 + q] : KEY_NONE;
-          val[4 * g + q] = in;
-        }
+      for (int j = 0; j < EPT; j++) {
+        const uint64_t i = start + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
+        const bool in = i < end;
+        a[j] = in ? ia[i] : 0u;
+        b[j] = in ? ib[i] : 0u;
+        key[j] = in ? ik[i] : KEY_NONE;
+        val[j] = in;
       }
     }
     if (INPUT) {
@@ -411,19 +471,37 @@ __global__ __launch_bounds__(BLK) void k_pass(
       // the giant only the lightest can be in the forest (cycle property),
       // so the edge is only offered to gmin[x]; k_giant_fold moves each
       // vertex's minimum to its root and k_giant_emit turns each
-      // component's minimum into one alive edge.
+      // component's minimum into one alive edge.  The bitmap words of the
+      // endpoints the LDS summary leaves open are read together.
+      bool ga[EPT], gb[EPT];
+      uint32_t wa[EPT], wb[EPT];
 #pragma unroll
       for (int j = 0; j < EPT; j++) {
         val[j] = val[j] && key[j] >= tau;
-        if (val[j]) {
-          const bool ga = giant_test(a[j], s_sum, covered, giant);
-          const bool gb = giant_test(b[j], s_sum, covered, giant);
-          if (ga != gb) min_store(gmin, ga ? b[j] : a[j], key[j]);
-          val[j] = !(ga || gb);
-        }
+        ga[j] = val[j] && giant_summary_hit(a[j], s_sum, covered);
+        gb[j] = val[j] && giant_summary_hit(b[j], s_sum, covered);
       }
+      // unconditional reads (word 0 stands in for settled endpoints)
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        wa[j] = giant[val[j] && !ga[j] ? a[j] >> 5 : 0u];
+        wb[j] = giant[val[j] && !gb[j] ? b[j] >> 5 : 0u];
+      }
+      bool one[EPT];
+      uint32_t x[EPT];
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        ga[j] = val[j] && (ga[j] || ((wa[j] >> (a[j] & 31)) & 1u));
+        gb[j] = val[j] && (gb[j] || ((wb[j] >> (b[j] & 31)) & 1u));
+        one[j] = val[j] && ga[j] != gb[j];
+        x[j] = ga[j] ? b[j] : a[j];
+        val[j] = val[j] && !(ga[j] || gb[j]);
+      }
+      min_store_n<EPT>(gmin, x, key, one);
     }
-    if (MODE != P_LIGHT) {
+    if (MODE == P_EDGES) {
+      // alive edges of late rounds name a few hot components; one edge's
+      // lookups at a time measured faster here than the batched form below
 #pragma unroll
       for (int j = 0; j < EPT; j++) {
         if (val[j]) {
@@ -431,14 +509,36 @@ __global__ __launch_bounds__(BLK) void k_pass(
           b[j] = par[b[j]];
         }
       }
+    } else if (MODE != P_LIGHT) {
+      uint32_t pa[EPT], pb[EPT];
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        pa[j] = par[val[j] ? a[j] : 0u];  // unconditional, as above
+        pb[j] = par[val[j] ? b[j] : 0u];
+      }
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        a[j] = pa[j];
+        b[j] = pb[j];
+      }
     }
     bool keep[EPT];
 #pragma unroll
-    for (int j = 0; j < EPT; j++) {
-      keep[j] = val[j] && a[j] != b[j];
-      if (keep[j]) {
-        min_store(best, a[j], key[j]);
-        min_store(best, b[j], key[j]);
+    for (int j = 0; j < EPT; j++) keep[j] = val[j] && a[j] != b[j];
+    if (MODE == P_LIGHT || MODE == P_HEAVY) {
+      // input edges spread their offers over many components: the
+      // pre-check reads of the tile are issued together
+      min_store_ends<EPT>(best, a, b, key, keep);
+    } else {
+      // relabelled edges concentrate on few components in late rounds: each
+      // pre-check follows the thread's previous atomic, so a component's
+      // lowered minimum filters the offers after it
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        if (keep[j]) {
+          min_store(best, a[j], key[j]);
+          min_store(best, b[j], key[j]);
+        }
       }
     }
     // survivors go straight from registers to their slots in the tile's

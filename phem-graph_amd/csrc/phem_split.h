@@ -154,6 +154,15 @@ __device__ __forceinline__ uint32_t gsum_load(uint32_t *s_sum,
   return nw * 1024u;
 }
 
+// The LDS summary's answer for x: true = x is in the giant component; false
+// leaves it open (the bitmap decides).
+__device__ __forceinline__ bool giant_summary_hit(uint32_t x,
+                                                  const uint32_t *s_sum,
+                                                  uint32_t covered) {
+  const uint32_t w = x >> 5;
+  return x < covered && ((s_sum[w >> 5] >> (w & 31)) & 1u);
+}
+
 __device__ __forceinline__ bool giant_test(uint32_t x, const uint32_t *s_sum,
                                            uint32_t covered,
                                            const uint32_t *__restrict__ bits) {
