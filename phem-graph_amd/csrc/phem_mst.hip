@@ -942,9 +942,16 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
 
   // Rounds over the alive edges in buffer `cur` until none is left.  The
   // first hook of a phase scans all n vertices (no list); later hooks scan
-  // the roots collected by the previous compress.
-  auto run_rounds = [&](uint64_t list_ub, bool first_all) -> int {
+  // the roots collected by the previous compress.  On entry m_r may be an
+  // upper bound of the alive edges (the producer's count not read yet): the
+  // first counter read then reports E(r) with the rounds' counters, together
+  // with the `extra` values (written to extra_out), so no read of its own is
+  // needed between the producer and the first rounds.
+  auto run_rounds = [&](uint64_t list_ub, bool first_all,
+                        std::initializer_list<std::pair<int, int>> extra,
+                        ull *extra_out) -> int {
     bool all = first_all;
+    bool first = true;
     while (m_r > 0) {
       // Up to SPEC rounds per counter read.  A round launched after the
       // phase has converged finds empty inputs and leaves its counters at
@@ -1002,9 +1009,20 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       }
       q.off[q.n] = cL(r + batch);
       q.segs[q.n++] = KSEG;
+      const int nbase = q.n;
+      if (first) {
+        for (auto &p : extra) {
+          q.off[q.n] = p.first;
+          q.segs[q.n++] = p.second;
+        }
+      }
       ull got[MAX_SUMS];
       int rc = read_sum_req(q, got);
       if (rc) return rc;
+      if (first) {
+        for (int i = nbase; i < q.n; i++) extra_out[i - nbase] = got[i];
+        first = false;
+      }
       int real = 0;  // rounds that started with at least one alive edge
       while (real < batch && got[real] > 0) {
         if (stats) {
@@ -1070,7 +1088,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     r = valid ? 1 : 0;
     m_r = hv[2];
     cur = 0;
-    rc = run_rounds(hv[3], false);
+    rc = run_rounds(hv[3], false, {}, nullptr);
     if (rc) return rc;
     if (stats) stats->light_rounds = (uint32_t)r;
   } else if (m > 0 && n > 1) {
@@ -1090,17 +1108,16 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          (ull *)nullptr, nou);
     }
     PHEM_LAUNCHED("k_light");
-    int rc = read_sums({{C_TAU, 1}, {cE(0), KSEG}}, hv);
+    // the light edge count arrives with the first rounds' counters
+    m_r = m;
+    cur = 0;
+    int rc = run_rounds(0, true, {{C_TAU, 1}, {cE(0), KSEG}}, hv);
     if (rc) return rc;
     if (stats) {
       stats->tau = hv[0];
       stats->light_edges = hv[1];
+      stats->light_rounds = (uint32_t)r;
     }
-    m_r = hv[1];
-    cur = 0;
-    rc = run_rounds(0, true);
-    if (rc) return rc;
-    if (stats) stats->light_rounds = (uint32_t)r;
     // heavy phase: flatten, mark the giant component, filter the rest
     {
       PHEM_PROF("k_flatten", s);
@@ -1150,15 +1167,16 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          Ek[cur], ctr + cE(r), L.ecap, best, ctr + cPart(3));
     }
     PHEM_LAUNCHED("k_giant_emit");
-    rc = read_sums({{cE(r), KSEG}, {cPart(3), KSEG}}, hv);
+    // the heavy survivors (at most m + n) are counted with the first heavy
+    // rounds' counters
+    m_r = m + n;
+    const int r_heavy = r;
+    rc = run_rounds(0, true, {{cE(r_heavy), KSEG}, {cPart(3), KSEG}}, hv);
     if (rc) return rc;
-    m_r = hv[0];
     if (stats) {
-      stats->heavy_edges = m_r;
+      stats->heavy_edges = hv[0];
       stats->giant_edges = hv[1];
     }
-    rc = run_rounds(0, true);
-    if (rc) return rc;
   }
 
   if (d_labels && n) {
