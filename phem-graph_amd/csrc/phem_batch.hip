@@ -26,23 +26,6 @@ __device__ __forceinline__ uint64_t ceil_div_u64(uint64_t a, uint64_t b) {
   return (a + b - 1) / b;
 }
 
-// index g with off[g] <= x < off[g + 1] (off ascending, off[0] = 0); empty
-// ranges are skipped because the last such g is taken
-__device__ __forceinline__ uint32_t seg_of(const ull *__restrict__ off,
-                                           uint32_t k, ull x) {
-  uint32_t lo = 0, hi = k - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= x) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
-// Graph-of-tile maps: map[t] = the graph holding element t * MT of the
-// concatenated range (one binary search per MT elements).  An element x of
-// tile t then belongs to graph g = map[t] advanced while off[g + 1] <= x.
-constexpr uint32_t MT = 256;
-
 __global__ __launch_bounds__(BBLK) void k_tile_map(const ull *__restrict__ off,
                                                    uint32_t k, uint64_t total,
                                                    uint32_t *__restrict__ map) {
@@ -50,14 +33,6 @@ __global__ __launch_bounds__(BBLK) void k_tile_map(const ull *__restrict__ off,
   for (uint64_t t = (uint64_t)blockIdx.x * BBLK + threadIdx.x; t < tiles;
        t += (uint64_t)gridDim.x * BBLK)
     map[t] = seg_of(off, k, t * MT);
-}
-
-__device__ __forceinline__ uint32_t graph_of(const uint32_t *__restrict__ map,
-                                             const ull *__restrict__ off,
-                                             uint32_t k, uint64_t x) {
-  uint32_t g = map[x / MT];
-  while (g + 1 < k && off[g + 1] <= x) g++;
-  return g;
 }
 
 // Global endpoints (a local id outside its graph maps to an invalid vertex)
@@ -198,7 +173,7 @@ __global__ __launch_bounds__(BBLK) void k_batch_zero(ull *a, uint64_t cnt) {
 
 struct BatchLayout {
   uint64_t mst_ws, mst_bytes, gu, gv, gw, ids, glab, voff, eoff, ptrs, first,
-      emap, vmap, total;
+      emap, vmap, groot, taus, total;
 };
 
 static BatchLayout batch_layout(uint64_t n_total, uint64_t m_total, uint32_t k,
@@ -223,6 +198,8 @@ static BatchLayout batch_layout(uint64_t n_total, uint64_t m_total, uint32_t k,
   L.first = take(4ull * k);
   L.emap = take(4 * (ceil_div(m_total, MT) + 1));
   L.vmap = take(4 * (ceil_div(n_total, MT) + 1));
+  L.groot = take(4ull * k);
+  L.taus = take(8ull * k);
   L.total = o;
   return L;
 }
@@ -339,8 +316,11 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
   }
   PHEM_LAUNCHED("k_batch_rebase");
   phem_mst_stats st;
-  rc = phem_mst_solve((uint32_t)n_total, m_total, gu, gv, gw, A.wmode,
-                      ws + L.mst_ws, L.mst_bytes, ids, glab, A.opts, &st, s);
+  const GraphGroups groups{k,    voff, eoff, vmap, emap, (ull *)(ws + L.taus),
+                           (uint32_t *)(ws + L.groot)};
+  rc = mst_solve_groups((uint32_t)n_total, m_total, gu, gv, gw, A.wmode,
+                        ws + L.mst_ws, L.mst_bytes, ids, glab, A.opts, &st, s,
+                        groups);
   if (rc) return rc;
   {
     PHEM_PROF("k_batch_counts", s);

@@ -154,6 +154,36 @@ __device__ __forceinline__ void block_append(const bool (&pred)[IPT],
   __syncthreads();
 }
 
+// ---- ranges of concatenated graphs ----------------------------------------
+//
+// Graph g of a batch owns elements [off[g], off[g + 1]) of a concatenated
+// vertex or edge range.
+
+// index g with off[g] <= x < off[g + 1] (off ascending, off[0] = 0); empty
+// ranges are skipped because the last such g is taken
+__device__ __forceinline__ uint32_t seg_of(const ull *__restrict__ off,
+                                           uint32_t k, ull x) {
+  uint32_t lo = 0, hi = k - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= x) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Graph-of-tile maps: map[t] = the graph holding element t * MT of the
+// concatenated range (one binary search per MT elements).  An element x of
+// tile t then belongs to graph g = map[t] advanced while off[g + 1] <= x.
+constexpr uint32_t MT = 256;
+
+__device__ __forceinline__ uint32_t graph_of(const uint32_t *__restrict__ map,
+                                             const ull *__restrict__ off,
+                                             uint32_t k, uint64_t x) {
+  uint32_t g = map[x / MT];
+  while (g + 1 < k && off[g + 1] <= x) g++;
+  return g;
+}
+
 // ---- minimum-key store ----------------------------------------------------
 
 // best[c] = min(best[c], key).  The plain read filters out the atomics that

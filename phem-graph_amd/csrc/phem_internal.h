@@ -32,6 +32,28 @@ inline unsigned grid_for(uint64_t work, uint64_t per_block) {
   return (unsigned)(g < cap ? g : cap);
 }
 
+// A batch's graphs seen by the solve over their disjoint union (device
+// arrays): graph g owns vertices [voff[g], voff[g + 1]) and edges
+// [eoff[g], eoff[g + 1]); vmap / emap are the graph-of-tile maps of the two
+// ranges.  The solve picks a light threshold per graph (taus, k keys) and a
+// giant component per graph (groot, k roots): edges never join two graphs,
+// so each graph is split and filtered as if it were solved alone.
+struct GraphGroups {
+  uint32_t k;
+  const unsigned long long *voff, *eoff;
+  const uint32_t *vmap, *emap;
+  unsigned long long *taus;
+  uint32_t *groot;
+};
+
+// phem_mst_solve over a batch's disjoint union (structure-of-arrays input).
+int mst_solve_groups(uint32_t n, uint64_t m, const uint32_t *d_u,
+                     const uint32_t *d_v, const uint32_t *d_w, int wmode,
+                     void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
+                     uint32_t *d_labels, const phem_mst_opts *opts,
+                     phem_mst_stats *stats, hipStream_t s,
+                     const GraphGroups &groups);
+
 // Records HIP events around the launches in its scope when profiling is on.
 class ProfScope {
  public:

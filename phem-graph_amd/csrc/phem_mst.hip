@@ -377,6 +377,14 @@ __global__ __launch_bounds__(BLK) void k_compress(
 //   P_EDGES segmented alive edges of the previous round, relabelled
 enum PassMode : int { P_ALL = 0, P_LIGHT = 1, P_HEAVY = 2, P_EDGES = 3 };
 
+// Light thresholds per graph of a batch (taus[g] for the edges of graph g),
+// or taus == NULL: one threshold for all edges.
+struct EdgeTaus {
+  const ull *taus, *eoff;
+  const uint32_t *emap;
+  uint32_t k;
+};
+
 template <int MODE, class Src>
 __global__ __launch_bounds__(BLK) void k_pass(
     Src src, const uint32_t *__restrict__ ia, const uint32_t *__restrict__ ib,
@@ -385,7 +393,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
     const ull *tau_ptr, const uint32_t *__restrict__ giant,
     uint32_t *__restrict__ oa, uint32_t *__restrict__ ob,
     ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *valid_parts,
-    ull *best, ull *gmin, const uint32_t *gsum) {
+    ull *best, ull *gmin, const uint32_t *gsum, EdgeTaus et) {
   __shared__ SegView sv;
   __shared__ uint32_t s_sum[MODE == P_HEAVY ? GSUM_HEAVY : 1];
   constexpr bool INPUT = MODE != P_EDGES;
@@ -461,9 +469,15 @@ __global__ __launch_bounds__(BLK) void k_pass(
 #pragma unroll
       for (int j = 0; j < EPT; j++) nval += val[j];
     }
+    // the threshold of edge j (its graph's, for a batch)
+    auto tau_of = [&](int j) -> ull {
+      if (!et.taus) return tau;
+      const uint64_t i = start + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
+      return et.taus[graph_of(et.emap, et.eoff, et.k, i < end ? i : start)];
+    };
     if (MODE == P_LIGHT) {
 #pragma unroll
-      for (int j = 0; j < EPT; j++) val[j] = val[j] && key[j] < tau;
+      for (int j = 0; j < EPT; j++) val[j] = val[j] && key[j] < tau_of(j);
     }
     if (MODE == P_HEAVY) {
       // both endpoints in the giant component: inside one component.  One
@@ -477,7 +491,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
       uint32_t wa[EPT], wb[EPT];
 #pragma unroll
       for (int j = 0; j < EPT; j++) {
-        val[j] = val[j] && key[j] >= tau;
+        val[j] = val[j] && key[j] >= tau_of(j);
         ga[j] = val[j] && giant_summary_hit(a[j], s_sum, covered);
         gb[j] = val[j] && giant_summary_hit(b[j], s_sum, covered);
       }
@@ -653,6 +667,106 @@ __global__ __launch_bounds__(BLK) void k_giant_emit(
   }
 }
 
+// Per graph g of a batch (vertices [voff[g], voff[g + 1])): the most frequent
+// root among up to GSAMP_G evenly spaced vertices of the graph (after
+// k_flatten), or U32_NONE for a graph without vertices.  One block per graph.
+constexpr int GSAMP_G = 256;
+__global__ __launch_bounds__(GSAMP_G) void k_pick_giant_groups(
+    uint32_t k, const ull *__restrict__ voff, const uint32_t *par,
+    uint32_t *__restrict__ groot) {
+  __shared__ ull s[GSAMP_G];
+  for (uint32_t g = blockIdx.x; g < k; g += gridDim.x) {
+    const ull base = voff[g], ng = voff[g + 1] - base;
+    const ull cnt = ng < (ull)GSAMP_G ? ng : (ull)GSAMP_G;
+    const uint32_t j = threadIdx.x;  // one sample per thread
+    s[j] = j < cnt ? (ull)par[base + (ull)j * ng / cnt] : ~0ull;
+    __syncthreads();
+    bitonic_sort_lds<GSAMP_G, GSAMP_G>(s);
+    const ull r = longest_run<GSAMP_G, GSAMP_G>(s);
+    if (threadIdx.x == 0) groot[g] = r == ~0ull ? U32_NONE : (uint32_t)r;
+    __syncthreads();  // s is refilled for the next graph
+  }
+}
+
+// bit x of `bits` = (par[x] == root of the giant of x's graph), for the
+// graphs of a batch (par flattened).
+__global__ __launch_bounds__(SPLIT_BLK) void k_giant_bits_groups(
+    uint32_t n, const uint32_t *par, const ull *__restrict__ voff,
+    const uint32_t *__restrict__ vmap, uint32_t k,
+    const uint32_t *__restrict__ groot, uint32_t *bits) {
+  const uint64_t words = ((uint64_t)n + 31) / 32;
+  for (uint64_t base = (uint64_t)blockIdx.x * SPLIT_BLK; base < n;
+       base += (uint64_t)gridDim.x * SPLIT_BLK) {
+    const uint64_t i = base + threadIdx.x;
+    const uint64_t mask =
+        __ballot(i < n && par[i] == groot[graph_of(vmap, voff, k, i)]);
+    if (lane_id() == 0) {
+      const uint64_t w = i >> 5;  // i is a multiple of 64 for lane 0
+      if (w < words) bits[w] = (uint32_t)mask;
+      if (w + 1 < words) bits[w + 1] = (uint32_t)(mask >> 32);
+    }
+  }
+}
+
+// k_giant_emit for the graphs of a batch: component x's edge goes to the
+// giant root of x's own graph, and each emitted key is offered to best[] of
+// both ends (one at a time: consecutive vertices share a graph, so each
+// pre-check sees the thread's previous offer to that giant).
+__global__ __launch_bounds__(BLK) void k_giant_emit_groups(
+    uint32_t n, ull *__restrict__ gmin, const ull *__restrict__ voff,
+    const uint32_t *__restrict__ vmap, uint32_t k,
+    const uint32_t *__restrict__ groot, uint32_t *__restrict__ oa,
+    uint32_t *__restrict__ ob, ull *__restrict__ ok, ull *out_cnt,
+    uint64_t out_cap, ull *best, ull *parts) {
+  const uint32_t ntiles = (uint32_t)ceil_div_dev(n, LTILE);
+  uint32_t emitted = 0;  // by this thread
+  for (uint32_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    bool keep[IPT];
+    ull key[IPT];
+    uint32_t G[IPT];
+    uint64_t slot[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+      key[j] = x < n ? gmin[x] : KEY_NONE;
+      keep[j] = key[j] != KEY_NONE;
+      G[j] = keep[j] ? groot[graph_of(vmap, voff, k, x)] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      if (keep[j]) {
+        const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+        emitted++;
+        min_store(best, (uint32_t)x, key[j]);
+        min_store(best, G[j], key[j]);
+      }
+    }
+    const uint32_t seg = v % KSEG;
+    block_append<BLK, IPT>(keep, slot, out_cnt + seg * CSTR,
+                           (uint64_t)seg * out_cap);
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      if (keep[j]) {
+        const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+        oa[slot[j]] = (uint32_t)x;
+        ob[slot[j]] = G[j];
+        ok[slot[j]] = key[j];
+      }
+    }
+  }
+  __shared__ ull s_w[BLK / WAVE];
+  ull x = emitted;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
+  if (lane_id() == 0) s_w[threadIdx.x / WAVE] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ull tot = 0;
+    for (int w = 0; w < BLK / WAVE; w++) tot += s_w[w];
+    if (tot) atomicAdd(parts + (blockIdx.x % KSEG) * CSTR, tot);
+  }
+}
+
 // tau = key of rank ~light_edges in a strided sample of TSAMP input keys
 // (KEY_NONE: every edge is light).  The light phase only needs about
 // light_edges edges; the results do not depend on tau.
@@ -675,6 +789,42 @@ __global__ __launch_bounds__(SBLK) void k_threshold(Src src, uint64_t m,
   if (threadIdx.x == 0) {
     ull r = (ull)((double)light_edges * TSAMP / (double)m);
     *tau_out = (light_edges >= m || r >= TSAMP) ? KEY_NONE : s[r];
+  }
+}
+
+// Per graph g of a batch: taus[g] = the key of rank ~light_factor * n_g
+// among up to TSAMP_G evenly spaced keys of the graph's edges (KEY_NONE:
+// every edge of the graph is light).  One block per graph.
+constexpr int TSAMP_G = 256;
+template <class Src>
+__global__ __launch_bounds__(TSAMP_G) void k_threshold_groups(
+    Src src, uint32_t k, const ull *__restrict__ voff,
+    const ull *__restrict__ eoff, uint32_t n, int ord, double light_factor,
+    ull *__restrict__ taus) {
+  __shared__ ull s[TSAMP_G];
+  for (uint32_t g = blockIdx.x; g < k; g += gridDim.x) {
+    const ull e0 = eoff[g], mg = eoff[g + 1] - e0;
+    const ull cnt = mg < (ull)TSAMP_G ? mg : (ull)TSAMP_G;
+    const uint32_t j = threadIdx.x;  // one sample per thread
+    ull key = KEY_NONE;
+    if (j < cnt) {
+      const uint64_t i = e0 + (ull)j * mg / cnt;
+      const uint32_t a = src.U(i), b = src.V(i);
+      if (a < n && b < n && a != b) key = make_key(src.W(i), i, ord);
+    }
+    s[j] = key;
+    __syncthreads();
+    bitonic_sort_lds<TSAMP_G, TSAMP_G>(s);
+    if (j == 0) {
+      const double want = light_factor * (double)(voff[g + 1] - voff[g]);
+      ull t = KEY_NONE;
+      if (want < (double)mg) {
+        const ull r = (ull)(want * (double)cnt / (double)mg);
+        if (r < cnt) t = s[r];
+      }
+      taus[g] = t;
+    }
+    __syncthreads();  // s is refilled for the next graph
   }
 }
 
@@ -854,11 +1004,14 @@ extern "C" int phem_mst_workspace_bytes(uint32_t n, uint64_t m,
 namespace phem {
 
 
+// grp: the graphs of a batch (giant component per graph), or NULL.
 template <class Src>
 static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                           void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
                           uint32_t *d_labels, const phem_mst_opts *opts,
-                          phem_mst_stats *stats, hipStream_t s) {
+                          phem_mst_stats *stats, hipStream_t s,
+                          const GraphGroups *grp = nullptr) {
+  const bool groups = grp && grp->k > 1;
   if (m > PHEM_MAX_EDGES || n > PHEM_MAX_VERTICES)
     return set_err(PHEM_EINVAL, "n=%u m=%llu beyond limits", n,
                    (unsigned long long)m);
@@ -993,7 +1146,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                              ctr + cE(rr), L.ecap, 0ull, n, ord, par, noull,
                              nou, Ea[c ^ 1], Eb[c ^ 1], Ek[c ^ 1],
                              ctr + cE(rr + 1), L.ecap, (ull *)nullptr, best,
-                             (ull *)nullptr, nou);
+                             (ull *)nullptr, nou, EdgeTaus{});
         }
         PHEM_LAUNCHED("k_compact");
       }
@@ -1071,7 +1224,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
                          n, ord, par, noull, nou, Ea[0], Eb[0], Ek[0],
                          ctr + cE(1), L.ecap, ctr + cPart(0), best,
-                         (ull *)nullptr, nou);
+                         (ull *)nullptr, nou, EdgeTaus{});
     }
     PHEM_LAUNCHED("k_compact<R0>");
     int rc = read_sums({{cPart(0), KSEG}, {cA(0), KSEG}, {cE(1), KSEG},
@@ -1092,11 +1245,19 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     if (rc) return rc;
     if (stats) stats->light_rounds = (uint32_t)r;
   } else if (m > 0 && n > 1) {
-    // light phase: edges with key < tau (about light_edges of them)
+    // light phase: edges with key < tau (about light_edges of them; for a
+    // batch, each graph's own threshold gives it about light_factor * n_g)
+    const EdgeTaus et = groups ? EdgeTaus{grp->taus, grp->eoff, grp->emap, grp->k}
+                               : EdgeTaus{};
     {
       PHEM_PROF("k_threshold", s);
-      hipLaunchKernelGGL(k_threshold<Src>, dim3(1), dim3(SBLK), 0, s, src, m,
-                         n, ord, light_edges, ctr + C_TAU);
+      if (groups)
+        hipLaunchKernelGGL(k_threshold_groups<Src>, dim3(grid_for(grp->k, 1)),
+                           dim3(TSAMP_G), 0, s, src, grp->k, grp->voff,
+                           grp->eoff, n, ord, light_factor, grp->taus);
+      else
+        hipLaunchKernelGGL(k_threshold<Src>, dim3(1), dim3(SBLK), 0, s, src, m,
+                           n, ord, light_edges, ctr + C_TAU);
     }
     PHEM_LAUNCHED("k_threshold");
     {
@@ -1105,7 +1266,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
                          n, ord, par, ctr + C_TAU, nou, Ea[0], Eb[0], Ek[0],
                          ctr + cE(0), L.ecap, ctr + cPart(0), best,
-                         (ull *)nullptr, nou);
+                         (ull *)nullptr, nou, et);
     }
     PHEM_LAUNCHED("k_light");
     // the light edge count arrives with the first rounds' counters
@@ -1125,18 +1286,35 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          par);
     }
     PHEM_LAUNCHED("k_flatten");
-    {
-      PHEM_PROF("k_pick_giant", s);
-      hipLaunchKernelGGL(k_pick_giant, dim3(1), dim3(SBLK), 0, s, n, par,
-                         (uint32_t *)(ctr + C_GIANT));
+    if (groups) {
+      {
+        PHEM_PROF("k_pick_giant", s);
+        hipLaunchKernelGGL(k_pick_giant_groups, dim3(grid_for(grp->k, 1)),
+                           dim3(GSAMP_G), 0, s, grp->k, grp->voff, par,
+                           grp->groot);
+      }
+      PHEM_LAUNCHED("k_pick_giant_groups");
+      {
+        PHEM_PROF("k_giant_bits", s);
+        hipLaunchKernelGGL(k_giant_bits_groups, dim3(grid_for(n, SPLIT_BLK)),
+                           dim3(SPLIT_BLK), 0, s, n, par, grp->voff, grp->vmap,
+                           grp->k, (const uint32_t *)grp->groot, gbits);
+      }
+      PHEM_LAUNCHED("k_giant_bits_groups");
+    } else {
+      {
+        PHEM_PROF("k_pick_giant", s);
+        hipLaunchKernelGGL(k_pick_giant, dim3(1), dim3(SBLK), 0, s, n, par,
+                           (uint32_t *)(ctr + C_GIANT));
+      }
+      PHEM_LAUNCHED("k_pick_giant");
+      {
+        PHEM_PROF("k_giant_bits", s);
+        hipLaunchKernelGGL(k_giant_bits, dim3(grid_for(n, BLK)), dim3(BLK), 0, s,
+                           n, par, (const uint32_t *)(ctr + C_GIANT), gbits);
+      }
+      PHEM_LAUNCHED("k_giant_bits");
     }
-    PHEM_LAUNCHED("k_pick_giant");
-    {
-      PHEM_PROF("k_giant_bits", s);
-      hipLaunchKernelGGL(k_giant_bits, dim3(grid_for(n, BLK)), dim3(BLK), 0, s,
-                         n, par, (const uint32_t *)(ctr + C_GIANT), gbits);
-    }
-    PHEM_LAUNCHED("k_giant_bits");
     {
       PHEM_PROF("k_giant_summary", s);
       hipLaunchKernelGGL(k_giant_summary,
@@ -1150,7 +1328,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
                          n, ord, par, ctr + C_TAU, gbits, Ea[cur], Eb[cur],
                          Ek[cur], ctr + cE(r), L.ecap, (ull *)nullptr, best,
-                         gmin, gsum);
+                         gmin, gsum, et);
     }
     PHEM_LAUNCHED("k_heavy");
     {
@@ -1161,10 +1339,17 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     PHEM_LAUNCHED("k_giant_fold");
     {
       PHEM_PROF("k_giant_emit", s);
-      hipLaunchKernelGGL(k_giant_emit, dim3(grid_for(ceil_div(n, LTILE), 1)),
-                         dim3(BLK), 0, s, n, gmin,
-                         (const uint32_t *)(ctr + C_GIANT), Ea[cur], Eb[cur],
-                         Ek[cur], ctr + cE(r), L.ecap, best, ctr + cPart(3));
+      if (groups)
+        hipLaunchKernelGGL(k_giant_emit_groups,
+                           dim3(grid_for(ceil_div(n, LTILE), 1)), dim3(BLK), 0,
+                           s, n, gmin, grp->voff, grp->vmap, grp->k,
+                           (const uint32_t *)grp->groot, Ea[cur], Eb[cur],
+                           Ek[cur], ctr + cE(r), L.ecap, best, ctr + cPart(3));
+      else
+        hipLaunchKernelGGL(k_giant_emit, dim3(grid_for(ceil_div(n, LTILE), 1)),
+                           dim3(BLK), 0, s, n, gmin,
+                           (const uint32_t *)(ctr + C_GIANT), Ea[cur], Eb[cur],
+                           Ek[cur], ctr + cE(r), L.ecap, best, ctr + cPart(3));
     }
     PHEM_LAUNCHED("k_giant_emit");
     // the heavy survivors (at most m + n) are counted with the first heavy
@@ -1223,6 +1408,22 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
 }
 
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+int mst_solve_groups(uint32_t n, uint64_t m, const uint32_t *d_u,
+                     const uint32_t *d_v, const uint32_t *d_w, int wmode,
+                     void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
+                     uint32_t *d_labels, const phem_mst_opts *opts,
+                     phem_mst_stats *stats, hipStream_t s,
+                     const GraphGroups &groups) {
+  if (m && (!d_u || !d_v || !d_w))
+    return set_err(PHEM_EINVAL, "edge arrays are NULL");
+  if (groups.k > 1 && (!groups.voff || !groups.eoff || !groups.vmap ||
+                       !groups.emap || !groups.taus || !groups.groot))
+    return set_err(PHEM_EINVAL, "graph ranges are NULL");
+  const bool vec = aligned16(d_u) && aligned16(d_v) && aligned16(d_w);
+  return mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w, vec}, wmode, d_ws, ws_bytes,
+                        d_mst_ids, d_labels, opts, stats, s, &groups);
+}
 
 }  // namespace phem
 

@@ -14,12 +14,12 @@ constexpr int SBLK = 1024;   // one block sorts the sample
 
 constexpr int GSAMP = 1024;  // giant-root sample size
 
-// In-LDS bitonic sort of N keys (a power of two) by SBLK threads.
-template <int N = NSAMP>
+// In-LDS bitonic sort of N keys (a power of two) by NT threads.
+template <int N = NSAMP, int NT = SBLK>
 __device__ __forceinline__ void bitonic_sort_lds(ull *s) {
   for (int k = 2; k <= N; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < N; i += SBLK) {
+      for (int i = threadIdx.x; i < N; i += NT) {
         int ixj = i ^ j;
         if (ixj > i) {
           ull x = s[i], y = s[ixj];
@@ -44,21 +44,16 @@ __global__ __launch_bounds__(SPLIT_BLK) void k_flatten(uint32_t n, uint32_t *par
   }
 }
 
-// The most frequent root among GSAMP strided vertices (after k_flatten).
-__global__ __launch_bounds__(SBLK) void k_pick_giant(uint32_t n,
-                                                     const uint32_t *par,
-                                                     uint32_t *giant_out) {
-  __shared__ ull s[GSAMP];
-  for (int j = threadIdx.x; j < GSAMP; j += SBLK)
-    s[j] = par[(uint64_t)j * n / GSAMP];
-  __syncthreads();
-  bitonic_sort_lds<GSAMP>(s);
-  // every run start finds its run's end by binary search; the longest run
-  // (smallest root on ties) wins
+// The most frequent value of the sorted s[0..N) (the smallest on ties),
+// ignoring the padding value ~0 (~0 when s holds padding only); every run
+// start finds its run's end by binary search.  All NT threads call; thread 0
+// receives the result.
+template <int N, int NT>
+__device__ __forceinline__ ull longest_run(const ull *s) {
   ull best_len = 0, best_val = ~0ull;
-  for (int j = threadIdx.x; j < GSAMP; j += SBLK) {
-    if (j > 0 && s[j] == s[j - 1]) continue;
-    int lo = j, hi = GSAMP;  // first index > j with a different value
+  for (int j = threadIdx.x; j < N; j += NT) {
+    if (s[j] == ~0ull || (j > 0 && s[j] == s[j - 1])) continue;
+    int lo = j, hi = N;  // first index > j with a different value
     while (lo + 1 < hi) {
       const int mid = (lo + hi) >> 1;
       if (s[mid] == s[j]) lo = mid; else hi = mid;
@@ -76,21 +71,35 @@ __global__ __launch_bounds__(SBLK) void k_pick_giant(uint32_t n,
       best_val = ov;
     }
   }
-  __shared__ ull s_len[SBLK / WAVE], s_val[SBLK / WAVE];
+  __shared__ ull s_len[NT / WAVE], s_val[NT / WAVE];
   if (lane_id() == 0) {
     s_len[threadIdx.x / WAVE] = best_len;
     s_val[threadIdx.x / WAVE] = best_val;
   }
   __syncthreads();
+  ull bl = 0, bv = ~0ull;
   if (threadIdx.x == 0) {
-    ull bl = 0, bv = ~0ull;
-    for (int w = 0; w < SBLK / WAVE; w++)
+    for (int w = 0; w < NT / WAVE; w++)
       if (s_len[w] > bl || (s_len[w] == bl && s_val[w] < bv)) {
         bl = s_len[w];
         bv = s_val[w];
       }
-    *giant_out = (uint32_t)bv;
   }
+  __syncthreads();
+  return bv;
+}
+
+// The most frequent root among GSAMP strided vertices (after k_flatten).
+__global__ __launch_bounds__(SBLK) void k_pick_giant(uint32_t n,
+                                                     const uint32_t *par,
+                                                     uint32_t *giant_out) {
+  __shared__ ull s[GSAMP];
+  for (int j = threadIdx.x; j < GSAMP; j += SBLK)
+    s[j] = par[(uint64_t)j * n / GSAMP];
+  __syncthreads();
+  bitonic_sort_lds<GSAMP>(s);
+  const ull r = longest_run<GSAMP, SBLK>(s);
+  if (threadIdx.x == 0) *giant_out = (uint32_t)r;
 }
 
 // bit x of `bits` = (par[x] == giant root); one ballot per 64 vertices.

@@ -123,6 +123,27 @@ def test_batch_random_mix(P, seed):
     check_batch(P, gs, 1)
 
 
+@pytest.mark.parametrize("lf", [None, 0.25, 4.0])
+def test_batch_split_per_graph(P, lf):
+    """Graphs of very different density in one batch: each gets its own light
+    threshold and its own giant component in the heavy-edge filter.  Sparse
+    uniform graphs (average degree 2..4), dense ones (degree 32), an RMAT
+    graph, a grid with many components, empty graphs in between."""
+    z = np.zeros(0, np.uint32)
+    gs = []
+    for g, (n, d) in enumerate([(1 << 14, 2), (1 << 13, 32), (5000, 4), (1 << 12, 16)]):
+        u, v, w = S.uniform(n, n * d // 2, S.stream_seed(7, g), positive=True)
+        gs.append((n, u, v, w))
+        gs.append((3, z, z, z))
+    u, v, w = S.rmat(13, 12 << 13, S.stream_seed(7, 9), positive=True)
+    gs.append((1 << 13, u, v, w))
+    u, v, w = S.grid(90, S.stream_seed(7, 10))
+    gs.append((90 * 90, u, v, w))
+    out = check_batch(P, gs, 1, lf)
+    st = out["stats"]
+    assert st["light_edges"] + st["heavy_edges"] > 0
+
+
 def test_batch_single_graph_equals_solo(P):
     n, u, v, w, wmode = cases.make("rmat_small")
     out = check_batch(P, [(n, u, v, w)], wmode)
