@@ -367,7 +367,7 @@ def prof_read():
 
 
 def batch_solve(voff, eoff, u, v, w, wmode=W_F32, labels=True, weights=True,
-                stream=None, light_factor=None, ws=None):
+                stream=None, light_factor=None, ws=None, maximum=False):
     """k independent graphs in one solve (phem_batch_solve).
 
     voff / eoff: k + 1 offsets (python ints) into the concatenated vertex and
@@ -390,7 +390,7 @@ def batch_solve(voff, eoff, u, v, w, wmode=W_F32, labels=True, weights=True,
     lab = torch.empty(max(n_total, 1), dtype=torch.int32, device=dev) if labels else None
     wfx = torch.empty(k, dtype=torch.int64, device=dev) if weights else None
     bad = torch.empty(k, dtype=torch.int64, device=dev) if weights else None
-    opts = _opts(light_factor, False)
+    opts = _opts(light_factor, maximum)
     st = MstStats()
     _check(lib().phem_batch_solve(k, hv, he, _p(u), _p(v), _p(w), wmode, _p(ws),
                                   ws.numel(), _p(ids), _p(cnt), _p(lab), _p(wfx), _p(bad),

@@ -144,6 +144,28 @@ def test_batch_split_per_graph(P, lf):
     assert st["light_edges"] + st["heavy_edges"] > 0
 
 
+@pytest.mark.parametrize("lf", [None, 0.0])
+def test_batch_maximum_forest(P, lf):
+    """PHEM_MST_MAXIMUM through the batched solve: every graph's forest is its
+    own maximum spanning forest ((weight, id) descending)."""
+    from max_forest import max_forest_oracle
+    gs = []
+    for g, (n, d) in enumerate([(3000, 3), (1 << 12, 16), (700, 40)]):
+        u, v, w = S.uniform(n, n * d // 2, S.stream_seed(11, g), positive=True)
+        gs.append((n, u, v, w))
+    u, v, w = S.rmat(11, 8 << 11, S.stream_seed(11, 5), positive=True)
+    gs.append((1 << 11, u, v, w))
+    voff, eoff, cu, cv, cw = concat(gs)
+    out = P.batch_solve(voff, eoff, dev(cu), dev(cv), dev(cw), 1,
+                        light_factor=lf, maximum=True)
+    for g, (n, gu, gv, gw) in enumerate(gs):
+        k_ids, k_lab = max_forest_oracle(n, gu, gv, gw, 1)
+        assert np.array_equal(out["ids"][g].numpy().view(np.uint32), k_ids), f"graph {g}"
+        assert np.array_equal(out["labels"][g].numpy().view(np.uint32), k_lab), f"graph {g}"
+        ws, exact = O.weight_sum(k_ids, gw, 1)
+        assert exact and out["weight_fx"][g] == ws, f"graph {g}"
+
+
 def test_batch_single_graph_equals_solo(P):
     n, u, v, w, wmode = cases.make("rmat_small")
     out = check_batch(P, [(n, u, v, w)], wmode)
