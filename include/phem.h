@@ -115,8 +115,9 @@ typedef struct phem_mst_stats {
  *   lightest edges (threshold from a sample of the keys), then drops every
  *   heavier edge whose endpoints are already connected and runs rounds over
  *   the rest (filter-Kruskal, as the CPU path does, MST/src/utils.cu:794-841).
- *   <= 0 runs the canonical Boruvka round loop over all edges
- *   (MST/src/kernels.cu:507-689); its per-round log is then the reference's.
+ *   This split runs when m > 1.5 * light_factor * n; otherwise, and for
+ *   <= 0, the solve runs the canonical Boruvka round loop over all edges
+ *   (MST/src/kernels.cu:507-689), whose per-round log is the reference's.
  *   NaN, like NULL options: the mst_light_factor knob (phem_set_tuning;
  *   1.0 unless set or given by the environment variable
  *   PHEM_MST_LIGHT_FACTOR).

@@ -1032,7 +1032,10 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                                   : mst_default_light_factor();
   const uint64_t light_edges =
       light_factor > 0 ? (uint64_t)(light_factor * (double)n) : m;
-  const bool split = light_edges < m;
+  // The split pays for its second input pass and giant filter only when the
+  // heavy edges are at least half as many as the light ones (m > 1.5 x
+  // light_edges); sparser graphs run the canonical rounds.
+  const bool split = light_edges < m && 2 * m > 3 * light_edges;
 
   char *ws = (char *)d_ws;
   uint32_t *par = (uint32_t *)(ws + L.par);
