@@ -80,6 +80,10 @@ int phem_device_check(int *device_count);
  * default.  phem_prof_count() waits for the recorded events and returns the
  * number of distinct kernels seen since the last reset. */
 int phem_prof_enable(int on);
+/* Records only the launches of the kernel called `name` (as reported by
+ * phem_prof_get); NULL or "" records every kernel.  Keeps the timing of one
+ * kernel from adding event records around all the others. */
+int phem_prof_only(const char *name);
 int phem_prof_reset(void);
 int phem_prof_count(void);
 int phem_prof_get(int i, char *name, int name_cap, uint64_t *launches,

@@ -93,6 +93,7 @@ struct Acc {
 };
 std::atomic<bool> g_prof{false};
 std::mutex g_prof_mu;
+std::string g_prof_only;  // record only this kernel (empty: all); under g_prof_mu
 std::vector<Pending> g_pending;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_free_events;
 std::vector<Acc> g_acc;
@@ -121,6 +122,7 @@ void prof_resolve_locked() {
 ProfScope::ProfScope(const char *name, hipStream_t s) : name_(name), s_(s) {
   if (!g_prof.load(std::memory_order_relaxed)) return;
   std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_only.empty() && g_prof_only != name) return;
   if (!g_free_events.empty()) {
     a_ = g_free_events.back().first;
     b_ = g_free_events.back().second;
@@ -143,6 +145,12 @@ ProfScope::~ProfScope() {
 
 extern "C" int phem_prof_enable(int on) {
   phem::g_prof.store(on != 0);
+  return PHEM_OK;
+}
+
+extern "C" int phem_prof_only(const char *name) {
+  std::lock_guard<std::mutex> lk(phem::g_prof_mu);
+  phem::g_prof_only = name ? name : "";
   return PHEM_OK;
 }
 

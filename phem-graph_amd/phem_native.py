@@ -179,6 +179,7 @@ def lib():
         "phem_write_mtx_weighted": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
         "phem_graph_file_free": (None, [c.POINTER(GraphFile)]),
         "phem_prof_enable": (i32, [i32]),
+        "phem_prof_only": (i32, [c.c_char_p]),
         "phem_prof_reset": (i32, []),
         "phem_prof_count": (i32, []),
         "phem_prof_get": (i32, [i32, c.c_char_p, i32, c.POINTER(c.c_uint64),
@@ -346,6 +347,11 @@ def pack_edges(u, v, out=None):
 
 def prof_enable(on=True):
     lib().phem_prof_enable(int(on))
+
+
+def prof_only(name=None):
+    """Record only kernel `name` (None: every kernel)."""
+    lib().phem_prof_only(name.encode() if name else None)
 
 
 def prof_reset():

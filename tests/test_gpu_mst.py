@@ -86,6 +86,33 @@ def test_mst_repeat_is_deterministic(P):
         assert got[2] == first[2]
 
 
+def test_profiler_records_one_kernel_or_all(P):
+    """phem_prof_only restricts the event profiler to one kernel; NULL
+    restores recording of every kernel."""
+    n, u, v, w, wmode = cases.make("rmat_small")
+    solver = P.MstSolver(n, len(u))
+    du, dv, dw = dev(u), dev(v), dev(w)
+    try:
+        P.prof_only("k_light")
+        P.prof_reset()
+        P.prof_enable(True)
+        solver.solve(du, dv, dw, wmode)
+        P.prof_enable(False)
+        one = P.prof_read()
+        assert list(one) == ["k_light"] and one["k_light"][0] == 1
+        P.prof_only(None)
+        P.prof_reset()
+        P.prof_enable(True)
+        solver.solve(du, dv, dw, wmode)
+        P.prof_enable(False)
+        every = P.prof_read()
+        assert {"k_light", "k_heavy", "k_hook", "k_compact"} <= set(every)
+    finally:
+        P.prof_enable(False)
+        P.prof_only(None)
+        P.prof_reset()
+
+
 def test_device_generators_match_numpy(P):
     n, m = 1 << 14, 1 << 17
     du, dv, dw = P.gen_uniform(n, m, 42)
