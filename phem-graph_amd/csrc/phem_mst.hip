@@ -332,6 +332,35 @@ __global__ __launch_bounds__(BLK) void k_hook(
   }
 }
 
+// Clears the KSEG counters of a segmented list (one thread per counter).
+__global__ __launch_bounds__(KSEG) void k_zero_seg(ull *cnt) {
+  cnt[threadIdx.x * CSTR] = 0ull;
+}
+
+// The vertices whose best[] holds an offer, into a segmented list: the first
+// hook of a phase then runs over active components only (a third or less of
+// the vertices at the start of either phase) instead of all n.
+__global__ __launch_bounds__(BLK) void k_collect_active(
+    uint32_t n, const ull *__restrict__ best, uint32_t *__restrict__ list,
+    ull *list_cnt, uint64_t list_cap) {
+  const uint32_t ntiles = (uint32_t)ceil_div_dev(n, LTILE);
+  for (uint32_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    bool on[IPT];
+    uint64_t slot[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+      on[j] = x < n && best[x] != KEY_NONE;
+    }
+    const uint32_t seg = v % KSEG;
+    block_append<BLK, IPT>(on, slot, list_cnt + seg * CSTR,
+                           (uint64_t)seg * list_cap);
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      if (on[j]) list[slot[j]] = (uint32_t)((uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x);
+  }
+}
+
 // Point every active component at its root; collect the roots.
 __global__ __launch_bounds__(BLK) void k_compress(
     const uint32_t *__restrict__ act, const ull *act_cnt, uint64_t act_cap,
@@ -1097,8 +1126,9 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   uint64_t m_r = 0;
 
   // Rounds over the alive edges in buffer `cur` until none is left.  The
-  // first hook of a phase scans all n vertices (no list); later hooks scan
-  // the roots collected by the previous compress.  On entry m_r may be an
+  // first hook of a phase runs over the vertices holding an offer (collected
+  // from best[]); later hooks over the roots collected by the previous
+  // compress.  On entry m_r may be an
   // upper bound of the alive edges (the producer's count not read yet): the
   // first counter read then reports E(r) with the rounds' counters, together
   // with the `extra` values (written to extra_out), so no read of its own is
@@ -1121,17 +1151,24 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       for (int k = 0; k < batch; k++) {
         const int rr = r + k;
         const int c = cur ^ (k & 1);
-        const bool ident = all && k == 0;
-        const uint64_t hook_tiles =
-            ident ? ceil_div(n, LTILE) : ceil_div(list_bound, LTILE) + KSEG;
+        if (all && k == 0) {
+          // the phase's first component list: the vertices holding an offer
+          // (its counters may hold a previous phase's roots)
+          PHEM_PROF("k_collect_active", s);
+          hipLaunchKernelGGL(k_zero_seg, dim3(1), dim3(KSEG), 0, s, ctr + cL(rr));
+          hipLaunchKernelGGL(k_collect_active,
+                             dim3(grid_for(ceil_div(n, LTILE), 1)), dim3(BLK), 0,
+                             s, n, (const ull *)best, listL, ctr + cL(rr),
+                             L.lcap);
+        }
+        PHEM_LAUNCHED("k_collect_active");
+        const uint64_t hook_tiles = ceil_div(list_bound, LTILE) + KSEG;
         {
           PHEM_PROF("k_hook", s);
           hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(hook_tiles, 1)),
-                             dim3(BLK), 0, s, listL,
-                             ident ? noull : ctr + cL(rr), L.lcap,
-                             ident ? n : 0u, best, src, ord, par, bitmap, listA,
-                             ctr + cA(rr), L.lcap, ctr + cPart(1),
-                             ctr + cPart(2));
+                             dim3(BLK), 0, s, listL, ctr + cL(rr), L.lcap, 0u,
+                             best, src, ord, par, bitmap, listA, ctr + cA(rr),
+                             L.lcap, ctr + cPart(1), ctr + cPart(2));
         }
         PHEM_LAUNCHED("k_hook");
         {
