@@ -13,9 +13,10 @@
 //                form the next round's component list.
 //   k_pass     : one pass over the alive edges relabels both endpoints to
 //                roots, drops edges inside one component, writes the
-//                survivors (u', v', key) through an LDS staging tile and
-//                offers each survivor's key = (wkey(w) << 32) | id to best[]
-//                of both components for the next hook (relabelling_step +
+//                survivors (u', v', key) to the tile's slots of the output
+//                segment (one atomic per tile) and offers each survivor's
+//                key = (wkey(w) << 32) | id to best[] of both components
+//                for the next hook (relabelling_step +
 //                filtering_step + the two remove_if passes,
 //                kernels.cu:273-321,641-649, fused with the next round's
 //                populate_min_weight / populate_eid sweep, kernels.cu:24-70).
@@ -29,7 +30,9 @@
 //              tau; the parent forest is flattened and its giant component
 //              marked in a bitmap; one pass keeps the heavier edges whose
 //              endpoints are still in different components; rounds finish
-//              those.
+//              those.  It runs when the heavy edges are at least half as
+//              many as the light ones; a batch's graphs (disjoint union)
+//              each get their own threshold and giant component.
 // The packed key makes every component's choice unique, so both give the
 // unique minimum spanning forest under (weight, id) order; compaction need
 // not be stable because ids travel with the edges.
@@ -396,9 +399,10 @@ __global__ __launch_bounds__(BLK) void k_compress(
 }
 
 // Edge passes.  Every pass reads a stream of edges, decides which survive,
-// writes the survivors (component u, component v, key) through an LDS
-// staging tile (one atomic per tile, on the counter of the tile's output
-// segment) and offers each survivor's key to best[] of both components.
+// writes the survivors (component u, component v, key) straight from
+// registers to the tile's slots of its output segment (one atomic per tile,
+// on the segment's counter) and offers each survivor's key to best[] of both
+// components.
 //   P_ALL   input edges, relabelled through par (canonical round 0)
 //   P_LIGHT input edges with key < tau, not relabelled (par is the identity)
 //   P_HEAVY input edges with key >= tau whose endpoints are not both in the
