@@ -58,6 +58,67 @@ def test_stream_forest_equals_recompute(P, wmode):
     st.close()
 
 
+def test_stream_edge_cases(P):
+    """Batches that are empty, self-loops only, out of range, re-inserted
+    copies of forest edges (lighter, equal and heavier) and equal-weight
+    ties: after every batch the forest is Kruskal's over every edge inserted
+    so far (invalid records count as inserted, keep their ids and are
+    ignored)."""
+    n = 3000
+    rng = np.random.default_rng(3)
+    z = np.zeros(0, np.uint32)
+    u0, v0, w0 = S.uniform(n, 4 * n, S.stream_seed(9, 0), positive=True)
+    batches = [(u0, v0, w0), (z, z, z)]
+    loops = rng.integers(0, n, 500).astype(np.uint32)
+    batches.append((loops, loops.copy(), w0[:500].copy()))
+    bad_u = rng.integers(0, 2 * n, 700).astype(np.uint32)
+    bad_v = (bad_u + n).astype(np.uint32)
+    batches.append((bad_u, bad_v, w0[:700].copy()))
+    st = P.Stream(n, 4 * n, wmode=1)
+    U, V, W = [], [], []
+
+    def check(t):
+        au, av, aw = np.concatenate(U), np.concatenate(V), np.concatenate(W)
+        k_ids, k_lab = O.kruskal(n, au, av, aw, 1)
+        fu, fv, fw, fg = st.forest()
+        torch.cuda.synchronize()
+        assert np.array_equal(fg.cpu().numpy().astype(np.uint64),
+                              k_ids.astype(np.uint64)), f"batch {t}"
+        assert np.array_equal(host32(st.labels()), k_lab), f"batch {t}"
+        s = st.stats()
+        ws, exact = O.weight_sum(k_ids, aw, 1)
+        # halved weights leave the 2^-24 grid: then no exact sum exists
+        assert bool(s["weight_exact"]) == exact, f"batch {t}"
+        if exact:
+            assert s["weight_fx"] == ws, f"batch {t}"
+        return fu, fv, fw
+
+    t = 0
+    for u, v, w in batches:
+        st.apply(dev(u), dev(v), dev(w))
+        U.append(u), V.append(v), W.append(w)
+        check(t)
+        t += 1
+    # copies of current forest edges: lighter, equal and heavier weights
+    fu, fv, fw = (host32(x) for x in check(t - 1))
+    k = min(len(fu), 600)
+    fwf = fw[:k].view(np.float32)
+    for scale in (0.5, 1.0, 2.0):
+        w = (fwf * np.float32(scale)).astype(np.float32).view(np.uint32)
+        u, v = fv[:k].copy(), fu[:k].copy()  # reversed endpoints
+        st.apply(dev(u), dev(v), dev(w))
+        U.append(u), V.append(v), W.append(w)
+        check(t)
+        t += 1
+    # equal weights everywhere: ties broken by insertion order
+    u, v, _ = S.uniform(n, 2000, S.stream_seed(9, 1))
+    w = np.full(2000, np.float32(0.25)).view(np.uint32)
+    st.apply(dev(u), dev(v), dev(w))
+    U.append(u), V.append(v), W.append(w)
+    check(t)
+    st.close()
+
+
 def test_stream_rejects_oversized_batch(P):
     st = P.Stream(100, 10)
     u = dev(np.zeros(11, np.uint32))
