@@ -169,3 +169,41 @@ def test_u32_weights_server(P):
         assert np.array_equal(lab[:n], k_lab)
         assert r["weight_fx"] == int(w[k_ids].astype(np.uint64).sum())
     srv.close()
+
+
+@pytest.mark.slow
+def test_serving_mix_full_size_equals_plain(P):
+    """128 graphs of the C5 serving mix at full size (n up to 2^20, degree
+    4..32, uniform and RMAT, device resident) through a three-slot server:
+    every result equals the plain one-graph solve of the same graph (forest
+    ids, labels, exact weight)."""
+    import phem_workloads as W
+    graphs = [W.mix_graph(5, g) for g in range(128)]
+    torch.cuda.synchronize()
+    nmax = max(g[0] for g in graphs)
+    mmax = max(g[1].numel() for g in graphs)
+    print(f"mix: {len(graphs)} graphs, {sum(g[1].numel() for g in graphs)} edges",
+          flush=True)
+    solver = P.MstSolver(nmax, mmax)
+    plain = []
+    for n, u, v, w in graphs:
+        ids, lab, st = solver.solve(u, v, w, P.W_F32, n=n)
+        k = st["mst_count"]
+        plain.append((ids[:k].cpu().numpy().view(np.uint32).copy(),
+                      lab[:n].cpu().numpy().view(np.uint32).copy(), st["weight_fx"]))
+    del solver
+    srv = P.Server(mem_bytes=0, slots=3, max_graphs=4096, max_batch_edges=1 << 26,
+                   wmode=P.W_F32)
+    outs = [(np.zeros(g[0], np.uint32), np.zeros(g[0], np.uint32)) for g in graphs]
+    ticks = [srv.submit(n, u, v, w, outs[i][0], outs[i][1])
+             for i, (n, u, v, w) in enumerate(graphs)]
+    res = [srv.wait(t) for t in ticks]
+    st = srv.stats()
+    srv.close()
+    assert st["batches"] > 1
+    for i, r in enumerate(res):
+        assert r["status"] == 0, f"graph {i}"
+        k = r["forest_count"]
+        assert np.array_equal(outs[i][0][:k], plain[i][0]), f"graph {i} ids"
+        assert np.array_equal(outs[i][1], plain[i][1]), f"graph {i} labels"
+        assert r["weight_fx"] == plain[i][2], f"graph {i} weight"
