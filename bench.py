@@ -371,8 +371,10 @@ def main():
 
     # roofline of the dominant kernel (largest share of the profiled pass)
     kb = mst_kernel_bytes(st, m, n)
-    m1 = min(m, int(float(os.environ.get("PHEM_CC_SAMPLE_FACTOR", "2.0")) * n))
+    sf = P.get_tuning()["cc_sample_factor"]
+    m1 = min(m, int(sf * n)) if sf > 0 else m
     kb["k_cc_union"] = 8 * m1
+    # two filtered launches per solve (records m1..3*m1, then the rest)
     kb["k_cc_union_filtered"] = 8 * (m - m1)
     kb["k_cc_labels"] = 8 * n
     dlaunch, dms = dprof[dname]  # the dominant kernel, inside the timed region

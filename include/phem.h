@@ -62,8 +62,9 @@ typedef struct {
                               (1..64; 0 in phem_set_tuning: unchanged) */
   uint32_t reserved;
   double mst_light_factor; /* default light_factor (NaN: unchanged) */
-  double cc_sample_factor; /* CC sampling phase: the first f*n records
-                              (<= 0: one phase; NaN: unchanged) */
+  double cc_sample_factor; /* CC sampling phase: the first f*n records,
+                              then a second giant after 3*f*n records
+                              (default 1.0; <= 0: one phase; NaN: unchanged) */
 } phem_tuning;
 int phem_get_tuning(phem_tuning *t);
 int phem_set_tuning(const phem_tuning *t);

@@ -161,11 +161,13 @@ extern "C" int phem_cc_labels(uint32_t n, const uint32_t *d_parent,
 namespace phem {
 }  // namespace phem
 
-// Two phases when m is well above n (sampling idea of Afforest, Sutton et al.,
-// IPDPS 2018): union the first ~2n records, flatten the parent forest, mark the
-// giant component, then union the remaining records, skipping those with both
-// endpoints in the giant component.  d_labels doubles as scratch for the
-// bitmap and the giant root before receiving the labels.
+// Sampling phases when m is well above n (the sampling idea of Afforest,
+// Sutton et al., IPDPS 2018): union the first m1 = f*n records (f: the
+// cc_sample_factor knob), flatten the parent forest and mark its giant
+// component; union the records up to 3*m1, skipping those with both
+// endpoints in the giant component; mark the giant again, now that it has
+// grown; union the remaining records the same way.  d_labels doubles as
+// scratch for the bitmap and the giant root before receiving the labels.
 extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
                              uint32_t *d_parent, uint64_t *d_forest,
                              uint32_t *d_labels, void *stream) {
@@ -185,40 +187,52 @@ extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
   uint32_t *gsum = bits + (n + 31) / 32;
   rc = phem_cc_union(n, d_edges, m1, d_parent, d_forest, stream);
   if (rc) return rc;
-  {
-    PHEM_PROF("k_flatten", s);
-    hipLaunchKernelGGL(k_flatten, dim3(grid_for(n, SPLIT_BLK)), dim3(SPLIT_BLK),
-                       0, s, n, d_parent);
-  }
-  PHEM_LAUNCHED("k_flatten");
-  {
-    PHEM_PROF("k_pick_giant", s);
-    hipLaunchKernelGGL(k_pick_giant, dim3(1), dim3(SBLK), 0, s, n,
-                       (const uint32_t *)d_parent, giant_root);
-  }
-  PHEM_LAUNCHED("k_pick_giant");
-  {
-    PHEM_PROF("k_giant_bits", s);
-    hipLaunchKernelGGL(k_giant_bits, dim3(grid_for(n, SPLIT_BLK)),
-                       dim3(SPLIT_BLK), 0, s, n, (const uint32_t *)d_parent,
-                       (const uint32_t *)giant_root, bits);
-  }
-  PHEM_LAUNCHED("k_giant_bits");
-  {
-    PHEM_PROF("k_giant_summary", s);
-    hipLaunchKernelGGL(k_giant_summary,
-                       dim3(grid_for(32ull * gsum_words(n, GSUM_MAX), SPLIT_BLK)),
-                       dim3(SPLIT_BLK), 0, s, n, (const uint32_t *)bits, gsum);
-  }
-  PHEM_LAUNCHED("k_giant_summary");
-  {
-    PHEM_PROF("k_cc_union_filtered", s);
-    hipLaunchKernelGGL(k_cc_union<true>, dim3(grid_for(m - m1, CTILE)),
-                       dim3(CBLK), 0, s, (const ull *)d_edges + m1, m - m1, n,
-                       d_parent, (ull *)d_forest, (const uint32_t *)bits,
-                       (const uint32_t *)gsum);
-  }
-  PHEM_LAUNCHED("k_cc_union_filtered");
+  // flatten, pick the giant root, giant bitmap and its LDS summary
+  auto mark = [&]() -> int {
+    {
+      PHEM_PROF("k_flatten", s);
+      hipLaunchKernelGGL(k_flatten, dim3(grid_for(n, SPLIT_BLK)),
+                         dim3(SPLIT_BLK), 0, s, n, d_parent);
+    }
+    PHEM_LAUNCHED("k_flatten");
+    {
+      PHEM_PROF("k_pick_giant", s);
+      hipLaunchKernelGGL(k_pick_giant, dim3(1), dim3(SBLK), 0, s, n,
+                         (const uint32_t *)d_parent, giant_root);
+    }
+    PHEM_LAUNCHED("k_pick_giant");
+    {
+      PHEM_PROF("k_giant_bits", s);
+      hipLaunchKernelGGL(k_giant_bits, dim3(grid_for(n, SPLIT_BLK)),
+                         dim3(SPLIT_BLK), 0, s, n, (const uint32_t *)d_parent,
+                         (const uint32_t *)giant_root, bits);
+    }
+    PHEM_LAUNCHED("k_giant_bits");
+    {
+      PHEM_PROF("k_giant_summary", s);
+      hipLaunchKernelGGL(k_giant_summary,
+                         dim3(grid_for(32ull * gsum_words(n, GSUM_MAX), SPLIT_BLK)),
+                         dim3(SPLIT_BLK), 0, s, n, (const uint32_t *)bits, gsum);
+    }
+    PHEM_LAUNCHED("k_giant_summary");
+    return PHEM_OK;
+  };
+  // records [a0, a1), skipping those inside the marked giant component
+  auto filtered = [&](uint64_t a0, uint64_t a1) -> int {
+    if (a1 <= a0) return PHEM_OK;
+    {
+      PHEM_PROF("k_cc_union_filtered", s);
+      hipLaunchKernelGGL(k_cc_union<true>, dim3(grid_for(a1 - a0, CTILE)),
+                         dim3(CBLK), 0, s, (const ull *)d_edges + a0, a1 - a0, n,
+                         d_parent, (ull *)d_forest, (const uint32_t *)bits,
+                         (const uint32_t *)gsum);
+    }
+    PHEM_LAUNCHED("k_cc_union_filtered");
+    return PHEM_OK;
+  };
+  const uint64_t m2 = m1 < m / 3 ? 3 * m1 : m;
+  if ((rc = mark()) || (rc = filtered(m1, m2))) return rc;
+  if (m2 < m && ((rc = mark()) || (rc = filtered(m2, m)))) return rc;
   return phem_cc_labels(n, d_parent, d_labels, stream);
 }
 

@@ -42,7 +42,7 @@ void tuning_init() {
     }
     unsigned z = 0;
     g_blocks_per_cu.compare_exchange_strong(z, per_cu);
-    double lf = 1.0, sf = 2.0;
+    double lf = 1.0, sf = 1.0;
     if (const char *e = getenv("PHEM_MST_LIGHT_FACTOR")) lf = atof(e);
     if (const char *e = getenv("PHEM_CC_SAMPLE_FACTOR")) sf = atof(e);
     // a value set before first use wins over the environment

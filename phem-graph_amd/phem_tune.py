@@ -31,7 +31,7 @@ CACHE = os.environ.get("PHEM_TUNE_CACHE",
 GRID = {
     "blocks_per_cu": [4, 8, 16, 32],
     "mst_light_factor": [0.25, 0.5, 1.0, 2.0],
-    "cc_sample_factor": [1.0, 1.5, 2.0, 3.0],
+    "cc_sample_factor": [0.5, 1.0, 1.5, 2.0],
 }
 
 
