@@ -58,8 +58,11 @@ constexpr int EPT = 8;         // edges per thread per tile
 constexpr int TILE = BLK * EPT;
 constexpr int WPT = 8;         // bitmap words per thread
 constexpr int WPB = BLK * WPT; // bitmap words per block
-constexpr int IPT = 4;         // components per thread in hook / compress
+constexpr int IPT = 4;         // vertices or components per thread (compress,
+                               // list collection, giant emission)
 constexpr int LTILE = BLK * IPT;
+constexpr int HPT = 1;         // components per thread in the hook
+constexpr int HTILE = BLK * HPT;
 constexpr int SPEC = 3;  // Boruvka rounds launched per counter read
 constexpr uint32_t GSUM_HEAVY = 8192;  // giant-summary words in LDS (32 KB)
 
@@ -238,78 +241,78 @@ __global__ __launch_bounds__(BLK) void k_hook(
     uint32_t *bitmap, uint32_t *act, ull *act_cnt, uint64_t act_cap,
     ull *wsum_parts, ull *inexact_parts) {
   __shared__ SegView sv;
-  if (list_cnt) seg_load<LTILE>(sv, list_cnt);
+  if (list_cnt) seg_load<HTILE>(sv, list_cnt);
   // exact weight of the forest edges this thread hooks (each forest edge is
   // hooked exactly once: keys are unique and the chosen edges form a forest)
   ull wsum = 0, wbad = 0;
-  const uint32_t ntiles_ident = (uint32_t)ceil_div_dev(n_ident, LTILE);
+  const uint32_t ntiles_ident = (uint32_t)ceil_div_dev(n_ident, HTILE);
   for (uint32_t v = blockIdx.x;; v += gridDim.x) {
     uint64_t start;
     uint32_t len;
     if (list_cnt) {
-      if (!seg_tile<LTILE>(sv, v, list_cap, &start, &len)) break;
+      if (!seg_tile<HTILE>(sv, v, list_cap, &start, &len)) break;
     } else {
       if (v >= ntiles_ident) break;
-      start = (uint64_t)v * LTILE;
-      len = (uint32_t)min_u64(LTILE, (uint64_t)n_ident - start);
+      start = (uint64_t)v * HTILE;
+      len = (uint32_t)min_u64(HTILE, (uint64_t)n_ident - start);
     }
     // Each step's loads (list, best, the chosen edges' endpoints, the bitmap
-    // words, the endpoints' parents) are issued for all IPT components
+    // words, the endpoints' parents) are issued for all HPT components
     // before any is used.
-    bool active[IPT];
-    uint32_t c[IPT];
-    ull key[IPT];
-    uint64_t slot[IPT];
+    bool active[HPT];
+    uint32_t c[HPT];
+    ull key[HPT];
+    uint64_t slot[HPT];
 #pragma unroll
-    for (int j = 0; j < IPT; j++) {
+    for (int j = 0; j < HPT; j++) {
       const uint32_t idx = (uint32_t)j * BLK + threadIdx.x;
       c[j] = idx < len ? (list_cnt ? list[start + idx] : (uint32_t)(start + idx))
                        : U32_NONE;
     }
 #pragma unroll
-    for (int j = 0; j < IPT; j++) {
+    for (int j = 0; j < HPT; j++) {
       key[j] = best[c[j] != U32_NONE ? c[j] : 0u];  // unconditional read
       if (c[j] == U32_NONE) key[j] = KEY_NONE;
     }
-    uint32_t e[IPT], eu[IPT], ev[IPT], old[IPT];
+    uint32_t e[HPT], eu[HPT], ev[HPT], old[HPT];
 #pragma unroll
-    for (int j = 0; j < IPT; j++) {
+    for (int j = 0; j < HPT; j++) {
       active[j] = key[j] != KEY_NONE;
       e[j] = active[j] ? key_id(key[j], ord) : 0u;
       if (active[j]) best[c[j]] = KEY_NONE;  // reset for the next round's sweep
     }
 #pragma unroll
-    for (int j = 0; j < IPT; j++) {
+    for (int j = 0; j < HPT; j++) {
       eu[j] = src.U(e[j]);  // e = 0 for inactive entries
       ev[j] = src.V(e[j]);
     }
     // a component whose edge was already marked (both sides chose the same
     // edge) adds no weight: only the atomicOr that sets the bit does
 #pragma unroll
-    for (int j = 0; j < IPT; j++)
+    for (int j = 0; j < HPT; j++)
       old[j] = active[j] ? atomicOr(&bitmap[e[j] >> 5], 1u << (e[j] & 31)) : ~0u;
 #pragma unroll
-    for (int j = 0; j < IPT; j++) {
+    for (int j = 0; j < HPT; j++) {
       if (!((old[j] >> (e[j] & 31)) & 1u)) {
         ull fx;
         if (weight_fx(key_weight(key[j], ord), ord & ORD_F32, &fx)) wsum += fx;
         else wbad++;
       }
     }
-    uint32_t pu[IPT], pv[IPT];
+    uint32_t pu[HPT], pv[HPT];
 #pragma unroll
-    for (int j = 0; j < IPT; j++) {
+    for (int j = 0; j < HPT; j++) {
       pu[j] = par_load(&par[active[j] ? eu[j] : 0u]);
       pv[j] = par_load(&par[active[j] ? ev[j] : 0u]);
     }
 #pragma unroll
-    for (int j = 0; j < IPT; j++)
+    for (int j = 0; j < HPT; j++)
       if (active[j]) uf_union_from(par, eu[j], pu[j], ev[j], pv[j]);
     const uint32_t seg = v % KSEG;
-    block_append<BLK, IPT>(active, slot, act_cnt + seg * CSTR,
+    block_append<BLK, HPT>(active, slot, act_cnt + seg * CSTR,
                            (uint64_t)seg * act_cap);
 #pragma unroll
-    for (int j = 0; j < IPT; j++)
+    for (int j = 0; j < HPT; j++)
       if (active[j]) act[slot[j]] = c[j];
   }
   __shared__ ull s_w[2][BLK / WAVE];
@@ -998,6 +1001,8 @@ static MstLayout mst_layout(uint32_t n, uint64_t m) {
   // same list, so its segments hold both producers' worst case.
   L.ecap = (ceil_div(ceil_div(m ? m : 1, TILE), KSEG) + 2) * TILE +
            ceil_div(ceil_div(n ? n : 1, LTILE), KSEG) * LTILE;
+  // Component lists also come from the hook's smaller tiles (HTILE items):
+  // ceil(ceil(n / HTILE) / KSEG) * HTILE <= n / KSEG + HTILE, below lcap.
   L.lcap = (ceil_div(ceil_div(n ? n : 1, LTILE), KSEG) + 2) * LTILE;
   const uint64_t E = L.ecap * KSEG, LN = L.lcap * KSEG;
   L.par = take(4ull * n);
@@ -1166,7 +1171,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                              L.lcap);
         }
         PHEM_LAUNCHED("k_collect_active");
-        const uint64_t hook_tiles = ceil_div(list_bound, LTILE) + KSEG;
+        const uint64_t hook_tiles = ceil_div(list_bound, HTILE) + KSEG;
         {
           PHEM_PROF("k_hook", s);
           hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(hook_tiles, 1)),
@@ -1177,7 +1182,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
         PHEM_LAUNCHED("k_hook");
         {
           PHEM_PROF("k_compress", s);
-          hipLaunchKernelGGL(k_compress, dim3(grid_for(hook_tiles + KSEG, 1)),
+          hipLaunchKernelGGL(k_compress,
+                             dim3(grid_for(ceil_div(list_bound, LTILE) + 2 * KSEG, 1)),
                              dim3(BLK), 0, s, listA, ctr + cA(rr), L.lcap, par,
                              listL, ctr + cL(rr + 1), L.lcap);
         }
@@ -1249,8 +1255,9 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     const uint64_t t0 = ceil_div(n, LTILE);
     {
       PHEM_PROF("k_hook", s);
-      hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(t0, 1)), dim3(BLK), 0, s,
-                         listL, noull, L.lcap, n, best, src, ord, par, bitmap,
+      hipLaunchKernelGGL(k_hook<Src>, dim3(grid_for(ceil_div(n, HTILE), 1)),
+                         dim3(BLK), 0, s, listL, noull, L.lcap, n, best, src, ord,
+                         par, bitmap,
                          listA, ctr + cA(0), L.lcap, ctr + cPart(1),
                          ctr + cPart(2));
     }
