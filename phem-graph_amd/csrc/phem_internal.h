@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+
 #include "phem.h"
 
 namespace phem {
@@ -53,6 +55,18 @@ int mst_solve_groups(uint32_t n, uint64_t m, const uint32_t *d_u,
                      uint32_t *d_labels, const phem_mst_opts *opts,
                      phem_mst_stats *stats, hipStream_t s,
                      const GraphGroups &groups);
+
+// Work a caller enqueues after the forest ids are written and before the
+// solve's final counter read, so that read's wait covers it: d_count is the
+// device word that holds the forest size (the ids are in d_mst_ids).
+using MstTail = std::function<int(const unsigned long long *d_count)>;
+
+// phem_mst_solve with a tail (structure-of-arrays input).
+int mst_solve_tail(uint32_t n, uint64_t m, const uint32_t *d_u,
+                   const uint32_t *d_v, const uint32_t *d_w, int wmode,
+                   void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
+                   uint32_t *d_labels, const phem_mst_opts *opts,
+                   phem_mst_stats *stats, hipStream_t s, const MstTail &tail);
 
 // Records HIP events around the launches in its scope when profiling is on.
 class ProfScope {

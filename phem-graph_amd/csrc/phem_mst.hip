@@ -1048,7 +1048,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                           void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
                           uint32_t *d_labels, const phem_mst_opts *opts,
                           phem_mst_stats *stats, hipStream_t s,
-                          const GraphGroups *grp = nullptr) {
+                          const GraphGroups *grp = nullptr,
+                          const MstTail *tail = nullptr) {
   const bool groups = grp && grp->k > 1;
   if (m > PHEM_MAX_EDGES || n > PHEM_MAX_VERTICES)
     return set_err(PHEM_EINVAL, "n=%u m=%llu beyond limits", n,
@@ -1278,23 +1279,23 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          (ull *)nullptr, nou, EdgeTaus{});
     }
     PHEM_LAUNCHED("k_compact<R0>");
-    int rc = read_sums({{cPart(0), KSEG}, {cA(0), KSEG}, {cE(1), KSEG},
-                        {cL(1), KSEG}}, hv);
+    // rounds 1.. start at once from upper bounds (m alive edges, n roots);
+    // round 0's counters arrive with their first counter read
+    r = 1;
+    m_r = m;
+    cur = 0;
+    int rc = run_rounds(n, false, {{cPart(0), KSEG}, {cA(0), KSEG}}, hv);
     if (rc) return rc;
     const uint64_t valid = hv[0];
+    // a round counts when it starts with at least one alive edge
+    if (!valid) r = 0;
     if (stats) {
       stats->m_log[0] = valid;
       stats->c_log[0] = hv[1];
       stats->light_edges = valid;
       stats->tau = KEY_NONE;
+      stats->light_rounds = (uint32_t)r;
     }
-    // a round counts when it starts with at least one alive edge
-    r = valid ? 1 : 0;
-    m_r = hv[2];
-    cur = 0;
-    rc = run_rounds(hv[3], false, {}, nullptr);
-    if (rc) return rc;
-    if (stats) stats->light_rounds = (uint32_t)r;
   } else if (m > 0 && n > 1) {
     // light phase: edges with key < tau (about light_edges of them; for a
     // batch, each graph's own threshold gives it about light_factor * n_g)
@@ -1441,6 +1442,10 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                        bitmap, L.words, blk, d_mst_ids);
   }
   PHEM_LAUNCHED("k_bm_emit");
+  if (tail) {
+    const int trc = (*tail)(ctr + C_MSTCNT);
+    if (trc) return trc;
+  }
   int rc = read_sums({{C_MSTCNT, 1}, {cPart(1), KSEG}, {cPart(2), KSEG},
                       {cPart(0), KSEG}}, hv);
   if (rc) return rc;
@@ -1474,6 +1479,18 @@ int mst_solve_groups(uint32_t n, uint64_t m, const uint32_t *d_u,
   const bool vec = aligned16(d_u) && aligned16(d_v) && aligned16(d_w);
   return mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w, vec}, wmode, d_ws, ws_bytes,
                         d_mst_ids, d_labels, opts, stats, s, &groups);
+}
+
+int mst_solve_tail(uint32_t n, uint64_t m, const uint32_t *d_u,
+                   const uint32_t *d_v, const uint32_t *d_w, int wmode,
+                   void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
+                   uint32_t *d_labels, const phem_mst_opts *opts,
+                   phem_mst_stats *stats, hipStream_t s, const MstTail &tail) {
+  if (m && (!d_u || !d_v || !d_w))
+    return set_err(PHEM_EINVAL, "edge arrays are NULL");
+  const bool vec = aligned16(d_u) && aligned16(d_v) && aligned16(d_w);
+  return mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w, vec}, wmode, d_ws, ws_bytes,
+                        d_mst_ids, d_labels, opts, stats, s, nullptr, &tail);
 }
 
 }  // namespace phem

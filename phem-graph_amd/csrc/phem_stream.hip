@@ -62,12 +62,14 @@ __global__ __launch_bounds__(SBLK_S) void k_stream_stage(
   }
 }
 
+// count: the device word holding the forest size (at most `bound`).
 __global__ __launch_bounds__(SBLK_S) void k_stream_keep(
-    const uint32_t *__restrict__ ids, uint64_t count,
+    const uint32_t *__restrict__ ids, const ull *__restrict__ count_p,
     const uint32_t *__restrict__ su, const uint32_t *__restrict__ sv,
     const uint32_t *__restrict__ sw, const ull *__restrict__ sg,
     uint32_t *__restrict__ du, uint32_t *__restrict__ dv,
     uint32_t *__restrict__ dw, ull *__restrict__ dg) {
+  const uint64_t count = *count_p;
   for (uint64_t k = (uint64_t)blockIdx.x * SBLK_S + threadIdx.x; k < count;
        k += (uint64_t)gridDim.x * SBLK_S) {
     const uint32_t p = ids[k];
@@ -176,25 +178,31 @@ extern "C" int phem_stream_apply(phem_stream_t *s, const uint32_t *d_u,
                        s->packed);
   }
   PHEM_LAUNCHED("k_stream_stage");
-  const uint64_t total = s->f + mb;
-  int rc = phem_mst_solve(s->n, total, s->u[c], s->v[c], s->w[c], s->wmode,
-                          s->ws, s->ws_bytes, s->ids, s->labels, nullptr,
-                          &s->last, stream);
-  if (rc) return rc;
-  const uint64_t count = s->last.mst_count;
-  if (count) {
-    PHEM_PROF("k_stream_keep", st);
-    hipLaunchKernelGGL(k_stream_keep, dim3(grid_for(count, SBLK_S)), dim3(SBLK_S),
-                       0, st, s->ids, count, s->u[c], s->v[c], s->w[c],
-                       s->gid[c], s->u[c ^ 1], s->v[c ^ 1], s->w[c ^ 1],
-                       s->gid[c ^ 1]);
-  }
-  PHEM_LAUNCHED("k_stream_keep");
+  int rc;
   if (mb) {
     rc = phem_cc_union(s->n, (const uint64_t *)s->packed, mb, s->parent,
                        (uint64_t *)s->cc_forest, stream);
     if (rc) return rc;
   }
+  // The new forest (at most n - 1 edges) is gathered before the solve's
+  // final counter read, so one wait covers the whole batch.
+  const uint64_t total = s->f + mb;
+  const uint64_t bound = s->n ? s->n - 1 : 0;
+  const MstTail keep = [&](const ull *d_count) -> int {
+    if (!bound) return PHEM_OK;
+    PHEM_PROF("k_stream_keep", st);
+    hipLaunchKernelGGL(k_stream_keep, dim3(grid_for(bound, SBLK_S)),
+                       dim3(SBLK_S), 0, st, s->ids, d_count, s->u[c], s->v[c],
+                       s->w[c], s->gid[c], s->u[c ^ 1], s->v[c ^ 1],
+                       s->w[c ^ 1], s->gid[c ^ 1]);
+    PHEM_LAUNCHED("k_stream_keep");
+    return PHEM_OK;
+  };
+  rc = mst_solve_tail(s->n, total, s->u[c], s->v[c], s->w[c], s->wmode, s->ws,
+                      s->ws_bytes, s->ids, s->labels, nullptr, &s->last, st,
+                      keep);
+  if (rc) return rc;
+  const uint64_t count = s->last.mst_count;
   PHEM_HIP(hipStreamSynchronize(st));
   s->cur ^= 1;
   s->f = count;
