@@ -161,6 +161,38 @@ int phem_mst_solve_aos(uint32_t n, uint64_t m, const uint32_t *d_edges,
                        const phem_mst_opts *opts, phem_mst_stats *stats,
                        void *stream);
 
+/* A whole phem_mst_solve recorded once into a HIP graph and replayed.
+ * Replaces the host-driven round loop of device_boruvka
+ * (MST/src/kernels.cu:526-689: a count_if / remove_if and a host read of
+ * num_edges every round, :577-649) by a fixed number of rounds per phase
+ * that end on the device: a round that starts with no alive edge does
+ * nothing, and no counter is read until phem_mst_graph_stats.
+ * phem_mst_graph_create records the solve of (d_u, d_v, d_w) with these
+ * arguments (the same as phem_mst_solve's, the pointers fixed); a replay
+ * solves whatever those buffers hold at launch time (n and m fixed).
+ * rounds: Boruvka rounds per phase (the canonical rounds after round 0; with
+ *   the light/heavy split, the light phase and the heavy phase each);
+ *   0 = one eager solve of the current inputs picks them (its rounds per
+ *   phase + 1) and writes its result to the outputs.
+ * phem_mst_graph_launch enqueues one replay on `stream` and returns.
+ * phem_mst_graph_stats waits for the last replay on `stream`, fills stats
+ *   (may be NULL) as phem_mst_solve would, and returns PHEM_ENOCONV when a
+ *   phase still had alive edges after its recorded rounds: the outputs are
+ *   then incomplete (record again with more rounds, or call phem_mst_solve).
+ * The graph holds no device memory of the solve: d_ws and the outputs stay
+ * the caller's and must outlive it.  Replays on one workspace must not
+ * overlap. */
+typedef struct phem_mst_graph phem_mst_graph_t;
+int phem_mst_graph_create(uint32_t n, uint64_t m, const uint32_t *d_u,
+                          const uint32_t *d_v, const uint32_t *d_w, int wmode,
+                          void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
+                          uint32_t *d_labels, const phem_mst_opts *opts,
+                          int rounds, void *stream, phem_mst_graph_t **out);
+int phem_mst_graph_launch(phem_mst_graph_t *g, void *stream);
+int phem_mst_graph_stats(phem_mst_graph_t *g, phem_mst_stats *stats,
+                         void *stream);
+int phem_mst_graph_destroy(phem_mst_graph_t *g);
+
 /* d_out[k] = record d_ids[k] of the 3-word {u, v, w} records in d_src, for
  * k < count: materialises forest edges as the reference's `edge` records
  * (the MSF copy-out of device_boruvka, MST/src/kernels.cu:590-596). */

@@ -121,6 +121,11 @@ void prof_resolve_locked() {
 
 ProfScope::ProfScope(const char *name, hipStream_t s) : name_(name), s_(s) {
   if (!g_prof.load(std::memory_order_relaxed)) return;
+  // launches recorded into a graph are not timed
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cap) != hipSuccess ||
+      cap != hipStreamCaptureStatusNone)
+    return;
   std::lock_guard<std::mutex> lk(g_prof_mu);
   if (!g_prof_only.empty() && g_prof_only != name) return;
   if (!g_free_events.empty()) {

@@ -45,6 +45,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <new>
 #include <utility>
 
 #include "phem_dev.h"
@@ -74,8 +75,9 @@ constexpr uint32_t GSUM_HEAVY = 8192;  // giant-summary words in LDS (32 KB)
 constexpr int C_MSTCNT = 0 * CSTR;  // forest size
 constexpr int C_TAU = 1 * CSTR;     // light / heavy key threshold
 constexpr int C_GIANT = 2 * CSTR;   // giant-component root (32-bit)
-constexpr int C_SUMS = 3 * CSTR;    // k_seg_sums output (up to 8 values)
-constexpr int C_PARTS = 4 * CSTR;
+constexpr int C_SUMS = 3 * CSTR;    // k_seg_sums output (up to 16 values)
+constexpr int C_GATE = 4 * CSTR;    // fixed rounds: light phase not converged
+constexpr int C_PARTS = 5 * CSTR;
 constexpr int C_ROUND = C_PARTS + 4 * KSEG * CSTR;
 constexpr int RSTRIDE = 3 * KSEG * CSTR;
 constexpr int NCTR = C_ROUND + RSTRIDE * (PHEM_MAX_ROUNDS + 1);
@@ -975,6 +977,48 @@ __global__ __launch_bounds__(WAVE) void k_seg_sums(const ull *ctr, SumReq q,
   }
 }
 
+// Fixed-round schedule (graph capture): before the heavy pass, a light phase
+// that still has alive edges (E(r) > 0) turns the heavy pass and the giant
+// emission into no-ops (tau = KEY_NONE selects no heavy edge, gmin stays
+// empty), so their appends cannot land on top of the light phase's leftovers;
+// the flag tells the host that the result is incomplete.
+__global__ __launch_bounds__(WAVE) void k_phase_gate(const ull *e_cnt, ull *tau,
+                                                     ull *gate) {
+  const uint32_t lane = threadIdx.x;
+  ull x = (int)lane < KSEG ? e_cnt[lane * CSTR] : 0ull;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
+  if (lane == 0 && x) {
+    *tau = KEY_NONE;
+    *gate = 1ull;
+  }
+}
+
+// Counters of a fixed-round solve for the host, in one copy: forest size,
+// tau, gate, the four partial sums, then E(r), A(r) for r = 0 .. r_end.
+constexpr int FS_HEAD = 7;
+__global__ __launch_bounds__(WAVE) void k_fixed_sums(const ull *ctr, int r_end,
+                                                     ull *out) {
+  const uint32_t lane = threadIdx.x;
+  const int nv = FS_HEAD + 2 * (r_end + 1);
+  for (int i = 0; i < nv; i++) {
+    int off, segs = KSEG;
+    if (i < 3) {
+      off = i == 0 ? C_MSTCNT : i == 1 ? C_TAU : C_GATE;
+      segs = 1;
+    } else if (i < FS_HEAD) {
+      off = cPart(i - 3);
+    } else {
+      const int r = (i - FS_HEAD) >> 1;
+      off = (i - FS_HEAD) & 1 ? cA(r) : cE(r);
+    }
+    ull x = (int)lane < segs ? ctr[off + lane * CSTR] : 0ull;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
+    if (lane == 0) out[i] = x;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // workspace layout
 
@@ -1041,15 +1085,42 @@ extern "C" int phem_mst_workspace_bytes(uint32_t n, uint64_t m,
 
 namespace phem {
 
+// Rounds launched per phase when the solve is recorded for replay: the first
+// phase (canonical rounds after round 0, or the light phase) and the heavy
+// phase.  Nothing is read back between rounds; a round launched after its
+// phase converged finds no alive edge and does nothing.
+struct FixedRounds {
+  int light, heavy;
+};
+
+static double mst_light_factor(const phem_mst_opts *opts) {
+  return opts && !std::isnan(opts->light_factor) ? opts->light_factor
+                                                 : mst_default_light_factor();
+}
+
+static uint64_t mst_light_edges(uint32_t n, uint64_t m, double light_factor) {
+  return light_factor > 0 ? (uint64_t)(light_factor * (double)n) : m;
+}
+
+// The split pays for its second input pass and giant filter only when the
+// heavy edges are at least half as many as the light ones (m > 1.5 x
+// light_edges); sparser graphs run the canonical rounds.
+static bool mst_split(uint32_t n, uint64_t m, double light_factor) {
+  const uint64_t le = mst_light_edges(n, m, light_factor);
+  return le < m && 2 * m > 3 * le;
+}
 
 // grp: the graphs of a batch (giant component per graph), or NULL.
+// tail: work enqueued before the final counter read, or NULL.
+// fr: fixed-round schedule with no counter read at all (stats unwritten).
 template <class Src>
 static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                           void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
                           uint32_t *d_labels, const phem_mst_opts *opts,
                           phem_mst_stats *stats, hipStream_t s,
                           const GraphGroups *grp = nullptr,
-                          const MstTail *tail = nullptr) {
+                          const MstTail *tail = nullptr,
+                          const FixedRounds *fr = nullptr) {
   const bool groups = grp && grp->k > 1;
   if (m > PHEM_MAX_EDGES || n > PHEM_MAX_VERTICES)
     return set_err(PHEM_EINVAL, "n=%u m=%llu beyond limits", n,
@@ -1066,15 +1137,9 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   if (!d_ws || ws_bytes < L.total)
     return set_err(PHEM_ENOMEM, "workspace %llu B < required %llu B",
                    (unsigned long long)ws_bytes, (unsigned long long)L.total);
-  const double light_factor = opts && !std::isnan(opts->light_factor)
-                                  ? opts->light_factor
-                                  : mst_default_light_factor();
-  const uint64_t light_edges =
-      light_factor > 0 ? (uint64_t)(light_factor * (double)n) : m;
-  // The split pays for its second input pass and giant filter only when the
-  // heavy edges are at least half as many as the light ones (m > 1.5 x
-  // light_edges); sparser graphs run the canonical rounds.
-  const bool split = light_edges < m && 2 * m > 3 * light_edges;
+  const double light_factor = mst_light_factor(opts);
+  const uint64_t light_edges = mst_light_edges(n, m, light_factor);
+  const bool split = mst_split(n, m, light_factor);
 
   char *ws = (char *)d_ws;
   uint32_t *par = (uint32_t *)(ws + L.par);
@@ -1109,8 +1174,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   // Counter values for the host: summed on the device, copied to a pinned
   // buffer (one per host thread, kept for the life of the process).
   static thread_local ull *hpin = nullptr;
-  if (!hpin) PHEM_HIP(hipHostMalloc((void **)&hpin, MAX_SUMS * sizeof(ull)));
   auto read_sum_req = [&](const SumReq &q, ull *out) -> int {
+    if (!hpin) PHEM_HIP(hipHostMalloc((void **)&hpin, MAX_SUMS * sizeof(ull)));
     hipLaunchKernelGGL(k_seg_sums, dim3(1), dim3(WAVE), 0, s, ctr, q,
                        ctr + C_SUMS);
     PHEM_LAUNCHED("k_seg_sums");
@@ -1145,17 +1210,19 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   // needed between the producer and the first rounds.
   auto run_rounds = [&](uint64_t list_ub, bool first_all,
                         std::initializer_list<std::pair<int, int>> extra,
-                        ull *extra_out) -> int {
+                        ull *extra_out, int fixed) -> int {
     bool all = first_all;
     bool first = true;
     while (m_r > 0) {
-      // Up to SPEC rounds per counter read.  A round launched after the
-      // phase has converged finds empty inputs and leaves its counters at
-      // zero; grids come from upper bounds (alive edges and component lists
-      // only shrink within a phase), and the kernels loop over the tiles the
-      // device counters hold.
-      const int batch = PHEM_MAX_ROUNDS - r < SPEC ? PHEM_MAX_ROUNDS - r : SPEC;
-      if (batch <= 0)
+      // Up to SPEC rounds per counter read (exactly `fixed` rounds and no
+      // read for a recorded solve).  A round launched after the phase has
+      // converged finds empty inputs and leaves its counters at zero; grids
+      // come from upper bounds (alive edges and component lists only shrink
+      // within a phase), and the kernels loop over the tiles the device
+      // counters hold.
+      const int want = fixed ? fixed : SPEC;
+      const int batch = PHEM_MAX_ROUNDS - r < want ? PHEM_MAX_ROUNDS - r : want;
+      if (batch <= 0 || (fixed && batch < fixed))
         return set_err(PHEM_ENOCONV, "no convergence after %d rounds", r);
       const uint64_t list_bound = all ? n : list_ub;
       for (int k = 0; k < batch; k++) {
@@ -1200,6 +1267,11 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                              (ull *)nullptr, nou, EdgeTaus{});
         }
         PHEM_LAUNCHED("k_compact");
+      }
+      if (fixed) {
+        r += batch;
+        cur ^= (batch & 1);
+        break;
       }
       // E(r .. r+batch), A(r .. r+batch-1), L(r+batch)
       SumReq q{};
@@ -1284,12 +1356,13 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     r = 1;
     m_r = m;
     cur = 0;
-    int rc = run_rounds(n, false, {{cPart(0), KSEG}, {cA(0), KSEG}}, hv);
+    int rc = run_rounds(n, false, {{cPart(0), KSEG}, {cA(0), KSEG}}, hv,
+                        fr ? fr->light : 0);
     if (rc) return rc;
-    const uint64_t valid = hv[0];
+    const uint64_t valid = fr ? 1 : hv[0];
     // a round counts when it starts with at least one alive edge
     if (!valid) r = 0;
-    if (stats) {
+    if (stats && !fr) {
       stats->m_log[0] = valid;
       stats->c_log[0] = hv[1];
       stats->light_edges = valid;
@@ -1324,9 +1397,14 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     // the light edge count arrives with the first rounds' counters
     m_r = m;
     cur = 0;
-    int rc = run_rounds(0, true, {{C_TAU, 1}, {cE(0), KSEG}}, hv);
+    int rc = run_rounds(0, true, {{C_TAU, 1}, {cE(0), KSEG}}, hv,
+                        fr ? fr->light : 0);
     if (rc) return rc;
-    if (stats) {
+    if (fr) {
+      hipLaunchKernelGGL(k_phase_gate, dim3(1), dim3(WAVE), 0, s,
+                         (const ull *)(ctr + cE(r)), ctr + C_TAU, ctr + C_GATE);
+      PHEM_LAUNCHED("k_phase_gate");
+    } else if (stats) {
       stats->tau = hv[0];
       stats->light_edges = hv[1];
       stats->light_rounds = (uint32_t)r;
@@ -1408,9 +1486,10 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     // rounds' counters
     m_r = m + n;
     const int r_heavy = r;
-    rc = run_rounds(0, true, {{cE(r_heavy), KSEG}, {cPart(3), KSEG}}, hv);
+    rc = run_rounds(0, true, {{cE(r_heavy), KSEG}, {cPart(3), KSEG}}, hv,
+                    fr ? fr->heavy : 0);
     if (rc) return rc;
-    if (stats) {
+    if (stats && !fr) {
       stats->heavy_edges = hv[0];
       stats->giant_edges = hv[1];
     }
@@ -1446,6 +1525,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     const int trc = (*tail)(ctr + C_MSTCNT);
     if (trc) return trc;
   }
+  if (fr) return PHEM_OK;
   int rc = read_sums({{C_MSTCNT, 1}, {cPart(1), KSEG}, {cPart(2), KSEG},
                       {cPart(0), KSEG}}, hv);
   if (rc) return rc;
@@ -1519,4 +1599,182 @@ extern "C" int phem_mst_solve_aos(uint32_t n, uint64_t m,
   return mst_solve_impl(n, m, SrcAoS{d_edges, aligned16(d_edges)}, wmode, d_ws,
                         ws_bytes, d_mst_ids, d_labels, opts, stats,
                         (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// A whole solve recorded once into a HIP graph and replayed.
+
+struct phem_mst_graph {
+  uint32_t n = 0;
+  uint64_t m = 0;
+  bool split = false;
+  bool launched = false;
+  phem::FixedRounds fr{0, 0};
+  int r_end = 0;  // last round whose counters the stats read
+  unsigned long long *ctr = nullptr;
+  unsigned long long *d_sums = nullptr, *h_sums = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void mst_graph_free(phem_mst_graph *g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  (void)hipFree(g->d_sums);
+  (void)hipHostFree(g->h_sums);
+  delete g;
+}
+
+extern "C" int phem_mst_graph_create(uint32_t n, uint64_t m, const uint32_t *d_u,
+                                     const uint32_t *d_v, const uint32_t *d_w,
+                                     int wmode, void *d_ws, uint64_t ws_bytes,
+                                     uint32_t *d_mst_ids, uint32_t *d_labels,
+                                     const phem_mst_opts *opts, int rounds,
+                                     void *stream, phem_mst_graph_t **out) {
+  if (!out) return set_err(PHEM_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (m && (!d_u || !d_v || !d_w))
+    return set_err(PHEM_EINVAL, "edge arrays are NULL");
+  if (rounds < 0) return set_err(PHEM_EINVAL, "rounds %d < 0", rounds);
+  const bool work = m > 0 && n > 1;
+  const bool split = work && mst_split(n, m, mst_light_factor(opts));
+  FixedRounds fr{rounds, split ? rounds : 0};
+  if (rounds == 0) {
+    // one solve on the current inputs gives the rounds of each phase; the
+    // recorded schedule keeps one spare round per phase
+    phem_mst_stats st;
+    const int rc = phem_mst_solve(n, m, d_u, d_v, d_w, wmode, d_ws, ws_bytes,
+                                  d_mst_ids, d_labels, opts, &st, stream);
+    if (rc) return rc;
+    const int l = (int)st.light_rounds, h = (int)(st.rounds - st.light_rounds);
+    fr.light = (split ? l : (l > 0 ? l - 1 : 0)) + 1;
+    fr.heavy = split ? h + 1 : 0;
+  }
+  const int r_end = work ? (split ? fr.light + fr.heavy : 1 + fr.light) : 0;
+  if (r_end > PHEM_MAX_ROUNDS)
+    return set_err(PHEM_EINVAL, "%d rounds per phase exceed the %d-round limit",
+                   rounds ? rounds : fr.light, PHEM_MAX_ROUNDS);
+  phem_mst_graph *g = new (std::nothrow) phem_mst_graph();
+  if (!g) return set_err(PHEM_ENOMEM, "host allocation failed");
+  g->n = n;
+  g->m = m;
+  g->split = split;
+  g->fr = fr;
+  g->r_end = r_end;
+  g->ctr = (unsigned long long *)((char *)d_ws + mst_layout(n, m).ctr);
+  const size_t sums = sizeof(ull) * (FS_HEAD + 2 * (PHEM_MAX_ROUNDS + 1));
+  if (hipMalloc((void **)&g->d_sums, sums) != hipSuccess ||
+      hipHostMalloc((void **)&g->h_sums, sums) != hipSuccess) {
+    mst_graph_free(g);
+    return set_err(PHEM_ENOMEM, "graph counter buffers");
+  }
+  // record on a stream of our own (the caller's may be the null stream)
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+    mst_graph_free(g);
+    return set_err(PHEM_EHIP, "hipStreamCreateWithFlags failed");
+  }
+  int rc = PHEM_OK;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = set_err(PHEM_EHIP, "hipStreamBeginCapture failed");
+  } else {
+    const bool vec = aligned16(d_u) && aligned16(d_v) && aligned16(d_w);
+    rc = mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w, vec}, wmode, d_ws,
+                        ws_bytes, d_mst_ids, d_labels, opts, nullptr, cs,
+                        nullptr, nullptr, &fr);
+    const hipError_t e = hipStreamEndCapture(cs, &g->graph);
+    if (!rc && e != hipSuccess)
+      rc = set_err(PHEM_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  (void)hipStreamDestroy(cs);
+  if (!rc && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) !=
+                 hipSuccess)
+    rc = set_err(PHEM_EHIP, "hipGraphInstantiate failed");
+  if (rc) {
+    mst_graph_free(g);
+    return rc;
+  }
+  *out = g;
+  return PHEM_OK;
+}
+
+extern "C" int phem_mst_graph_launch(phem_mst_graph_t *g, void *stream) {
+  if (!g) return set_err(PHEM_EINVAL, "graph handle is NULL");
+  PHEM_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  g->launched = true;
+  return PHEM_OK;
+}
+
+extern "C" int phem_mst_graph_stats(phem_mst_graph_t *g, phem_mst_stats *stats,
+                                    void *stream) {
+  if (!g) return set_err(PHEM_EINVAL, "graph handle is NULL");
+  if (!g->launched) return set_err(PHEM_EINVAL, "graph not launched yet");
+  hipStream_t s = (hipStream_t)stream;
+  const int nv = FS_HEAD + 2 * (g->r_end + 1);
+  hipLaunchKernelGGL(k_fixed_sums, dim3(1), dim3(WAVE), 0, s,
+                     (const ull *)g->ctr, g->r_end, g->d_sums);
+  PHEM_LAUNCHED("k_fixed_sums");
+  PHEM_HIP(hipMemcpyAsync(g->h_sums, g->d_sums, sizeof(ull) * nv,
+                          hipMemcpyDeviceToHost, s));
+  PHEM_HIP(hipStreamSynchronize(s));
+  const ull *h = g->h_sums;
+  auto E = [&](int r) { return h[FS_HEAD + 2 * r]; };
+  auto A = [&](int r) { return h[FS_HEAD + 2 * r + 1]; };
+  phem_mst_stats t;
+  std::memset(&t, 0, sizeof(t));
+  t.mst_count = (uint32_t)h[0];
+  t.valid_edges = h[3];
+  t.weight_fx = h[4];
+  t.weight_exact = h[5] == 0;
+  bool converged = true;
+  if (g->m > 0 && g->n > 1) {
+    int r = 0;
+    // the rounds that started with an alive edge are a prefix of each phase
+    auto phase = [&](int k0, int k1) {
+      for (int k = k0; k < k1 && E(k) > 0; k++, r++) {
+        t.m_log[r] = E(k);
+        t.c_log[r] = A(k);
+      }
+    };
+    if (!g->split) {
+      if (h[3]) {  // round 0 sweeps the input
+        t.m_log[0] = h[3];
+        t.c_log[0] = A(0);
+        r = 1;
+        phase(1, 1 + g->fr.light);
+      }
+      t.light_edges = h[3];
+      t.tau = KEY_NONE;
+      converged = E(1 + g->fr.light) == 0;
+      t.light_rounds = (uint32_t)r;
+    } else {
+      phase(0, g->fr.light);
+      t.light_rounds = (uint32_t)r;
+      const int rh = g->fr.light;
+      phase(rh, rh + g->fr.heavy);
+      t.tau = h[1];
+      t.light_edges = E(0);
+      t.heavy_edges = E(rh);
+      t.giant_edges = h[6];
+      converged = h[2] == 0 && E(rh + g->fr.heavy) == 0;
+    }
+    t.rounds = (uint32_t)r;
+    for (int k = 0; k < r; k++) {
+      t.sum_m += t.m_log[k];
+      t.sum_c += t.c_log[k];
+    }
+  }
+  if (stats) *stats = t;
+  if (!converged)
+    return set_err(PHEM_ENOCONV,
+                   "a phase needed more than its %d recorded rounds",
+                   g->split ? (g->fr.light > g->fr.heavy ? g->fr.light : g->fr.heavy)
+                            : g->fr.light);
+  return PHEM_OK;
+}
+
+extern "C" int phem_mst_graph_destroy(phem_mst_graph_t *g) {
+  mst_graph_free(g);
+  return PHEM_OK;
 }

@@ -133,6 +133,11 @@ def lib():
                                  c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
         "phem_mst_solve_aos": (i32, [u32, u64, vp, i32, vp, u64, vp, vp,
                                      c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
+        "phem_mst_graph_create": (i32, [u32, u64, vp, vp, vp, i32, vp, u64, vp, vp,
+                                        c.POINTER(MstOpts), i32, vp, c.POINTER(vp)]),
+        "phem_mst_graph_launch": (i32, [vp, vp]),
+        "phem_mst_graph_stats": (i32, [vp, c.POINTER(MstStats), vp]),
+        "phem_mst_graph_destroy": (i32, [vp]),
         "phem_cc_init": (i32, [u32, vp, vp, vp]),
         "phem_cc_union": (i32, [u32, vp, u64, vp, vp, vp]),
         "phem_cc_labels": (i32, [u32, vp, vp, vp]),
@@ -282,6 +287,57 @@ class MstSolver:
                "phem_mst_solve_aos")
         s = st.as_dict()
         return self.ids[: s["mst_count"]], (self.labels[: self.n] if labels else None), s
+
+
+class MstGraph:
+    """phem_mst_graph_*: one solve of the device buffers u, v, w recorded into
+    a HIP graph, replayed by launch(); stats() waits for the replay and
+    returns the round log (PhemError with PHEM_ENOCONV when a phase needed
+    more than its recorded rounds).  The buffers' contents may change between
+    replays; their addresses and sizes may not."""
+
+    def __init__(self, n, u, v, w, wmode=W_F32, rounds=0, light_factor=None,
+                 maximum=False, labels=True, stream=None):
+        self.n, self.m = n, u.numel()
+        dev = u.device
+        self.inputs = (u, v, w)  # kept alive: the graph holds their addresses
+        self.ws_bytes = mst_workspace_bytes(n, self.m)
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
+        self.ids = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        self.labels = torch.empty(max(n, 1), dtype=torch.int32, device=dev) if labels else None
+        h = ctypes.c_void_p()
+        _check(lib().phem_mst_graph_create(n, self.m, _p(u), _p(v), _p(w), wmode,
+                                           _p(self.ws), self.ws_bytes, _p(self.ids),
+                                           _p(self.labels), _opts(light_factor, maximum),
+                                           rounds, _stream(stream), ctypes.byref(h)),
+               "phem_mst_graph_create")
+        self.h = h
+
+    def launch(self, stream=None):
+        _check(lib().phem_mst_graph_launch(self.h, _stream(stream)), "phem_mst_graph_launch")
+
+    def stats(self, stream=None):
+        st = MstStats()
+        _check(lib().phem_mst_graph_stats(self.h, ctypes.byref(st), _stream(stream)),
+               "phem_mst_graph_stats")
+        return st.as_dict()
+
+    def result(self, stream=None):
+        """(ids, labels, stats) of the last replay."""
+        s = self.stats(stream)
+        lab = self.labels[: self.n] if self.labels is not None else None
+        return self.ids[: s["mst_count"]], lab, s
+
+    def close(self):
+        if self.h:
+            lib().phem_mst_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def mst_solve(n, u, v, w, wmode=W_F32, labels=True, light_factor=None, maximum=False):
