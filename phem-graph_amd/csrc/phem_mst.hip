@@ -507,9 +507,18 @@ __global__ __launch_bounds__(BLK) void k_pass(
 #pragma unroll
       for (int j = 0; j < EPT; j++) nval += val[j];
     }
-    // the threshold of edge j (its graph's, for a batch)
+    // the threshold of edge j (its graph's, for a batch): one lookup per
+    // tile when the whole tile lies in one graph (graph ids grow with the
+    // edge index, so the first and last edge decide)
+    ull ttau = tau;
+    bool one_graph = true;
+    if ((MODE == P_LIGHT || MODE == P_HEAVY) && et.taus) {
+      const uint32_t g0 = graph_of(et.emap, et.eoff, et.k, start);
+      one_graph = g0 == graph_of(et.emap, et.eoff, et.k, end - 1);
+      ttau = et.taus[g0];
+    }
     auto tau_of = [&](int j) -> ull {
-      if (!et.taus) return tau;
+      if (one_graph) return ttau;
       const uint64_t i = start + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
       return et.taus[graph_of(et.emap, et.eoff, et.k, i < end ? i : start)];
     };
