@@ -1678,7 +1678,9 @@ extern "C" int phem_mst_graph_create(uint32_t n, uint64_t m, const uint32_t *d_u
     mst_graph_free(g);
     return set_err(PHEM_ENOMEM, "graph counter buffers");
   }
-  // record on a stream of our own (the caller's may be the null stream)
+  // record on a stream of our own (the caller's may be the null stream);
+  // the launch sizing reads the device's CU count before recording starts
+  (void)resident_blocks();
   hipStream_t cs = nullptr;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
     mst_graph_free(g);
