@@ -278,6 +278,10 @@ int phem_stream_labels(const phem_stream_t *s, uint32_t *d_labels,
                        void *stream);
 /* Union-find spanning forest (packed hooking records, n slots). */
 int phem_stream_cc_forest(const phem_stream_t *s, const uint64_t **d_forest);
+/* Copies those n slots to d_out (the T1edges array that the reference's
+ * SpanningTree fills, CC/ExternalSpanningTree.cu:46-59). */
+int phem_stream_copy_cc_forest(const phem_stream_t *s, uint64_t *d_out,
+                               void *stream);
 
 /* ---------------------------------------------------------------------------
  * Batched solve: k independent graphs in one pass (one disjoint union).

@@ -246,6 +246,16 @@ extern "C" int phem_stream_cc_forest(const phem_stream_t *s,
   return PHEM_OK;
 }
 
+extern "C" int phem_stream_copy_cc_forest(const phem_stream_t *s,
+                                          uint64_t *d_out, void *stream) {
+  if (!s) return set_err(PHEM_EINVAL, "stream handle is NULL");
+  if (!d_out) return set_err(PHEM_EINVAL, "d_out is NULL");
+  if (s->n)
+    PHEM_HIP(hipMemcpyAsync(d_out, s->cc_forest, 8ull * s->n,
+                            hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return PHEM_OK;
+}
+
 extern "C" int phem_stream_copy_forest(const phem_stream_t *s, uint32_t *d_u,
                                        uint32_t *d_v, uint32_t *d_w,
                                        uint64_t *d_gid, void *stream) {

@@ -156,6 +156,7 @@ def lib():
                                     c.POINTER(c.c_uint64)]),
         "phem_stream_labels": (i32, [vp, vp, vp]),
         "phem_stream_cc_forest": (i32, [vp, c.POINTER(vp)]),
+        "phem_stream_copy_cc_forest": (i32, [vp, vp, vp]),
         "phem_batch_workspace_bytes": (i32, [u64, u64, u32, c.POINTER(c.c_uint64)]),
         "phem_batch_solve": (i32, [u32, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64),
                                    vp, vp, vp, i32, vp, u64, vp, vp, vp, vp, vp,
@@ -525,6 +526,14 @@ class Stream:
         lab = torch.empty(max(self.n, 1), dtype=torch.int32, device=self.device)
         _check(lib().phem_stream_labels(self.h, _p(lab), _stream()), "phem_stream_labels")
         return lab[: self.n]
+
+    def cc_forest(self):
+        """Copy of the union-find forest: n packed (u << 32 | v) slots, all
+        ones where no edge hooked that vertex."""
+        f = torch.empty(max(self.n, 1), dtype=torch.int64, device=self.device)
+        _check(lib().phem_stream_copy_cc_forest(self.h, _p(f), _stream()),
+               "phem_stream_copy_cc_forest")
+        return f[: self.n]
 
 
 def _host_ptr(a):

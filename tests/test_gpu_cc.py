@@ -40,6 +40,18 @@ def forest_ids(n, u, v, forest):
     return np.sort(order[pos]).astype(np.uint32)
 
 
+def assert_spanning_forest(n, fo, want):
+    """fo (n packed slots) holds n - c records whose union-find partition is
+    `want`: a graph with n - c edges and c components has no cycle, so the
+    records form a spanning forest of every component."""
+    f = fo[fo != NO_EDGE]
+    c = len(np.unique(want))
+    assert len(f) == n - c
+    fu = (f >> np.uint64(32)).astype(np.uint32)
+    fv = (f & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    assert np.array_equal(O.par_cc(n, fu, fv), want)
+
+
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_cc_matches_oracle(P, name):
     n, u, v, w, wmode = cases.make(name)
@@ -53,9 +65,9 @@ def test_cc_matches_oracle(P, name):
     want = O.bfs_labels(n, u, v)
     assert np.array_equal(lab, want)
     fo = f.cpu().numpy().view(np.uint64)
-    ids = forest_ids(n, u, v, fo)
-    # duplicates may map two slots to one id; use the record count
-    assert int((fo != NO_EDGE).sum()) == n - len(np.unique(want))
+    ids = forest_ids(n, u, v, fo)  # every forest record is an input record
+    assert_spanning_forest(n, fo, want)
+    # duplicates may map two slots to one id
     if len(np.unique(ids)) == len(ids):
         assert O.check_forest(n, u, v, ids, want) == 0
 
@@ -106,10 +118,13 @@ def test_cc_rmat_full_size(P):
     e = P.pack_edges(du, dv)
     lab, _, f = P.cc_solve(n, e)
     torch.cuda.synchronize()
+    # every forest record is an input record (membership tested on the device)
+    fd = f[f != -1]
+    assert bool(torch.isin(fd, e).all())
+    del fd
     lab = lab.cpu().numpy().view(np.uint32)
     u, v = du.cpu().numpy().view(np.uint32), dv.cpu().numpy().view(np.uint32)
     del e, du, dv
     want = O.par_cc(n, u, v)
     assert np.array_equal(lab, want)
-    fo = f.cpu().numpy().view(np.uint64)
-    assert int((fo != NO_EDGE).sum()) == n - len(np.unique(want))
+    assert_spanning_forest(n, f.cpu().numpy().view(np.uint64), want)

@@ -28,6 +28,24 @@ def host32(t):
     return t.cpu().numpy().view(np.uint32)
 
 
+NO_EDGE = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def check_cc_forest(st, n, au, av, labels):
+    """The stream's union-find forest: n - c packed records, each one an
+    inserted edge, with the partition of every edge so far (so acyclic)."""
+    fo = st.cc_forest().cpu().numpy().view(np.uint64)
+    f = fo[fo != NO_EDGE]
+    assert len(f) == n - len(np.unique(labels))
+    lo, hi = np.minimum(au, av).astype(np.uint64), np.maximum(au, av).astype(np.uint64)
+    rec = np.unique((lo << np.uint64(32)) | hi)
+    pos = np.minimum(np.searchsorted(rec, f), len(rec) - 1)
+    assert (rec[pos] == f).all()
+    fu = (f >> np.uint64(32)).astype(np.uint32)
+    fv = (f & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    assert np.array_equal(O.bfs_labels(n, fu, fv), labels)
+
+
 @pytest.mark.parametrize("wmode", [1, 0])
 def test_stream_forest_equals_recompute(P, wmode):
     scale, m0 = 13, 1 << 15
@@ -55,6 +73,7 @@ def test_stream_forest_equals_recompute(P, wmode):
             assert s["weight_fx"] == ws
         assert s["edges_inserted"] == len(au)
         assert np.array_equal(host32(st.labels()), k_lab), f"batch {t}"
+        check_cc_forest(st, n, au, av, k_lab)
     st.close()
 
 
@@ -85,6 +104,7 @@ def test_stream_edge_cases(P):
         assert np.array_equal(fg.cpu().numpy().astype(np.uint64),
                               k_ids.astype(np.uint64)), f"batch {t}"
         assert np.array_equal(host32(st.labels()), k_lab), f"batch {t}"
+        check_cc_forest(st, n, au, av, k_lab)
         s = st.stats()
         ws, exact = O.weight_sum(k_ids, aw, 1)
         # halved weights leave the 2^-24 grid: then no exact sum exists
