@@ -215,6 +215,38 @@ def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_gra
     }
 
 
+def replay_bench(P, torch, scale, m_log2, seed, reps=30):
+    """Recorded solve: one RMAT graph (n = 2^scale, 2^m_log2 edges) solved
+    eagerly (phem_mst_solve) and replayed from its HIP graph
+    (phem_mst_graph_launch + phem_mst_graph_stats); host time per solve,
+    median of `reps`, and whether the replay's forest equals the eager one."""
+    dev = torch.device("cuda:0")
+    n, m = 1 << scale, 1 << m_log2
+    u, v, w = P.gen_rmat(scale, m, seed, positive=True, device=dev)
+    solver = P.MstSolver(n, m, device=dev)
+    g = P.MstGraph(n, u, v, w, P.W_F32, rounds=0)
+
+    def timed(f):
+        for _ in range(3):
+            f()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            r = f()
+            ts.append(time.perf_counter() - a)
+        ts.sort()
+        return ts[len(ts) // 2], r
+
+    t_e, (ids, _, st) = timed(lambda: solver.solve(u, v, w, P.W_F32))
+    t_g, (gids, _, gst) = timed(lambda: (g.launch(), g.result())[1])
+    same = bool(torch.equal(ids, gids) and st["weight_fx"] == gst["weight_fx"])
+    g.close()
+    return {"workload": f"rmat_n2^{scale}_m2^{m_log2}", "eager_ms": round(t_e * 1e3, 4),
+            "replay_ms": round(t_g * 1e3, 4), "rounds": st["rounds"],
+            "replay_equals_eager": same}
+
+
 def stream_bench(P, torch, scale, m0_log2, batches, batch_log2, seed):
     """Streaming forest: an RMAT graph (n = 2^scale, 2^m0_log2 edges) applied
     as the first batch, then `batches` batches of 2^batch_log2 uniform
@@ -276,6 +308,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix run")
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming run")
+    ap.add_argument("--no-replay", action="store_true",
+                    help="skip the recorded-solve (HIP graph) run")
     ap.add_argument("--serve-graphs", type=int, default=4096)
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-batch-log2", type=int, default=27,
@@ -457,6 +491,8 @@ def main():
             del du, dv, dw, de, solver, cc_par, cc_lab, cc_forest
             torch.cuda.empty_cache()
         out["streaming"] = stream_bench(P, torch, 22, 24, 256, 20, 4)
+    if not args.no_replay:
+        out["recorded_solve"] = replay_bench(P, torch, 16, 20, 1)
 
     print(json.dumps(out), flush=True)
 

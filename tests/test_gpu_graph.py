@@ -117,3 +117,24 @@ def test_graph_rejects_bad_arguments(P):
         P.MstGraph(4, u, u, u, rounds=-1)
     with pytest.raises(P.PhemError):
         P.MstGraph(4, u, u, u, rounds=64)  # round 0 + 64 > 64 rounds
+
+
+@pytest.mark.parametrize("lf", [None, 0.0])
+@pytest.mark.parametrize("name", ["rmat_small", "grid_small", "u32_weights", "many_components"])
+def test_graph_maximum_forest(P, name, lf):
+    """The recorded solve with PHEM_MST_MAXIMUM equals the eager maximum
+    spanning forest, replay after replay."""
+    n, u, v, w, wmode = cases.make(name)
+    du, dv, dw = dev(u), dev(v), dev(w)
+    e_ids, e_lab, eager = P.mst_solve(n, du, dv, dw, wmode, light_factor=lf, maximum=True)
+    e_ids, e_lab = host(e_ids).copy(), host(e_lab).copy()
+    g = P.MstGraph(n, du, dv, dw, wmode, rounds=0, light_factor=lf, maximum=True)
+    for rep in range(2):
+        g.launch()
+        ids, lab, st = g.result()
+        assert np.array_equal(host(ids), e_ids), rep
+        assert np.array_equal(host(lab), e_lab), rep
+        for k in LOG_KEYS:
+            assert st[k] == eager[k], (rep, k)
+        torch.cuda.synchronize()
+    g.close()
