@@ -37,7 +37,10 @@ __global__ __launch_bounds__(BBLK) void k_tile_map(const ull *__restrict__ off,
 
 // Global endpoints (a local id outside its graph maps to an invalid vertex)
 // and, from pointer tables, the gathered weights.  Consecutive threads take
-// consecutive edges, so the reads of one graph's arrays are coalesced.
+// consecutive edges, so the reads of one graph's arrays are coalesced.  A
+// block takes RB_PT * BBLK consecutive edges at a time and looks their graph
+// up once when they all lie in one graph.
+constexpr int RB_PT = 4;
 template <bool TABLE>
 __global__ __launch_bounds__(BBLK) void k_batch_rebase(
     const uint32_t *__restrict__ u, const uint32_t *__restrict__ v,
@@ -46,22 +49,31 @@ __global__ __launch_bounds__(BBLK) void k_batch_rebase(
     const ull *__restrict__ eoff, const uint32_t *__restrict__ emap, uint32_t k,
     uint32_t *__restrict__ gu, uint32_t *__restrict__ gv,
     uint32_t *__restrict__ gw) {
-  for (uint64_t i = (uint64_t)blockIdx.x * BBLK + threadIdx.x; i < m;
-       i += (uint64_t)gridDim.x * BBLK) {
-    const uint32_t g = graph_of(emap, eoff, k, i);
-    const ull base = voff[g], ng = voff[g + 1] - base;
-    uint32_t a, b;
-    if (TABLE) {
-      const uint64_t j = i - eoff[g];
-      a = ((const uint32_t *)pu[g])[j];
-      b = ((const uint32_t *)pv[g])[j];
-      gw[i] = ((const uint32_t *)pw[g])[j];
-    } else {
-      a = u[i];
-      b = v[i];
+  constexpr uint64_t T = (uint64_t)RB_PT * BBLK;
+  for (uint64_t t0 = (uint64_t)blockIdx.x * T; t0 < m; t0 += (uint64_t)gridDim.x * T) {
+    const uint64_t t1 = t0 + T < m ? t0 + T : m;
+    const uint32_t g0 = graph_of(emap, eoff, k, t0);
+    const bool one = g0 == graph_of(emap, eoff, k, t1 - 1);
+#pragma unroll
+    for (int r = 0; r < RB_PT; r++) {
+      const uint64_t i = t0 + (uint64_t)r * BBLK + threadIdx.x;
+      if (i < t1) {
+        const uint32_t g = one ? g0 : graph_of(emap, eoff, k, i);
+        const ull base = voff[g], ng = voff[g + 1] - base;
+        uint32_t a, b;
+        if (TABLE) {
+          const uint64_t j = i - eoff[g];
+          a = ((const uint32_t *)pu[g])[j];
+          b = ((const uint32_t *)pv[g])[j];
+          gw[i] = ((const uint32_t *)pw[g])[j];
+        } else {
+          a = u[i];
+          b = v[i];
+        }
+        gu[i] = a < ng ? (uint32_t)(base + a) : U32_NONE;
+        gv[i] = b < ng ? (uint32_t)(base + b) : U32_NONE;
+      }
     }
-    gu[i] = a < ng ? (uint32_t)(base + a) : U32_NONE;
-    gv[i] = b < ng ? (uint32_t)(base + b) : U32_NONE;
   }
 }
 
@@ -132,17 +144,25 @@ __global__ __launch_bounds__(BBLK) void k_batch_weights(
     ull sum = 0, bad = 0;
     if (t < nthreads) {
       const uint64_t a = t * WCH, b = a + WCH < count ? a + WCH : count;
-      g = graph_of(emap, eoff, k, ids[a]);
-      for (uint64_t i = a; i < b; i++) {
-        const uint32_t e = ids[i];
-        while (g + 1 < k && eoff[g + 1] <= e) {
-          if (sum) atomicAdd(&weights[g], sum);
-          if (bad) atomicAdd(&inexact[g], bad);
-          sum = bad = 0;
-          g++;
+      // the chunk's ids, then its weights, are loaded together
+      uint32_t e[WCH], we[WCH];
+#pragma unroll
+      for (int j = 0; j < WCH; j++) e[j] = ids[a + j < b ? a + j : a];
+#pragma unroll
+      for (int j = 0; j < WCH; j++) we[j] = w[e[j]];
+      g = graph_of(emap, eoff, k, e[0]);
+#pragma unroll
+      for (int j = 0; j < WCH; j++) {
+        if (a + j < b) {
+          while (g + 1 < k && eoff[g + 1] <= e[j]) {
+            if (sum) atomicAdd(&weights[g], sum);
+            if (bad) atomicAdd(&inexact[g], bad);
+            sum = bad = 0;
+            g++;
+          }
+          ull fx;
+          if (weight_fx(we[j], wmode, &fx)) sum += fx; else bad++;
         }
-        ull fx;
-        if (weight_fx(w[e], wmode, &fx)) sum += fx; else bad++;
       }
     }
     // segmented inclusive scan over runs of equal g (lane order = id order)
@@ -305,12 +325,12 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
   if (m_total) {
     PHEM_PROF("k_batch_rebase", s);
     if (table)
-      hipLaunchKernelGGL(k_batch_rebase<true>, dim3(grid_for(m_total, BBLK)),
+      hipLaunchKernelGGL(k_batch_rebase<true>, dim3(grid_for(m_total, RB_PT * BBLK)),
                          dim3(BBLK), 0, s, nullptr, nullptr, ptrs, ptrs + k,
                          ptrs + 2ull * k, m_total, voff, eoff, emap, k, gu, gv,
                          (uint32_t *)(ws + L.gw));
     else
-      hipLaunchKernelGGL(k_batch_rebase<false>, dim3(grid_for(m_total, BBLK)),
+      hipLaunchKernelGGL(k_batch_rebase<false>, dim3(grid_for(m_total, RB_PT * BBLK)),
                          dim3(BBLK), 0, s, A.d_u, A.d_v, nullptr, nullptr,
                          nullptr, m_total, voff, eoff, emap, k, gu, gv, nullptr);
   }
