@@ -757,36 +757,60 @@ __global__ __launch_bounds__(SPLIT_BLK) void k_giant_bits_groups(
 
 // k_giant_emit for the graphs of a batch: component x's edge goes to the
 // giant root of x's own graph, and each emitted key is offered to best[] of
-// both ends (one at a time: consecutive vertices share a graph, so each
-// pre-check sees the thread's previous offer to that giant).
+// both ends.  A tile inside one graph offers the block's minimum to that
+// graph's giant once; a tile across graphs offers one key at a time
+// (consecutive vertices share a graph, so each pre-check sees the thread's
+// previous offer to that giant).
 __global__ __launch_bounds__(BLK) void k_giant_emit_groups(
     uint32_t n, ull *__restrict__ gmin, const ull *__restrict__ voff,
     const uint32_t *__restrict__ vmap, uint32_t k,
     const uint32_t *__restrict__ groot, uint32_t *__restrict__ oa,
     uint32_t *__restrict__ ob, ull *__restrict__ ok, ull *out_cnt,
     uint64_t out_cap, ull *best, ull *parts) {
+  __shared__ ull s_g[BLK / WAVE];
   const uint32_t ntiles = (uint32_t)ceil_div_dev(n, LTILE);
   uint32_t emitted = 0;  // by this thread
   for (uint32_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    const uint64_t x0 = (uint64_t)v * LTILE;
+    const uint64_t x1 = x0 + LTILE < n ? x0 + LTILE : n;
+    const uint32_t g0 = graph_of(vmap, voff, k, x0);
+    const bool one = g0 == graph_of(vmap, voff, k, x1 - 1);  // block-uniform
     bool keep[IPT];
     ull key[IPT];
     uint32_t G[IPT];
     uint64_t slot[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-      const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+      const uint64_t x = x0 + (uint64_t)j * BLK + threadIdx.x;
       key[j] = x < n ? gmin[x] : KEY_NONE;
       keep[j] = key[j] != KEY_NONE;
-      G[j] = keep[j] ? groot[graph_of(vmap, voff, k, x)] : 0u;
+      G[j] = keep[j] ? groot[one ? g0 : graph_of(vmap, voff, k, x)] : 0u;
     }
+    ull gbest = KEY_NONE;  // this thread's lightest key for the tile's giant
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
       if (keep[j]) {
-        const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+        const uint64_t x = x0 + (uint64_t)j * BLK + threadIdx.x;
         emitted++;
         min_store(best, (uint32_t)x, key[j]);
-        min_store(best, G[j], key[j]);
+        if (one) gbest = key[j] < gbest ? key[j] : gbest;
+        else min_store(best, G[j], key[j]);
       }
+    }
+    if (one) {
+#pragma unroll
+      for (int d = 1; d < WAVE; d <<= 1) {
+        const ull o = __shfl_xor(gbest, d);
+        gbest = o < gbest ? o : gbest;
+      }
+      if (lane_id() == 0) s_g[threadIdx.x / WAVE] = gbest;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        ull g = KEY_NONE;
+        for (int w = 0; w < BLK / WAVE; w++) g = s_g[w] < g ? s_g[w] : g;
+        if (g != KEY_NONE) min_store(best, groot[g0], g);
+      }
+      __syncthreads();
     }
     const uint32_t seg = v % KSEG;
     block_append<BLK, IPT>(keep, slot, out_cnt + seg * CSTR,
@@ -794,7 +818,7 @@ __global__ __launch_bounds__(BLK) void k_giant_emit_groups(
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
       if (keep[j]) {
-        const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+        const uint64_t x = x0 + (uint64_t)j * BLK + threadIdx.x;
         oa[slot[j]] = (uint32_t)x;
         ob[slot[j]] = G[j];
         ok[slot[j]] = key[j];
