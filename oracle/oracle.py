@@ -41,6 +41,9 @@ def lib():
         L.or_boruvka.argtypes = [c.c_uint32, c.c_uint64, _u32p, _u32p, _u32p, c.c_int,
                                  _u32p, c.c_void_p, c.c_void_p, c.c_void_p, c.c_int,
                                  c.POINTER(c.c_int)]
+        L.or_boruvka_rounds.restype = c.c_int64
+        L.or_boruvka_rounds.argtypes = [c.c_uint32, c.c_uint64, _u32p, _u32p, _u32p,
+                                        c.c_int, c.c_int, _u32p, _u32p]
         L.or_uf_labels.restype = None
         L.or_uf_labels.argtypes = [c.c_uint32, c.c_uint64, _u32p, _u32p, _u32p, c.c_void_p]
         L.or_bfs_labels.restype = None
@@ -95,6 +98,18 @@ def boruvka(n, u, v, w, wmode=1, max_rounds=200):
         raise RuntimeError("or_boruvka: round cap hit or out of memory")
     return dict(ids=ids[:c].copy(), labels=lab, log_m=lm[:r.value].copy(),
                 log_c=lc[:r.value].copy(), rounds=r.value)
+
+
+def boruvka_rounds(n, u, v, w, wmode, rounds):
+    """State of the canonical Boruvka loop after `rounds` rounds: (forest ids
+    chosen so far, ascending; min-id labels of the components so far)."""
+    u, v, w = _arr(u), _arr(v), _arr(w)
+    ids = np.empty(max(n, 1), np.uint32)
+    lab = np.empty(max(n, 1), np.uint32)
+    c = lib().or_boruvka_rounds(n, len(u), u, v, w, wmode, rounds, ids, lab)
+    if c < 0:
+        raise MemoryError("or_boruvka_rounds failed")
+    return ids[:c].copy(), lab[:n].copy()
 
 
 def uf_labels(n, u, v, forest=False):

@@ -148,10 +148,13 @@ int64_t or_kruskal(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
 
 /* ------------------------------------------------------------------------ */
 
-int64_t or_boruvka(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
-                   const uint32_t *w, int wmode, uint32_t *out_ids,
-                   uint32_t *out_labels, uint64_t *log_m, uint64_t *log_c,
-                   int max_rounds, int *rounds_out) {
+/* partial: after max_rounds rounds, return the forest edges chosen so far and
+ * the components so far instead of failing. */
+static int64_t boruvka_impl(uint32_t n, uint64_t m, const uint32_t *u,
+                            const uint32_t *v, const uint32_t *w, int wmode,
+                            uint32_t *out_ids, uint32_t *out_labels,
+                            uint64_t *log_m, uint64_t *log_c, int max_rounds,
+                            int *rounds_out, int partial) {
   /* alive edge list: current component endpoints + original id */
   uint32_t *cu = (uint32_t *)malloc(sizeof(uint32_t) * (m ? m : 1));
   uint32_t *cv = (uint32_t *)malloc(sizeof(uint32_t) * (m ? m : 1));
@@ -176,7 +179,7 @@ int64_t or_boruvka(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
   int64_t result = 0;
   while (alive > 0) {
     if (r >= max_rounds) {
-      result = -1;
+      result = partial ? 0 : -1;
       break;
     }
     /* reset_outgoing_edge_array_to_sentinel + populate_min_weight/eid */
@@ -245,6 +248,24 @@ int64_t or_boruvka(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
   free(best);
   free(flag);
   return result;
+}
+
+int64_t or_boruvka(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
+                   const uint32_t *w, int wmode, uint32_t *out_ids,
+                   uint32_t *out_labels, uint64_t *log_m, uint64_t *log_c,
+                   int max_rounds, int *rounds_out) {
+  return boruvka_impl(n, m, u, v, w, wmode, out_ids, out_labels, log_m, log_c,
+                      max_rounds, rounds_out, 0);
+}
+
+/* The state of the same round loop after `rounds` rounds (fewer if it ends
+ * earlier): the forest edges chosen so far (ascending ids, returned count)
+ * and min-id labels of the components so far. */
+int64_t or_boruvka_rounds(uint32_t n, uint64_t m, const uint32_t *u,
+                          const uint32_t *v, const uint32_t *w, int wmode,
+                          int rounds, uint32_t *out_ids, uint32_t *out_labels) {
+  return boruvka_impl(n, m, u, v, w, wmode, out_ids, out_labels, NULL, NULL,
+                      rounds, NULL, 1);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -52,6 +52,11 @@ int64_t or_boruvka(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
                    const uint32_t *w, int wmode, uint32_t *out_ids,
                    uint32_t *out_labels, uint64_t *log_m, uint64_t *log_c,
                    int max_rounds, int *rounds_out);
+/* State of or_boruvka's round loop after `rounds` rounds: forest edges chosen
+ * so far (count returned) and min-id labels of the components so far. */
+int64_t or_boruvka_rounds(uint32_t n, uint64_t m, const uint32_t *u,
+                          const uint32_t *v, const uint32_t *w, int wmode,
+                          int rounds, uint32_t *out_ids, uint32_t *out_labels);
 
 /* Union-find labels with the reference hook rule (larger root -> smaller root,
  * CC/host_spanning_tree.cu:10-42), processed sequentially in edge order.

@@ -3,6 +3,8 @@ captured into a HIP graph with a fixed number of Boruvka rounds per phase
 that end on the device.  Every replay, with eager library work and host
 synchronisation between replays, must give the forest, labels, weight and
 round log of the eager phem_mst_solve and of the oracle."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -138,3 +140,26 @@ def test_graph_maximum_forest(P, name, lf):
             assert st[k] == eager[k], (rep, k)
         torch.cuda.synchronize()
     g.close()
+
+
+@pytest.mark.parametrize("name", ["rmat_small", "grid_small", "path_chords_small",
+                                  "many_components", "u32_weights", "equal_weight_path"])
+def test_round_by_round_state_equals_oracle(P, name):
+    """The canonical schedule stopped after r rounds (round 0 plus r - 1
+    recorded rounds): the forest edges hooked so far and the min-id labels of
+    the components so far equal the oracle's round loop stopped after r
+    rounds, for every r; only the last r reports convergence."""
+    n, u, v, w, wmode = cases.make(name)
+    du, dv, dw = dev(u), dev(v), dev(w)
+    full = O.boruvka(n, u, v, w, wmode)
+    print(f"case {name}: {full['rounds']} rounds", flush=True)
+    for r in range(2, full["rounds"] + 1):
+        g = P.MstGraph(n, du, dv, dw, wmode, rounds=r - 1, light_factor=0.0)
+        g.launch()
+        st = P.MstStats()
+        rc = P.lib().phem_mst_graph_stats(g.h, ctypes.byref(st), P._stream())
+        assert rc == (0 if r == full["rounds"] else P.ENOCONV), r
+        o_ids, o_lab = O.boruvka_rounds(n, u, v, w, wmode, r)
+        assert np.array_equal(host(g.ids[: st.mst_count]), o_ids), r
+        assert np.array_equal(host(g.labels[:n]), o_lab), r
+        g.close()

@@ -194,6 +194,36 @@ def test_boruvka_round_log_properties():
     assert b["rounds"] <= int(np.log2(n)) + 1
 
 
+@pytest.mark.parametrize("name", ["rmat_small", "grid_small", "path_chords_small",
+                                  "many_components", "u32_weights", "duplicates_equal_w"])
+def test_boruvka_partial_state(name):
+    """The round loop stopped after r rounds: round 1 alone gives every
+    vertex's minimum-key incident edge (the closed form of one Boruvka round
+    from singletons); each partial forest lies inside the next and inside the
+    full one; the partial labels are the components of the partial forest;
+    enough rounds give the full answer."""
+    n, u, v, w, wmode = cases.make(name)
+    full = O.boruvka(n, u, v, w, wmode)
+    key = (O.wkey(w, wmode).astype(np.uint64) << np.uint64(32)) | \
+        np.arange(len(u), dtype=np.uint64)
+    valid = (u < n) & (v < n) & (u != v)
+    best = np.full(n, np.iinfo(np.uint64).max, np.uint64)
+    np.minimum.at(best, u[valid], key[valid])
+    np.minimum.at(best, v[valid], key[valid])
+    first = np.unique((best[best != np.iinfo(np.uint64).max] &
+                       np.uint64(0xFFFFFFFF)).astype(np.uint32))
+    prev = np.zeros(0, np.uint32)
+    for r in range(1, full["rounds"] + 2):
+        ids, lab = O.boruvka_rounds(n, u, v, w, wmode, r)
+        if r == 1:
+            assert np.array_equal(ids, first)
+        assert np.isin(prev, ids).all() and np.isin(ids, full["ids"]).all()
+        assert np.array_equal(lab, O.bfs_labels(n, u[ids], v[ids]))
+        prev = ids
+    assert np.array_equal(prev, full["ids"])
+    assert np.array_equal(lab, full["labels"])
+
+
 def test_weight_sum_exact():
     ids = np.array([0, 1, 2], np.uint32)
     w = np.array([0.5, 2.0 ** -24, 3.0], np.float32).view(np.uint32)
