@@ -1,4 +1,4 @@
-// phem_common.hip -- error reporting, version and device check.
+// phem_common.hip -- tuning knobs, profiler, version and device check.
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
@@ -14,16 +14,6 @@
 #include "phem_internal.h"
 
 namespace phem {
-
-static thread_local char g_err[512] = "";
-
-int set_err(int code, const char *fmt, ...) {
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(g_err, sizeof(g_err), fmt, ap);
-  va_end(ap);
-  return code;
-}
 
 // Process-wide tunables: built-in defaults, overridden by the environment at
 // first use, then by phem_set_tuning.
@@ -188,8 +178,6 @@ extern "C" int phem_prof_get(int i, char *name, int name_cap,
 }
 
 extern "C" const char *phem_version(void) { return "phem-mi355x 0.1 (gfx950)"; }
-
-extern "C" const char *phem_last_error(void) { return phem::g_err; }
 
 extern "C" int phem_device_check(int *device_count) {
   int n = 0;

@@ -120,7 +120,7 @@ extern "C" int phem_cpu_msf(uint32_t n, uint64_t m, const uint32_t *u,
     chosen.push_back(i);
   }
   std::sort(chosen.begin(), chosen.end());
-  std::memcpy(ids_out, chosen.data(), 4 * chosen.size());
+  if (!chosen.empty()) std::memcpy(ids_out, chosen.data(), 4 * chosen.size());
   *count = chosen.size();
   return PHEM_OK;
 }
