@@ -87,12 +87,16 @@ class ProfScope {
 
 #define PHEM_PROF(name, s) phem::ProfScope phem_prof_scope_(name, s)
 
+// A failed call also leaves its error as the thread's last HIP error; it is
+// cleared here so that the next launch check does not report it again.
 #define PHEM_HIP(x)                                                     \
   do {                                                                     \
     hipError_t e_ = (x);                                                   \
-    if (e_ != hipSuccess)                                                  \
+    if (e_ != hipSuccess) {                                                \
+      (void)hipGetLastError();                                             \
       return phem::set_err(PHEM_EHIP, "%s failed: %s (%s:%d)", #x,         \
                            hipGetErrorString(e_), __FILE__, __LINE__);     \
+    }                                                                      \
   } while (0)
 
 #define PHEM_LAUNCHED(name)                                                \

@@ -1699,6 +1699,7 @@ extern "C" int phem_mst_graph_create(uint32_t n, uint64_t m, const uint32_t *d_u
   const size_t sums = sizeof(ull) * (FS_HEAD + 2 * (PHEM_MAX_ROUNDS + 1));
   if (hipMalloc((void **)&g->d_sums, sums) != hipSuccess ||
       hipHostMalloc((void **)&g->h_sums, sums) != hipSuccess) {
+    (void)hipGetLastError();
     mst_graph_free(g);
     return set_err(PHEM_ENOMEM, "graph counter buffers");
   }

@@ -165,6 +165,7 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
   if (rc == PHEM_OK && mh && !S.hin) {
     hipError_t e = hipHostMalloc((void **)&S.hin, 12 * o.max_batch_edges);
     if (e != hipSuccess) {
+      (void)hipGetLastError();
       S.hin = nullptr;
       rc = set_err(PHEM_ENOMEM, "pinned input staging: %s", hipGetErrorString(e));
     }
@@ -354,6 +355,7 @@ extern "C" int phem_server_create(const phem_server_opts *opts,
                     2 * a256(4 * o.max_batch_edges);
     if (e == hipSuccess) e = hipHostMalloc((void **)&sl.hout, sl.hout_bytes);
     if (e != hipSuccess) {
+      (void)hipGetLastError();  // not left for the next launch check
       int rc = set_err(PHEM_ENOMEM, "slot allocation (%llu device bytes): %s",
                        (unsigned long long)S->slot_bytes, hipGetErrorString(e));
       phem_server_destroy(S.release());

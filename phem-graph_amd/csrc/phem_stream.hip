@@ -138,6 +138,7 @@ extern "C" int phem_stream_create(uint32_t n, uint64_t max_batch, int wmode,
        hipMalloc(&s->cc_forest, 8ull * n) == hipSuccess &&
        hipMalloc(&s->packed, 8 * max_batch) == hipSuccess;
   if (!ok) {
+    (void)hipGetLastError();  // not left for the next launch check
     stream_free(s);
     return set_err(PHEM_ENOMEM, "device allocation for the stream failed");
   }

@@ -124,6 +124,22 @@ def test_oversized_request_refused(P):
     srv.close()
 
 
+def test_failed_allocation_is_reported_and_forgotten(P):
+    """A server whose slot memory cannot be allocated (256 TiB) is refused
+    with PHEM_ENOMEM, and the failed allocation does not leak into the next
+    call's launch checks: a solve right after it succeeds."""
+    with pytest.raises(P.PhemError) as e:
+        P.Server(mem_bytes=1 << 48, slots=1)
+    assert e.value.code == P.ENOMEM
+    n, u, v, w = host_graph(2)
+    ids, lab, st = P.mst_solve(n, torch.from_numpy(u.view(np.int32)).cuda(),
+                               torch.from_numpy(v.view(np.int32)).cuda(),
+                               torch.from_numpy(w.view(np.int32)).cuda(), P.W_F32)
+    torch.cuda.synchronize()
+    k_ids, _ = O.kruskal(n, u, v, w, 1)
+    assert np.array_equal(ids.cpu().numpy().view(np.uint32), k_ids)
+
+
 def test_admission_by_footprint_splits_batches(P):
     """With room for only a few graphs per slot, a burst is served in many
     batches, each within the slot's memory, all correct."""
