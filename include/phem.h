@@ -282,6 +282,22 @@ int phem_stream_cc_forest(const phem_stream_t *s, const uint64_t **d_forest);
  * SpanningTree fills, CC/ExternalSpanningTree.cu:46-59). */
 int phem_stream_copy_cc_forest(const phem_stream_t *s, uint64_t *d_out,
                                void *stream);
+/* Host snapshot of a stream, for checkpoint and resume.  No counterpart in
+ * the reference, which keeps no state between runs (its only persistence is
+ * the weighted graph dump of new_boruvka, MST/src/new_boruvka.cu:399-410).
+ * phem_stream_save writes the forest, the union-find state, the insertion
+ * count and the last statistics into `host` (any host memory of at least
+ * phem_stream_snapshot_bytes bytes; the size grows with the forest) and
+ * returns when the copy is complete.  phem_stream_restore creates a stream
+ * on the current device with that state and room for batches of max_batch
+ * edges; later batches then give exactly what the saved stream would have
+ * given.  A buffer that is not a snapshot, or is shorter than its header
+ * says, returns PHEM_EINVAL. */
+int phem_stream_snapshot_bytes(const phem_stream_t *s, uint64_t *bytes);
+int phem_stream_save(const phem_stream_t *s, void *host, uint64_t bytes,
+                     void *stream);
+int phem_stream_restore(const void *host, uint64_t bytes, uint64_t max_batch,
+                        phem_stream_t **out);
 
 /* ---------------------------------------------------------------------------
  * Batched solve: k independent graphs in one pass (one disjoint union).

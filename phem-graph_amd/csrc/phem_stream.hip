@@ -272,3 +272,133 @@ extern "C" int phem_stream_copy_forest(const phem_stream_t *s, uint32_t *d_u,
   }
   return PHEM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Host snapshot of a stream: a header, then the forest (u, v, w as 32-bit
+// words, global ids as 64-bit words, f entries each), the union-find parent
+// array (n words) and its forest (n 64-bit records).
+
+namespace phem {
+
+struct StreamSnapHeader {
+  char magic[8];
+  uint32_t n;
+  int32_t wmode;
+  uint64_t f, next_gid, batches;
+  phem_mst_stats last;
+};
+constexpr char SNAP_MAGIC[8] = {'P', 'H', 'E', 'M', 'S', 'T', 'R', '1'};
+
+static uint64_t snap_bytes(uint64_t n, uint64_t f) {
+  return sizeof(StreamSnapHeader) + 20ull * f + 12ull * n;
+}
+
+}  // namespace phem
+
+extern "C" int phem_stream_snapshot_bytes(const phem_stream_t *s,
+                                          uint64_t *bytes) {
+  if (!s || !bytes) return set_err(PHEM_EINVAL, "stream handle or bytes is NULL");
+  *bytes = snap_bytes(s->n, s->f);
+  return PHEM_OK;
+}
+
+extern "C" int phem_stream_save(const phem_stream_t *s, void *host,
+                                uint64_t bytes, void *stream) {
+  if (!s || !host) return set_err(PHEM_EINVAL, "stream handle or buffer is NULL");
+  const uint64_t need = snap_bytes(s->n, s->f);
+  if (bytes < need)
+    return set_err(PHEM_EINVAL, "snapshot buffer of %llu B < %llu B",
+                   (unsigned long long)bytes, (unsigned long long)need);
+  StreamSnapHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, SNAP_MAGIC, 8);
+  h.n = s->n;
+  h.wmode = s->wmode;
+  h.f = s->f;
+  h.next_gid = s->next_gid;
+  h.batches = s->batches;
+  h.last = s->last;
+  char *p = (char *)host;
+  std::memcpy(p, &h, sizeof(h));
+  p += sizeof(h);
+  hipStream_t st = (hipStream_t)stream;
+  const int c = s->cur;
+  const uint64_t f = s->f, n = s->n;
+  if (f) {
+    PHEM_HIP(hipMemcpyAsync(p, s->u[c], 4 * f, hipMemcpyDeviceToHost, st));
+    PHEM_HIP(hipMemcpyAsync(p + 4 * f, s->v[c], 4 * f, hipMemcpyDeviceToHost, st));
+    PHEM_HIP(hipMemcpyAsync(p + 8 * f, s->w[c], 4 * f, hipMemcpyDeviceToHost, st));
+    PHEM_HIP(hipMemcpyAsync(p + 12 * f, s->gid[c], 8 * f, hipMemcpyDeviceToHost, st));
+  }
+  p += 20 * f;
+  PHEM_HIP(hipMemcpyAsync(p, s->parent, 4 * n, hipMemcpyDeviceToHost, st));
+  PHEM_HIP(hipMemcpyAsync(p + 4 * n, s->cc_forest, 8 * n, hipMemcpyDeviceToHost, st));
+  PHEM_HIP(hipStreamSynchronize(st));
+  return PHEM_OK;
+}
+
+extern "C" int phem_stream_restore(const void *host, uint64_t bytes,
+                                   uint64_t max_batch, phem_stream_t **out) {
+  if (!out) return set_err(PHEM_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!host || bytes < sizeof(StreamSnapHeader))
+    return set_err(PHEM_EINVAL, "snapshot is NULL or shorter than its header");
+  StreamSnapHeader h;
+  std::memcpy(&h, host, sizeof(h));
+  if (std::memcmp(h.magic, SNAP_MAGIC, 8) != 0)
+    return set_err(PHEM_EINVAL, "not a stream snapshot");
+  if (h.n < 1 || h.f >= h.n || bytes < snap_bytes(h.n, h.f))
+    return set_err(PHEM_EINVAL, "snapshot of n=%u, %llu forest edges is inconsistent "
+                   "with its %llu bytes", h.n, (unsigned long long)h.f,
+                   (unsigned long long)bytes);
+  // the device kernels index with these values: check them before upload
+  const char *p = (const char *)host + sizeof(h);
+  const uint64_t f = h.f, n = h.n;
+  {
+    const uint32_t *su = (const uint32_t *)p, *sv = su + f;
+    const char *gp = p + 12 * f;
+    for (uint64_t i = 0; i < f; i++) {
+      uint64_t g;
+      std::memcpy(&g, gp + 8 * i, 8);
+      if (su[i] >= n || sv[i] >= n || su[i] == sv[i] || g >= h.next_gid)
+        return set_err(PHEM_EINVAL, "snapshot forest edge %llu is invalid",
+                       (unsigned long long)i);
+    }
+    const char *pp = p + 20 * f, *fp = pp + 4 * n;
+    for (uint64_t x = 0; x < n; x++) {
+      uint32_t par;
+      uint64_t rec;
+      std::memcpy(&par, pp + 4 * x, 4);
+      std::memcpy(&rec, fp + 8 * x, 8);
+      const uint64_t a = rec >> 32, b = rec & 0xFFFFFFFFull;
+      if (par > x || (rec != ~0ull && !(a < b && b < n)))
+        return set_err(PHEM_EINVAL, "snapshot union-find entry %llu is invalid",
+                       (unsigned long long)x);
+    }
+  }
+  phem_stream_t *s = nullptr;
+  int rc = phem_stream_create(h.n, max_batch, h.wmode, &s);
+  if (rc) return rc;
+  hipError_t e = hipSuccess;
+  if (f) {
+    e = hipMemcpy(s->u[0], p, 4 * f, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->v[0], p + 4 * f, 4 * f, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->w[0], p + 8 * f, 4 * f, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->gid[0], p + 12 * f, 8 * f, hipMemcpyHostToDevice);
+  }
+  p += 20 * f;
+  if (e == hipSuccess) e = hipMemcpy(s->parent, p, 4 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s->cc_forest, p + 4 * n, 8 * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    phem_stream_destroy(s);
+    return set_err(PHEM_EHIP, "snapshot upload: %s", hipGetErrorString(e));
+  }
+  s->cur = 0;
+  s->f = f;
+  s->next_gid = h.next_gid;
+  s->batches = h.batches;
+  s->last = h.last;
+  *out = s;
+  return PHEM_OK;
+}

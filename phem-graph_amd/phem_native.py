@@ -157,6 +157,9 @@ def lib():
         "phem_stream_labels": (i32, [vp, vp, vp]),
         "phem_stream_cc_forest": (i32, [vp, c.POINTER(vp)]),
         "phem_stream_copy_cc_forest": (i32, [vp, vp, vp]),
+        "phem_stream_snapshot_bytes": (i32, [vp, c.POINTER(c.c_uint64)]),
+        "phem_stream_save": (i32, [vp, vp, u64, vp]),
+        "phem_stream_restore": (i32, [vp, u64, u64, c.POINTER(vp)]),
         "phem_batch_workspace_bytes": (i32, [u64, u64, u32, c.POINTER(c.c_uint64)]),
         "phem_batch_solve": (i32, [u32, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64),
                                    vp, vp, vp, i32, vp, u64, vp, vp, vp, vp, vp,
@@ -526,6 +529,33 @@ class Stream:
         lab = torch.empty(max(self.n, 1), dtype=torch.int32, device=self.device)
         _check(lib().phem_stream_labels(self.h, _p(lab), _stream()), "phem_stream_labels")
         return lab[: self.n]
+
+    def save(self, stream=None):
+        """Host snapshot (numpy uint8 array) of the stream's state."""
+        import numpy as np
+        b = ctypes.c_uint64(0)
+        _check(lib().phem_stream_snapshot_bytes(self.h, ctypes.byref(b)),
+               "phem_stream_snapshot_bytes")
+        buf = np.zeros(b.value, np.uint8)
+        _check(lib().phem_stream_save(self.h, buf.ctypes.data, b.value, _stream(stream)),
+               "phem_stream_save")
+        return buf
+
+    @classmethod
+    def restore(cls, snapshot, max_batch, device="cuda"):
+        """A stream resumed from a save() snapshot (numpy uint8 array)."""
+        import numpy as np
+        snap = np.ascontiguousarray(snapshot, np.uint8)
+        h = ctypes.c_void_p()
+        _check(lib().phem_stream_restore(snap.ctypes.data if snap.size else None,
+                                         snap.size, max_batch, ctypes.byref(h)),
+               "phem_stream_restore")
+        self = cls.__new__(cls)
+        self.device = torch.device(device)
+        self.h = h
+        n = ctypes.c_uint32.from_buffer_copy(snap[8:12].tobytes()).value
+        self.n = n
+        return self
 
     def cc_forest(self):
         """Copy of the union-find forest: n packed (u << 32 | v) slots, all

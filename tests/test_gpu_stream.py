@@ -139,6 +139,81 @@ def test_stream_edge_cases(P):
     st.close()
 
 
+def stream_state(st):
+    fu, fv, fw, fg = st.forest()
+    torch.cuda.synchronize()
+    s = st.stats()
+    return (host32(fu).copy(), host32(fv).copy(), host32(fw).copy(),
+            fg.cpu().numpy().copy(), host32(st.labels()).copy(),
+            st.cc_forest().cpu().numpy().copy(),
+            {k: s[k] for k in ("mst_count", "weight_fx", "edges_inserted", "batches")})
+
+
+def same_state(a, b, cc_forest=True):
+    """Equal forests, labels and statistics; the union-find forests too when
+    cc_forest (the edge that hooks a root depends on thread timing, so after
+    further batches only their invariants are comparable)."""
+    k = 6 if cc_forest else 5
+    return all(np.array_equal(x, y) for x, y in zip(a[:k], b[:k])) and a[6] == b[6]
+
+
+def test_stream_snapshot_resume(P):
+    """Checkpoint and resume: a stream restored from a host snapshot (with
+    the original destroyed) and the original continued in parallel reach the
+    same forest, labels, union-find forest and statistics after further
+    batches, and both equal the oracle."""
+    n = 1 << 12
+    batches = [S.rmat(12, 1 << 14, 4, positive=True)]
+    for b in range(8):
+        batches.append(S.uniform(n, 1 << 10, S.stream_seed(6, b)))
+    st = P.Stream(n, 1 << 14)
+    for u, v, w in batches[:4]:
+        st.apply(dev(u), dev(v), dev(w))
+    snap = st.save()
+    st2 = P.Stream.restore(snap, 1 << 14)
+    assert same_state(stream_state(st), stream_state(st2))
+    for u, v, w in batches[4:6]:
+        st.apply(dev(u), dev(v), dev(w))
+        st2.apply(dev(u), dev(v), dev(w))
+    assert same_state(stream_state(st), stream_state(st2), cc_forest=False)
+    snap2 = st2.save()
+    st2.close()
+    st3 = P.Stream.restore(snap2, 1 << 10)  # smaller batches from here on
+    for u, v, w in batches[6:]:
+        st.apply(dev(u), dev(v), dev(w))
+        st3.apply(dev(u), dev(v), dev(w))
+    a, b = stream_state(st), stream_state(st3)
+    assert same_state(a, b, cc_forest=False)
+    au = np.concatenate([x[0] for x in batches])
+    av = np.concatenate([x[1] for x in batches])
+    aw = np.concatenate([x[2] for x in batches])
+    k_ids, k_lab = O.kruskal(n, au, av, aw, 1)
+    assert np.array_equal(b[3].astype(np.uint64), k_ids.astype(np.uint64))
+    assert np.array_equal(b[4], k_lab)
+    check_cc_forest(st3, n, au, av, k_lab)
+    st.close()
+    st3.close()
+
+
+def test_stream_restore_rejects_bad_snapshots(P):
+    st = P.Stream(1000, 100)
+    u, v, w = S.uniform(1000, 100, 3)
+    st.apply(dev(u), dev(v), dev(w))
+    snap = st.save()
+    st.close()
+    bad = snap.copy()
+    bad[:8] = 0  # not a snapshot
+    truncated = snap[: len(snap) - 1]
+    corrupt = snap.copy()
+    # the union-find forest record of vertex n - 2 (the second-last 8 bytes):
+    # u = 5 > v = 2 is no record the library writes
+    corrupt[-16:-8] = np.array([(5 << 32) | 2], np.uint64).view(np.uint8)
+    for b in (bad, truncated, corrupt, snap[:10]):
+        with pytest.raises(P.PhemError) as e:
+            P.Stream.restore(b, 100)
+        assert e.value.code == P.EINVAL
+
+
 def test_stream_rejects_oversized_batch(P):
     st = P.Stream(100, 10)
     u = dev(np.zeros(11, np.uint32))
