@@ -108,9 +108,11 @@ typedef struct phem_mst_stats {
   uint64_t valid_edges;   /* input edges that are part of the graph */
   uint64_t tau;           /* light/heavy key threshold (all ones: no split) */
   uint64_t light_edges;   /* edges of the light phase */
-  uint64_t heavy_edges;   /* heavy edges left after the filter */
+  uint64_t heavy_edges;   /* alive edges entering the heavy rounds */
   uint64_t giant_edges;   /* of those: one per component adjacent to the giant
-                             component (its lightest such edge) */
+                             component (its lightest such edge), for the
+                             components not hooked to the giant at once
+                             because a lighter edge or a smaller id decides */
   uint64_t m_log[PHEM_MAX_ROUNDS]; /* alive edges at the start of round r */
   uint64_t c_log[PHEM_MAX_ROUNDS]; /* components with >= 1 alive edge in round r */
 } phem_mst_stats;

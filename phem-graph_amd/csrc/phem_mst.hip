@@ -649,31 +649,107 @@ __global__ __launch_bounds__(BLK) void k_giant_fold(
   }
 }
 
-// After the heavy pass: every component x with an edge to the giant
-// component G becomes one alive edge (x, G, gmin[x]) offered to best[] of
-// both sides.  Tile v appends to segment v mod KSEG of the alive-edge list.
+// After the heavy pass, gmin[x] holds the lightest edge between component x
+// and the giant component G.  When nothing lighter reached best[x] in the
+// heavy pass, that edge is x's Boruvka edge for the first heavy round; if
+// also x > G, x is hooked under G here (union-find, forest bit, exact
+// weight) and leaves the round (best[x] cleared; G stays the root, so no
+// component list changes).  Otherwise the edge becomes one alive edge
+// (x, G, key), offered to best[x].  Every key is offered to best[G] (one
+// block minimum).  Tile v appends to segment v mod KSEG of the alive-edge
+// list.  A hook here never meets a hook of the same edge by G in the round's
+// k_hook: that one finds the forest bit set and adds no weight.  The hooked
+// x are in no component list, so the round flattens the whole parent array
+// before its compaction relabels edges through it.
+struct GiantHook {
+  uint32_t *par, *bitmap;
+  int ord;
+  ull *wsum_parts, *inexact_parts;
+};
+
+// Hooks root x > G under G with the edge of key k; adds its weight to wsum /
+// wbad.  Within the emit kernel only x's own thread writes par[x], and G
+// stays a root (every hook there puts a larger root under G), so a plain
+// store is the union.
+__device__ __forceinline__ void giant_hook(const GiantHook &gh, ull *best,
+                                           uint32_t x, uint32_t G, ull k,
+                                           ull &wsum, ull &wbad) {
+  const uint32_t e = key_id(k, gh.ord);
+  atomicOr(&gh.bitmap[e >> 5], 1u << (e & 31));
+  ull fx;
+  if (weight_fx(key_weight(k, gh.ord), gh.ord & ORD_F32, &fx)) wsum += fx;
+  else wbad++;
+  best[x] = KEY_NONE;
+  par_store(&gh.par[x], G);
+}
+
+// Block sums of the edges appended to the alive list, and of the hooked
+// edges' weights and inexact weights, into their KSEG-segment counters
+// (every thread of the block calls it).
+__device__ __forceinline__ void giant_emit_flush(ull emitted, ull wsum,
+                                                 ull wbad, ull *parts,
+                                                 const GiantHook &gh) {
+  __shared__ ull s_w[3][BLK / WAVE];
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    emitted += __shfl_xor(emitted, d);
+    wsum += __shfl_xor(wsum, d);
+    wbad += __shfl_xor(wbad, d);
+  }
+  if (lane_id() == 0) {
+    s_w[0][threadIdx.x / WAVE] = emitted;
+    s_w[1][threadIdx.x / WAVE] = wsum;
+    s_w[2][threadIdx.x / WAVE] = wbad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ull t[3] = {0, 0, 0};
+    for (int w = 0; w < BLK / WAVE; w++)
+      for (int q = 0; q < 3; q++) t[q] += s_w[q][w];
+    const uint32_t seg = blockIdx.x % KSEG;
+    if (t[0]) atomicAdd(parts + seg * CSTR, t[0]);
+    if (t[1]) atomicAdd(gh.wsum_parts + seg * CSTR, t[1]);
+    if (t[2]) atomicAdd(gh.inexact_parts + seg * CSTR, t[2]);
+  }
+}
+
 __global__ __launch_bounds__(BLK) void k_giant_emit(
     uint32_t n, ull *__restrict__ gmin, const uint32_t *giant_root,
     uint32_t *__restrict__ oa, uint32_t *__restrict__ ob,
     ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *best,
-    ull *parts) {
+    ull *parts, GiantHook gh) {
   const uint32_t G = *giant_root;
   const uint32_t ntiles = (uint32_t)ceil_div_dev(n, LTILE);
-  uint32_t emitted = 0;  // by this thread
+  ull emitted = 0, wsum = 0, wbad = 0;  // by this thread
   ull gbest = KEY_NONE;  // lightest emitted key seen by this thread (for G)
   for (uint32_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
-    bool keep[IPT];
-    ull key[IPT];
+    bool keep[IPT], on[IPT];
+    ull key[IPT], bx[IPT];
     uint64_t slot[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
       const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
       key[j] = x < n ? gmin[x] : KEY_NONE;
-      keep[j] = key[j] != KEY_NONE;
-      if (keep[j]) {
-        emitted++;
-        min_store(best, (uint32_t)x, key[j]);
+      on[j] = key[j] != KEY_NONE;
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+      bx[j] = best[on[j] ? x : 0u];  // unconditional read
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      const uint64_t x = (uint64_t)v * LTILE + (uint64_t)j * BLK + threadIdx.x;
+      const bool lightest = key[j] < bx[j];
+      keep[j] = on[j] && !(lightest && x > G);
+      if (on[j]) {
         gbest = key[j] < gbest ? key[j] : gbest;
+        if (!keep[j]) {
+          giant_hook(gh, best, (uint32_t)x, G, key[j], wsum, wbad);
+        } else {
+          emitted++;
+          if (lightest) atomicMin(&best[x], key[j]);
+        }
       }
     }
     const uint32_t seg = v % KSEG;
@@ -689,29 +765,21 @@ __global__ __launch_bounds__(BLK) void k_giant_emit(
       }
     }
   }
-  // one count and one giant-side offer per block
-  __shared__ ull s_w[BLK / WAVE], s_g[BLK / WAVE];
-  ull x = emitted;
+  // one giant-side offer per block
+  __shared__ ull s_g[BLK / WAVE];
 #pragma unroll
   for (int d = 1; d < WAVE; d <<= 1) {
-    x += __shfl_xor(x, d);
     const ull o = __shfl_xor(gbest, d);
     gbest = o < gbest ? o : gbest;
   }
-  if (lane_id() == 0) {
-    s_w[threadIdx.x / WAVE] = x;
-    s_g[threadIdx.x / WAVE] = gbest;
-  }
+  if (lane_id() == 0) s_g[threadIdx.x / WAVE] = gbest;
   __syncthreads();
   if (threadIdx.x == 0) {
-    ull tot = 0, g = KEY_NONE;
-    for (int w = 0; w < BLK / WAVE; w++) {
-      tot += s_w[w];
-      g = s_g[w] < g ? s_g[w] : g;
-    }
-    if (tot) atomicAdd(parts + (blockIdx.x % KSEG) * CSTR, tot);
+    ull g = KEY_NONE;
+    for (int w = 0; w < BLK / WAVE; w++) g = s_g[w] < g ? s_g[w] : g;
     if (g != KEY_NONE) min_store(best, G, g);
   }
+  giant_emit_flush(emitted, wsum, wbad, parts, gh);
 }
 
 // Per graph g of a batch (vertices [voff[g], voff[g + 1])): the most frequent
@@ -756,8 +824,8 @@ __global__ __launch_bounds__(SPLIT_BLK) void k_giant_bits_groups(
 }
 
 // k_giant_emit for the graphs of a batch: component x's edge goes to the
-// giant root of x's own graph, and each emitted key is offered to best[] of
-// both ends.  A tile inside one graph offers the block's minimum to that
+// giant root of x's own graph (hooked at once, or kept as an alive edge, as
+// above).  A tile inside one graph offers the block's minimum to that
 // graph's giant once; a tile across graphs offers one key at a time
 // (consecutive vertices share a graph, so each pre-check sees the thread's
 // previous offer to that giant).
@@ -766,33 +834,44 @@ __global__ __launch_bounds__(BLK) void k_giant_emit_groups(
     const uint32_t *__restrict__ vmap, uint32_t k,
     const uint32_t *__restrict__ groot, uint32_t *__restrict__ oa,
     uint32_t *__restrict__ ob, ull *__restrict__ ok, ull *out_cnt,
-    uint64_t out_cap, ull *best, ull *parts) {
+    uint64_t out_cap, ull *best, ull *parts, GiantHook gh) {
   __shared__ ull s_g[BLK / WAVE];
   const uint32_t ntiles = (uint32_t)ceil_div_dev(n, LTILE);
-  uint32_t emitted = 0;  // by this thread
+  ull emitted = 0, wsum = 0, wbad = 0;  // by this thread
   for (uint32_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
     const uint64_t x0 = (uint64_t)v * LTILE;
     const uint64_t x1 = x0 + LTILE < n ? x0 + LTILE : n;
     const uint32_t g0 = graph_of(vmap, voff, k, x0);
     const bool one = g0 == graph_of(vmap, voff, k, x1 - 1);  // block-uniform
-    bool keep[IPT];
-    ull key[IPT];
+    bool keep[IPT], on[IPT];
+    ull key[IPT], bx[IPT];
     uint32_t G[IPT];
     uint64_t slot[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
       const uint64_t x = x0 + (uint64_t)j * BLK + threadIdx.x;
       key[j] = x < n ? gmin[x] : KEY_NONE;
-      keep[j] = key[j] != KEY_NONE;
-      G[j] = keep[j] ? groot[one ? g0 : graph_of(vmap, voff, k, x)] : 0u;
+      on[j] = key[j] != KEY_NONE;
+      G[j] = on[j] ? groot[one ? g0 : graph_of(vmap, voff, k, x)] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      const uint64_t x = x0 + (uint64_t)j * BLK + threadIdx.x;
+      bx[j] = best[on[j] ? x : 0u];  // unconditional read
     }
     ull gbest = KEY_NONE;  // this thread's lightest key for the tile's giant
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-      if (keep[j]) {
-        const uint64_t x = x0 + (uint64_t)j * BLK + threadIdx.x;
-        emitted++;
-        min_store(best, (uint32_t)x, key[j]);
+      const uint64_t x = x0 + (uint64_t)j * BLK + threadIdx.x;
+      const bool lightest = key[j] < bx[j];
+      keep[j] = on[j] && !(lightest && x > G[j]);
+      if (on[j]) {
+        if (!keep[j]) {
+          giant_hook(gh, best, (uint32_t)x, G[j], key[j], wsum, wbad);
+        } else {
+          emitted++;
+          if (lightest) atomicMin(&best[x], key[j]);
+        }
         if (one) gbest = key[j] < gbest ? key[j] : gbest;
         else min_store(best, G[j], key[j]);
       }
@@ -825,17 +904,7 @@ __global__ __launch_bounds__(BLK) void k_giant_emit_groups(
       }
     }
   }
-  __shared__ ull s_w[BLK / WAVE];
-  ull x = emitted;
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
-  if (lane_id() == 0) s_w[threadIdx.x / WAVE] = x;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ull tot = 0;
-    for (int w = 0; w < BLK / WAVE; w++) tot += s_w[w];
-    if (tot) atomicAdd(parts + (blockIdx.x % KSEG) * CSTR, tot);
-  }
+  giant_emit_flush(emitted, wsum, wbad, parts, gh);
 }
 
 // tau = key of rank ~light_edges in a strided sample of TSAMP input keys
@@ -1243,7 +1312,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   // needed between the producer and the first rounds.
   auto run_rounds = [&](uint64_t list_ub, bool first_all,
                         std::initializer_list<std::pair<int, int>> extra,
-                        ull *extra_out, int fixed) -> int {
+                        ull *extra_out, int fixed, bool flatten_first = false) -> int {
     bool all = first_all;
     bool first = true;
     while (m_r > 0) {
@@ -1289,6 +1358,13 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                              listL, ctr + cL(rr + 1), L.lcap);
         }
         PHEM_LAUNCHED("k_compress");
+        if (flatten_first && first && k == 0) {
+          // components hooked outside this round's lists (k_giant_emit)
+          PHEM_PROF("k_flatten", s);
+          hipLaunchKernelGGL(k_flatten, dim3(grid_for(n, SPLIT_BLK)),
+                             dim3(SPLIT_BLK), 0, s, n, par);
+        }
+        PHEM_LAUNCHED("k_flatten");
         {
           PHEM_PROF("k_compact", s);
           hipLaunchKernelGGL((k_pass<P_EDGES, Src>),
@@ -1500,6 +1576,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          n, par, gmin);
     }
     PHEM_LAUNCHED("k_giant_fold");
+    const GiantHook gh{par, bitmap, ord, ctr + cPart(1), ctr + cPart(2)};
     {
       PHEM_PROF("k_giant_emit", s);
       if (groups)
@@ -1507,12 +1584,14 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                            dim3(grid_for(ceil_div(n, LTILE), 1)), dim3(BLK), 0,
                            s, n, gmin, grp->voff, grp->vmap, grp->k,
                            (const uint32_t *)grp->groot, Ea[cur], Eb[cur],
-                           Ek[cur], ctr + cE(r), L.ecap, best, ctr + cPart(3));
+                           Ek[cur], ctr + cE(r), L.ecap, best, ctr + cPart(3),
+                           gh);
       else
         hipLaunchKernelGGL(k_giant_emit, dim3(grid_for(ceil_div(n, LTILE), 1)),
                            dim3(BLK), 0, s, n, gmin,
                            (const uint32_t *)(ctr + C_GIANT), Ea[cur], Eb[cur],
-                           Ek[cur], ctr + cE(r), L.ecap, best, ctr + cPart(3));
+                           Ek[cur], ctr + cE(r), L.ecap, best, ctr + cPart(3),
+                           gh);
     }
     PHEM_LAUNCHED("k_giant_emit");
     // the heavy survivors (at most m + n) are counted with the first heavy
@@ -1520,7 +1599,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     m_r = m + n;
     const int r_heavy = r;
     rc = run_rounds(0, true, {{cE(r_heavy), KSEG}, {cPart(3), KSEG}}, hv,
-                    fr ? fr->heavy : 0);
+                    fr ? fr->heavy : 0, true);
     if (rc) return rc;
     if (stats && !fr) {
       stats->heavy_edges = hv[0];
