@@ -29,10 +29,16 @@
 //              fixes a threshold tau; rounds run over the edges with key <
 //              tau; the parent forest is flattened and its giant component
 //              marked in a bitmap; one pass keeps the heavier edges whose
-//              endpoints are still in different components; rounds finish
-//              those.  It runs when the heavy edges are at least half as
-//              many as the light ones; a batch's graphs (disjoint union)
-//              each get their own threshold and giant component.
+//              endpoints are still in different components (an edge between
+//              a component and the giant only offers its key to that
+//              component's minimum: one edge per component comes back, and
+//              a component whose lightest edge it is gets hooked to the
+//              giant at once); rounds finish those.  It runs when the heavy
+//              edges are at least half as many as the light ones; a batch's
+//              graphs (disjoint union) each get their own threshold and
+//              giant component.
+// A whole solve can also be recorded into a HIP graph with a fixed number of
+// rounds per phase and no counter read (phem_mst_graph_*).
 // The packed key makes every component's choice unique, so both give the
 // unique minimum spanning forest under (weight, id) order; compaction need
 // not be stable because ids travel with the edges.
