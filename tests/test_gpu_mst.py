@@ -235,3 +235,59 @@ def test_maximum_spanning_forest(P, name, lf):
     if exact:
         assert st["weight_fx"] == ws
 
+
+
+def giant_above_neighbours(seed):
+    """A dense random graph on the upper half of the ids (it becomes the giant
+    component of the light phase, so its root G >= n/2) and small components
+    on the lower ids, joined inside by light edges and to the upper half only
+    by heavy edges: their components x < G reach the giant-adjacent minimum
+    path that is kept as an alive edge instead of hooked at once."""
+    rng = np.random.default_rng(seed)
+    n = 1 << 13
+    h = n // 2
+    mg = 16 * h
+    gu = rng.integers(h, n, mg)
+    gv = rng.integers(h, n, mg)
+    gw = rng.integers(1, 1 << 20, mg)              # light-ish weights
+    cu = rng.integers(0, h, 2 * h)                  # small components below h
+    cv = np.minimum(cu + rng.integers(1, 4, 2 * h), h - 1)
+    cw = rng.integers(1, 1 << 20, 2 * h)
+    au = rng.integers(0, h, 4 * h)                  # attachments: heavy only
+    av = rng.integers(h, n, 4 * h)
+    aw = rng.integers(1 << 23, 1 << 24, 4 * h)
+    u = np.concatenate([gu, cu, au]).astype(np.uint32)
+    v = np.concatenate([gv, cv, av]).astype(np.uint32)
+    w = (np.concatenate([gw, cw, aw]).astype(np.float32) / np.float32(1 << 24)).view(np.uint32)
+    perm = rng.permutation(len(u))  # edge ids in random order
+    return n, u[perm].copy(), v[perm].copy(), w[perm].copy()
+
+
+@pytest.mark.parametrize("lf", [None, 0.5, 2.0])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_split_giant_above_its_neighbours(P, seed, lf):
+    n, u, v, w = giant_above_neighbours(seed)
+    ids, lab, st = run_mst(P, n, u, v, w, 1, light_factor=lf)
+    k_ids, k_lab = O.kruskal(n, u, v, w, 1)
+    assert np.array_equal(ids, k_ids)
+    assert np.array_equal(lab, k_lab)
+    ws, exact = O.weight_sum(k_ids, w, 1)
+    assert exact and st["weight_fx"] == ws
+    if lf is None:
+        assert st["tau"] != 0xFFFFFFFFFFFFFFFF  # the split schedule ran
+        assert st["giant_edges"] > 0           # kept giant edges (x < G)
+
+
+def test_batch_giant_above_its_neighbours(P):
+    graphs = [giant_above_neighbours(s) for s in (3, 4, 5)]
+    voff, eoff = [0], [0]
+    for n, u, v, w in graphs:
+        voff.append(voff[-1] + n)
+        eoff.append(eoff[-1] + len(u))
+    cat = [torch.from_numpy(np.concatenate([g[i] for g in graphs]).view(np.int32)).cuda()
+           for i in (1, 2, 3)]
+    r = P.batch_solve(voff, eoff, *cat)
+    for g, (n, u, v, w) in enumerate(graphs):
+        k_ids, k_lab = O.kruskal(n, u, v, w, 1)
+        assert np.array_equal(r["ids"][g].numpy().view(np.uint32), k_ids), g
+        assert np.array_equal(r["labels"][g].numpy().view(np.uint32), k_lab), g
