@@ -293,8 +293,10 @@ int phem_stream_copy_cc_forest(const phem_stream_t *s, uint64_t *d_out,
  * returns when the copy is complete.  phem_stream_restore creates a stream
  * on the current device with that state and room for batches of max_batch
  * edges; later batches then give exactly what the saved stream would have
- * given.  A buffer that is not a snapshot, or is shorter than its header
- * says, returns PHEM_EINVAL. */
+ * given.  A buffer that is not a snapshot, is shorter than its header says,
+ * or holds a value the kernels could not index with, returns PHEM_EINVAL.
+ * The format is this library's in-memory layout (host byte order, the stats
+ * struct as compiled): restore with the same build that saved. */
 int phem_stream_snapshot_bytes(const phem_stream_t *s, uint64_t *bytes);
 int phem_stream_save(const phem_stream_t *s, void *host, uint64_t bytes,
                      void *stream);
