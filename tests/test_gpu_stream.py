@@ -195,6 +195,24 @@ def test_stream_snapshot_resume(P):
     st3.close()
 
 
+@pytest.mark.parametrize("n", [1, 5])
+def test_stream_snapshot_of_a_fresh_stream(P, n):
+    """A snapshot taken before any batch (empty forest) restores to a stream
+    that then behaves as a fresh one."""
+    st = P.Stream(n, 16)
+    st2 = P.Stream.restore(st.save(), 16)
+    u = np.array([0, n - 1, 0], np.uint32)
+    v = np.array([n - 1, 0, 0], np.uint32)
+    w = np.array([0.5, 0.25, 0.75], np.float32).view(np.uint32)
+    for s_ in (st, st2):
+        s_.apply(dev(u), dev(v), dev(w))
+    assert same_state(stream_state(st), stream_state(st2), cc_forest=False)
+    k_ids, k_lab = O.kruskal(n, u, v, w, 1)
+    assert np.array_equal(stream_state(st2)[3].astype(np.uint64), k_ids.astype(np.uint64))
+    st.close()
+    st2.close()
+
+
 def test_stream_restore_rejects_bad_snapshots(P):
     st = P.Stream(1000, 100)
     u, v, w = S.uniform(1000, 100, 3)
