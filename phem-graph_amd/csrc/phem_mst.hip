@@ -437,7 +437,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
     const ull *tau_ptr, const uint32_t *__restrict__ giant,
     uint32_t *__restrict__ oa, uint32_t *__restrict__ ob,
     ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *valid_parts,
-    ull *best, ull *gmin, const uint32_t *gsum, EdgeTaus et) {
+    ull *best, ull *gmin, const uint32_t *gsum, EdgeTaus et, int two_step) {
   __shared__ SegView sv;
   __shared__ uint32_t s_sum[MODE == P_HEAVY ? GSUM_HEAVY : 1];
   constexpr bool INPUT = MODE != P_EDGES;
@@ -574,6 +574,10 @@ __global__ __launch_bounds__(BLK) void k_pass(
         if (val[j]) {
           a[j] = par[a[j]];
           b[j] = par[b[j]];
+          if (two_step) {  // see run_rounds: one more step reaches the root
+            a[j] = par[a[j]];
+            b[j] = par[b[j]];
+          }
         }
       }
     } else if (MODE != P_LIGHT) {
@@ -665,8 +669,8 @@ __global__ __launch_bounds__(BLK) void k_giant_fold(
 // block minimum).  Tile v appends to segment v mod KSEG of the alive-edge
 // list.  A hook here never meets a hook of the same edge by G in the round's
 // k_hook: that one finds the forest bit set and adds no weight.  The hooked
-// x are in no component list, so the round flattens the whole parent array
-// before its compaction relabels edges through it.
+// x are in no component list, so the round's compaction relabels through
+// two parent steps (x -> G -> root).
 struct GiantHook {
   uint32_t *par, *bitmap;
   int ord;
@@ -1316,9 +1320,13 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   // first counter read then reports E(r) with the rounds' counters, together
   // with the `extra` values (written to extra_out), so no read of its own is
   // needed between the producer and the first rounds.
+  // giant_hooked: k_giant_emit hooked components outside the round's lists
+  // (parent G, or an ancestor of G after path halving).  The round's compress
+  // flattens every active component, G and the roots it hooks to included,
+  // so the phase's first compaction relabels in two steps.
   auto run_rounds = [&](uint64_t list_ub, bool first_all,
                         std::initializer_list<std::pair<int, int>> extra,
-                        ull *extra_out, int fixed, bool flatten_first = false) -> int {
+                        ull *extra_out, int fixed, bool giant_hooked = false) -> int {
     bool all = first_all;
     bool first = true;
     while (m_r > 0) {
@@ -1364,13 +1372,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                              listL, ctr + cL(rr + 1), L.lcap);
         }
         PHEM_LAUNCHED("k_compress");
-        if (flatten_first && first && k == 0) {
-          // components hooked outside this round's lists (k_giant_emit)
-          PHEM_PROF("k_flatten", s);
-          hipLaunchKernelGGL(k_flatten, dim3(grid_for(n, SPLIT_BLK)),
-                             dim3(SPLIT_BLK), 0, s, n, par);
-        }
-        PHEM_LAUNCHED("k_flatten");
+        const int two_step = giant_hooked && first && k == 0;
         {
           PHEM_PROF("k_compact", s);
           hipLaunchKernelGGL((k_pass<P_EDGES, Src>),
@@ -1379,7 +1381,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                              ctr + cE(rr), L.ecap, 0ull, n, ord, par, noull,
                              nou, Ea[c ^ 1], Eb[c ^ 1], Ek[c ^ 1],
                              ctr + cE(rr + 1), L.ecap, (ull *)nullptr, best,
-                             (ull *)nullptr, nou, EdgeTaus{});
+                             (ull *)nullptr, nou, EdgeTaus{}, two_step);
         }
         PHEM_LAUNCHED("k_compact");
       }
@@ -1463,7 +1465,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
                          n, ord, par, noull, nou, Ea[0], Eb[0], Ek[0],
                          ctr + cE(1), L.ecap, ctr + cPart(0), best,
-                         (ull *)nullptr, nou, EdgeTaus{});
+                         (ull *)nullptr, nou, EdgeTaus{}, 0);
     }
     PHEM_LAUNCHED("k_compact<R0>");
     // rounds 1.. start at once from upper bounds (m alive edges, n roots);
@@ -1506,7 +1508,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
                          n, ord, par, ctr + C_TAU, nou, Ea[0], Eb[0], Ek[0],
                          ctr + cE(0), L.ecap, ctr + cPart(0), best,
-                         (ull *)nullptr, nou, et);
+                         (ull *)nullptr, nou, et, 0);
     }
     PHEM_LAUNCHED("k_light");
     // the light edge count arrives with the first rounds' counters
@@ -1573,7 +1575,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                          dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
                          n, ord, par, ctr + C_TAU, gbits, Ea[cur], Eb[cur],
                          Ek[cur], ctr + cE(r), L.ecap, (ull *)nullptr, best,
-                         gmin, gsum, et);
+                         gmin, gsum, et, 0);
     }
     PHEM_LAUNCHED("k_heavy");
     {
