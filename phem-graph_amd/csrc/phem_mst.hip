@@ -46,6 +46,7 @@
 // Every appended list (alive edges, component lists) is segmented: tile v of
 // a producer appends to segment v mod KSEG, whose counter has its own cache
 // line, so no global atomic is shared by all tiles.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -61,8 +62,14 @@
 namespace phem {
 
 constexpr int BLK = 256;       // threads per block (4 waves)
-constexpr int EPT = 8;         // edges per thread per tile
+constexpr int EPT = 8;         // edges per thread per tile (input passes)
 constexpr int TILE = BLK * EPT;
+constexpr int EPT_E = 16;      // edges per thread per tile (compaction rounds
+constexpr int TILE_E = BLK * EPT_E;  // of large graphs, see wide_edges)
+
+// Compaction rounds of graphs with at least 2^22 edges use EPT_E edges per
+// thread; smaller graphs keep EPT, whose smaller tiles need less segment room.
+static inline bool wide_edges(uint64_t m) { return m >= (1ull << 22); }
 constexpr int WPT = 8;         // bitmap words per thread
 constexpr int WPB = BLK * WPT; // bitmap words per block
 constexpr int IPT = 4;         // vertices or components per thread (compress,
@@ -429,7 +436,7 @@ struct EdgeTaus {
   uint32_t k;
 };
 
-template <int MODE, class Src>
+template <int MODE, class Src, int NE_ = EPT>
 __global__ __launch_bounds__(BLK) void k_pass(
     Src src, const uint32_t *__restrict__ ia, const uint32_t *__restrict__ ib,
     const ull *__restrict__ ik, const ull *in_cnt, uint64_t in_cap,
@@ -438,54 +445,60 @@ __global__ __launch_bounds__(BLK) void k_pass(
     uint32_t *__restrict__ oa, uint32_t *__restrict__ ob,
     ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *valid_parts,
     ull *best, ull *gmin, const uint32_t *gsum, EdgeTaus et, int two_step) {
+  // tile shape: the compaction rounds of large graphs measured faster with
+  // EPT_E edges per thread (16: 1.43 against 1.52 ms of rounds at n = 2^24,
+  // m = 2^28), the input passes with EPT (8; 16 made k_light and k_heavy
+  // 0.15-0.4 ms slower)
+  constexpr int NE = NE_;
+  constexpr int KT = BLK * NE;
   __shared__ SegView sv;
   __shared__ uint32_t s_sum[MODE == P_HEAVY ? GSUM_HEAVY : 1];
   constexpr bool INPUT = MODE != P_EDGES;
-  if (!INPUT) seg_load<TILE>(sv, in_cnt);
+  if (!INPUT) seg_load<KT>(sv, in_cnt);
   uint32_t covered = 0;
   if (MODE == P_HEAVY) covered = gsum_load<BLK, GSUM_HEAVY>(s_sum, gsum, n);
   const ull tau = (MODE == P_LIGHT || MODE == P_HEAVY) ? *tau_ptr : 0ull;
-  const uint32_t ntiles_in = INPUT ? (uint32_t)ceil_div_dev(m0, TILE) : 0u;
+  const uint32_t ntiles_in = INPUT ? (uint32_t)ceil_div_dev(m0, KT) : 0u;
   uint32_t nval = 0;  // valid input edges seen by this thread
   for (uint32_t v = blockIdx.x;; v += gridDim.x) {
     uint64_t start;
     uint32_t len;
     if (INPUT) {
       if (v >= ntiles_in) break;
-      start = (uint64_t)v * TILE;
-      len = (uint32_t)min_u64(TILE, m0 - start);
+      start = (uint64_t)v * KT;
+      len = (uint32_t)min_u64(KT, m0 - start);
     } else {
-      if (!seg_tile<TILE>(sv, v, in_cap, &start, &len)) break;
+      if (!seg_tile<KT>(sv, v, in_cap, &start, &len)) break;
     }
     const uint64_t end = start + len;
-    uint32_t a[EPT], b[EPT];
-    ull key[EPT];
-    bool val[EPT];
+    uint32_t a[NE], b[NE];
+    ull key[NE];
+    bool val[NE];
     // A full tile (block-uniform) issues all its loads unconditionally, so
     // they are in flight together; a partial tile takes the guarded path.
-    const bool full = len == (uint32_t)TILE && (!INPUT || src.vec);
+    const bool full = len == (uint32_t)KT && (!INPUT || src.vec);
     if (INPUT) {
-      uint32_t wv[EPT];
+      uint32_t wv[NE];
       if (full) {
 #pragma unroll
-        for (int g = 0; g < EPT / 4; g++)
+        for (int g = 0; g < NE / 4; g++)
           src.load4_full(start + ((uint64_t)g * BLK + threadIdx.x) * 4, a + 4 * g,
                          b + 4 * g, wv + 4 * g);
       } else {
 #pragma unroll
-        for (int g = 0; g < EPT / 4; g++)
+        for (int g = 0; g < NE / 4; g++)
           src.load4(start + ((uint64_t)g * BLK + threadIdx.x) * 4, end, a + 4 * g,
                     b + 4 * g, wv + 4 * g);
       }
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
+      for (int j = 0; j < NE; j++) {
         const uint64_t i = start + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
         key[j] = make_key(wv[j], i, ord);
         val[j] = (a[j] < n) & (b[j] < n) & (a[j] != b[j]);
       }
     } else if (full) {
 #pragma unroll
-      for (int g = 0; g < EPT / 4; g++) {
+      for (int g = 0; g < NE / 4; g++) {
         const uint64_t i4 = start + ((uint64_t)g * BLK + threadIdx.x) * 4;
         u32x4 x = __builtin_nontemporal_load((const u32x4 *)(ia + i4));
         u32x4 y = __builtin_nontemporal_load((const u32x4 *)(ib + i4));
@@ -497,10 +510,10 @@ __global__ __launch_bounds__(BLK) void k_pass(
         key[4 * g + 2] = k1.x; key[4 * g + 3] = k1.y;
       }
 #pragma unroll
-      for (int j = 0; j < EPT; j++) val[j] = true;
+      for (int j = 0; j < NE; j++) val[j] = true;
     } else {
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
+      for (int j = 0; j < NE; j++) {
         const uint64_t i = start + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
         const bool in = i < end;
         a[j] = in ? ia[i] : 0u;
@@ -511,7 +524,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
     }
     if (INPUT) {
 #pragma unroll
-      for (int j = 0; j < EPT; j++) nval += val[j];
+      for (int j = 0; j < NE; j++) nval += val[j];
     }
     // the threshold of edge j (its graph's, for a batch): one lookup per
     // tile when the whole tile lies in one graph (graph ids grow with the
@@ -530,7 +543,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
     };
     if (MODE == P_LIGHT) {
 #pragma unroll
-      for (int j = 0; j < EPT; j++) val[j] = val[j] && key[j] < tau_of(j);
+      for (int j = 0; j < NE; j++) val[j] = val[j] && key[j] < tau_of(j);
     }
     if (MODE == P_HEAVY) {
       // both endpoints in the giant component: inside one component.  One
@@ -540,37 +553,37 @@ __global__ __launch_bounds__(BLK) void k_pass(
       // vertex's minimum to its root and k_giant_emit turns each
       // component's minimum into one alive edge.  The bitmap words of the
       // endpoints the LDS summary leaves open are read together.
-      bool ga[EPT], gb[EPT];
-      uint32_t wa[EPT], wb[EPT];
+      bool ga[NE], gb[NE];
+      uint32_t wa[NE], wb[NE];
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
+      for (int j = 0; j < NE; j++) {
         val[j] = val[j] && key[j] >= tau_of(j);
         ga[j] = val[j] && giant_summary_hit(a[j], s_sum, covered);
         gb[j] = val[j] && giant_summary_hit(b[j], s_sum, covered);
       }
       // unconditional reads (word 0 stands in for settled endpoints)
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
+      for (int j = 0; j < NE; j++) {
         wa[j] = giant[val[j] && !ga[j] ? a[j] >> 5 : 0u];
         wb[j] = giant[val[j] && !gb[j] ? b[j] >> 5 : 0u];
       }
-      bool one[EPT];
-      uint32_t x[EPT];
+      bool one[NE];
+      uint32_t x[NE];
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
+      for (int j = 0; j < NE; j++) {
         ga[j] = val[j] && (ga[j] || ((wa[j] >> (a[j] & 31)) & 1u));
         gb[j] = val[j] && (gb[j] || ((wb[j] >> (b[j] & 31)) & 1u));
         one[j] = val[j] && ga[j] != gb[j];
         x[j] = ga[j] ? b[j] : a[j];
         val[j] = val[j] && !(ga[j] || gb[j]);
       }
-      min_store_n<EPT>(gmin, x, key, one);
+      min_store_n<NE>(gmin, x, key, one);
     }
     if (MODE == P_EDGES) {
       // alive edges of late rounds name a few hot components; one edge's
       // lookups at a time measured faster here than the batched form below
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
+      for (int j = 0; j < NE; j++) {
         if (val[j]) {
           a[j] = par[a[j]];
           b[j] = par[b[j]];
@@ -581,31 +594,31 @@ __global__ __launch_bounds__(BLK) void k_pass(
         }
       }
     } else if (MODE != P_LIGHT) {
-      uint32_t pa[EPT], pb[EPT];
+      uint32_t pa[NE], pb[NE];
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
+      for (int j = 0; j < NE; j++) {
         pa[j] = par[val[j] ? a[j] : 0u];  // unconditional, as above
         pb[j] = par[val[j] ? b[j] : 0u];
       }
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
+      for (int j = 0; j < NE; j++) {
         a[j] = pa[j];
         b[j] = pb[j];
       }
     }
-    bool keep[EPT];
+    bool keep[NE];
 #pragma unroll
-    for (int j = 0; j < EPT; j++) keep[j] = val[j] && a[j] != b[j];
+    for (int j = 0; j < NE; j++) keep[j] = val[j] && a[j] != b[j];
     if (MODE == P_LIGHT || MODE == P_HEAVY) {
       // input edges spread their offers over many components: the
       // pre-check reads of the tile are issued together
-      min_store_ends<EPT>(best, a, b, key, keep);
+      min_store_ends<NE>(best, a, b, key, keep);
     } else {
       // relabelled edges concentrate on few components in late rounds: each
       // pre-check follows the thread's previous atomic, so a component's
       // lowered minimum filters the offers after it
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
+      for (int j = 0; j < NE; j++) {
         if (keep[j]) {
           min_store(best, a[j], key[j]);
           min_store(best, b[j], key[j]);
@@ -615,11 +628,11 @@ __global__ __launch_bounds__(BLK) void k_pass(
     // survivors go straight from registers to their slots in the tile's
     // output segment (one atomic per tile)
     const uint32_t seg = v % KSEG;
-    uint64_t slot[EPT];
-    block_append<BLK, EPT>(keep, slot, out_cnt + seg * CSTR,
+    uint64_t slot[NE];
+    block_append<BLK, NE>(keep, slot, out_cnt + seg * CSTR,
                            (uint64_t)seg * out_cap);
 #pragma unroll
-    for (int j = 0; j < EPT; j++) {
+    for (int j = 0; j < NE; j++) {
       if (keep[j]) {
         oa[slot[j]] = a[j];
         ob[slot[j]] = b[j];
@@ -1152,14 +1165,25 @@ static MstLayout mst_layout(uint32_t n, uint64_t m) {
   L.words = ceil_div(m ? m : 1, 32);
   L.nb = ceil_div(L.words, WPB);
   // Segment capacities: a producer pass over at most ceil(x / T) + KSEG tiles
-  // sends at most ceil(ceil(x / T) / KSEG) + 1 tiles to one segment.
+  // of T items sends at most ceil(ceil(x / T) / KSEG) + 1 tiles to one
+  // segment, at most x / KSEG + 2 T items; the bound with the compaction
+  // rounds' tile (TILE_E for wide_edges graphs) covers the input passes'
+  // TILE too.
   // The heavy pass and k_giant_emit (LTILE vertices per tile) append to the
   // same list, so its segments hold both producers' worst case.
-  L.ecap = (ceil_div(ceil_div(m ? m : 1, TILE), KSEG) + 2) * TILE +
+  // No list ever holds more than m + n alive edges (heavy survivors plus one
+  // giant edge per component), so no segment does either: small graphs get
+  // segments of m + n items instead of the tile-based bound.
+  static_assert(TILE_E >= TILE, "segment capacity uses the largest edge tile");
+  const uint64_t TE = wide_edges(m) ? TILE_E : TILE;
+  L.ecap = (ceil_div(ceil_div(m ? m : 1, TE), KSEG) + 2) * TE +
            ceil_div(ceil_div(n ? n : 1, LTILE), KSEG) * LTILE;
+  L.ecap = std::min<uint64_t>(L.ecap, align_up(m + n + 1, 4));
   // Component lists also come from the hook's smaller tiles (HTILE items):
   // ceil(ceil(n / HTILE) / KSEG) * HTILE <= n / KSEG + HTILE, below lcap.
+  // A component list holds at most n entries.
   L.lcap = (ceil_div(ceil_div(n ? n : 1, LTILE), KSEG) + 2) * LTILE;
+  L.lcap = std::min<uint64_t>(L.lcap, align_up((uint64_t)n + 1, 4));
   const uint64_t E = L.ecap * KSEG, LN = L.lcap * KSEG;
   L.par = take(4ull * n);
   L.best = take(8ull * n);
@@ -1246,6 +1270,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   const int ord = wmode | ((flags & PHEM_MST_MAXIMUM) ? ORD_MAX : 0);
   if (n && !d_mst_ids) return set_err(PHEM_EINVAL, "d_mst_ids is NULL");
   const MstLayout L = mst_layout(n, m);
+  const bool wide = wide_edges(m);  // compaction tile, as in the layout
   if (!d_ws || ws_bytes < L.total)
     return set_err(PHEM_ENOMEM, "workspace %llu B < required %llu B",
                    (unsigned long long)ws_bytes, (unsigned long long)L.total);
@@ -1375,13 +1400,22 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
         const int two_step = giant_hooked && first && k == 0;
         {
           PHEM_PROF("k_compact", s);
-          hipLaunchKernelGGL((k_pass<P_EDGES, Src>),
-                             dim3(grid_for(ceil_div(m_r, TILE) + KSEG, 1)),
-                             dim3(BLK), 0, s, src, Ea[c], Eb[c], Ek[c],
-                             ctr + cE(rr), L.ecap, 0ull, n, ord, par, noull,
-                             nou, Ea[c ^ 1], Eb[c ^ 1], Ek[c ^ 1],
-                             ctr + cE(rr + 1), L.ecap, (ull *)nullptr, best,
-                             (ull *)nullptr, nou, EdgeTaus{}, two_step);
+          if (wide)
+            hipLaunchKernelGGL((k_pass<P_EDGES, Src, EPT_E>),
+                               dim3(grid_for(ceil_div(m_r, TILE_E) + KSEG, 1)),
+                               dim3(BLK), 0, s, src, Ea[c], Eb[c], Ek[c],
+                               ctr + cE(rr), L.ecap, 0ull, n, ord, par, noull,
+                               nou, Ea[c ^ 1], Eb[c ^ 1], Ek[c ^ 1],
+                               ctr + cE(rr + 1), L.ecap, (ull *)nullptr, best,
+                               (ull *)nullptr, nou, EdgeTaus{}, two_step);
+          else
+            hipLaunchKernelGGL((k_pass<P_EDGES, Src>),
+                               dim3(grid_for(ceil_div(m_r, TILE) + KSEG, 1)),
+                               dim3(BLK), 0, s, src, Ea[c], Eb[c], Ek[c],
+                               ctr + cE(rr), L.ecap, 0ull, n, ord, par, noull,
+                               nou, Ea[c ^ 1], Eb[c ^ 1], Ek[c ^ 1],
+                               ctr + cE(rr + 1), L.ecap, (ull *)nullptr, best,
+                               (ull *)nullptr, nou, EdgeTaus{}, two_step);
         }
         PHEM_LAUNCHED("k_compact");
       }
