@@ -321,8 +321,8 @@ def main():
                          "(a known-bytes reference for hardware counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "multi-GPU sharding is described in DESIGN.md and not run", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); "
+              "sharding a graph over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
