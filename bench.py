@@ -70,28 +70,71 @@ def b_cc(m, n):
     return 8 * m + 12 * n
 
 
+def host_cpus():
+    """What the host offers this process: nproc (os.cpu_count), the CPUs in
+    its affinity mask, the cgroup CPU quota (cpu.max, in CPUs; None when
+    unlimited or unreadable) and the CPU model name."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity"] = info["nproc"]
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpus"] = quota
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info["cpu_model"] = model
+    return info
+
+
 def cpu_baseline(u_np, v_np, w_np, n, threads, runs):
-    """The CPU restatement (OpenMP PBBS filter-Kruskal MST + HAST union-find
-    CC) on the same edges: one warm-up, then the median of `runs` timed runs
-    (the PBBS time_loop convention, common/time_loop.h:3-18).  Returns edges/s
-    of MST + CC and the median MST and CC seconds."""
+    """The CPU path on the same edges: MST by the OpenMP restatement of PBBS
+    filter-Kruskal (MST/src/utils.cu:794-841; the reference's own file cannot
+    be compiled here, it includes CUB), CC by the reference's own HAST
+    host_spanning_tree (CC/host_spanning_tree.cu:67-92, built from its
+    sources into oracle/_ref with its ParlayLib) when that build is present,
+    else by the OpenMP restatement.  One warm-up, then the median of `runs`
+    timed runs (the PBBS time_loop convention, common/time_loop.h:3-18).
+    Returns (edges/s of MST + CC, MST s, CC s, CC kind)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    import numpy as np
+    ref_cc = os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libref_hostcc.so"))
+    if ref_cc:
+        os.environ["PARLAY_NUM_THREADS"] = str(threads)  # read at the scheduler's start
+        rec = np.ascontiguousarray((np.minimum(u_np, v_np).astype(np.uint64) << np.uint64(32))
+                                   | np.maximum(u_np, v_np).astype(np.uint64))
     tm, tc = [], []
     for r in range(runs + 1):
         t0 = time.perf_counter()
         O.par_mst(n, u_np, v_np, w_np, 1, threads=threads)
         t1 = time.perf_counter()
-        O.par_cc(n, u_np, v_np, threads=threads)
-        t2 = time.perf_counter()
-        print(f"# cpu run {r}: mst {t1 - t0:.2f} s cc {t2 - t1:.2f} s", file=sys.stderr,
-              flush=True)
+        if ref_cc:
+            cc_s, workers = O.ref_host_cc_time(n, rec)
+        else:
+            O.par_cc(n, u_np, v_np, threads=threads)
+            cc_s = time.perf_counter() - t1
+        print(f"# cpu run {r}: mst {t1 - t0:.2f} s cc {cc_s:.2f} s"
+              + (f" (reference, {workers} ParlayLib workers)" if ref_cc else ""),
+              file=sys.stderr, flush=True)
         if r:
             tm.append(t1 - t0)
-            tc.append(t2 - t1)
+            tc.append(cc_s)
     tm.sort(), tc.sort()
     mst_s, cc_s = tm[len(tm) // 2], tc[len(tc) // 2]
-    return len(u_np) / (mst_s + cc_s), mst_s, cc_s
+    return len(u_np) / (mst_s + cc_s), mst_s, cc_s, ("reference" if ref_cc else "port")
 
 
 def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_graphs=256):
@@ -466,20 +509,35 @@ def main():
     }
 
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        host = host_cpus()
+        # every CPU the process may use, within its cgroup quota; 16 at most,
+        # the host share of one GPU on the measurement boxes
+        threads = host["affinity"] or 1
+        if host["cgroup_cpus"]:
+            threads = min(threads, max(1, int(host["cgroup_cpus"])))
+        threads = min(threads, 16)
         k = min(m, 1 << args.cpu_sample_log2)
         u_np = du[:k].cpu().numpy().view(np.uint32)
         v_np = dv[:k].cpu().numpy().view(np.uint32)
         w_np = dw[:k].cpu().numpy().view(np.uint32)
-        print(f"# cpu baseline on {k} edges, {threads} threads", file=sys.stderr, flush=True)
-        eps, t_mst, t_cc = cpu_baseline(u_np, v_np, w_np, n, threads, args.cpu_runs)
+        print(f"# cpu baseline on {k} edges, {threads} threads, host {host}",
+              file=sys.stderr, flush=True)
+        eps, t_mst, t_cc, cc_kind = cpu_baseline(u_np, v_np, w_np, n, threads, args.cpu_runs)
         what = "all" if k == m else f"the first {k} of the"
-        out["cpu_baseline"] = {"value": eps, "unit": "edges/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"{what} {m} edges of the same RMAT graph (n={n}), "
-                                         f"median of {args.cpu_runs} runs after a warm-up: "
-                                         f"OpenMP PBBS filter-Kruskal MST {t_mst:.2f} s + "
-                                         f"HAST union-find CC {t_cc:.2f} s"}
+        out["cpu_baseline"] = {
+            "value": eps, "unit": "edges/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{what} {m} edges of the same RMAT graph (n={n}), median of "
+                      f"{args.cpu_runs} runs after a warm-up: MST by the OpenMP PBBS "
+                      f"filter-Kruskal restatement {t_mst:.2f} s + CC by "
+                      + ("the reference's HAST host_spanning_tree (ParlayLib)"
+                         if cc_kind == "reference" else "the OpenMP HAST restatement")
+                      + f" {t_cc:.2f} s",
+            "parts": {"mst": {"kind": "port", "s": round(t_mst, 4),
+                              "edges_per_s": k / t_mst},
+                      "cc": {"kind": cc_kind, "s": round(t_cc, 4),
+                             "edges_per_s": k / t_cc}},
+            "host": host}
     if not args.no_serve:
         del du, dv, dw, de, solver, cc_par, cc_lab, cc_forest
         torch.cuda.empty_cache()

@@ -258,7 +258,10 @@ int phem_stream_create(uint32_t n, uint64_t max_batch, int wmode,
                        phem_stream_t **out);
 int phem_stream_destroy(phem_stream_t *s);
 /* Inserts mb edges (device arrays); returns once forest and parent array are
- * updated. */
+ * updated.  On an error before the batch's union-find ran, the stream is as
+ * before the call; on a later error (a HIP failure while the result is read)
+ * the union-find state already holds the batch, so the stream refuses every
+ * further phem_stream_apply and phem_stream_save with PHEM_EINVAL. */
 int phem_stream_apply(phem_stream_t *s, const uint32_t *d_u,
                       const uint32_t *d_v, const uint32_t *d_w, uint64_t mb,
                       void *stream);
@@ -383,6 +386,9 @@ typedef struct {
   int32_t weight_exact;
   uint32_t batch_graphs; /* graphs in the batch the request ran in */
   double arrival, start, done;
+  char error[160];       /* the batch's error message ("" when status is 0);
+                            phem_last_error() of the client thread does not
+                            hold it, the batch ran on a worker thread */
 } phem_result;
 
 typedef struct {
@@ -398,9 +404,12 @@ int phem_server_create(const phem_server_opts *opts, phem_server **out);
 int phem_server_destroy(phem_server *srv);
 int phem_server_submit(phem_server *srv, const phem_request *req,
                        uint64_t *ticket);
-/* Blocks until the request is done, fills *res and forgets the ticket. */
+/* Blocks until the request is done, fills *res and forgets the ticket.  One
+ * waiter per ticket: a second wait, or a poll, while one waits returns
+ * PHEM_EBUSY. */
 int phem_server_wait(phem_server *srv, uint64_t ticket, phem_result *res);
-/* *done = 1 and *res filled (ticket forgotten) if done, else *done = 0. */
+/* *done = 1 and *res filled (ticket forgotten) if done, else *done = 0
+ * (PHEM_EBUSY while a phem_server_wait waits for the ticket). */
 int phem_server_poll(phem_server *srv, uint64_t ticket, phem_result *res,
                      int *done);
 /* Blocks until every submitted request is done. */

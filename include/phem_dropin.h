@@ -78,9 +78,16 @@ void mmapPopulateEdgeArray(const std::string &filename, edge *edges,
  *    its own forest on the CPU (phem_cpu_msf) while the GPU folds in the
  *    others (MST/src/kernels.cu:1155-1826);
  *  - use_malloc_managed: runs the default path (device memory);
- *  - use_cpu_only: the ParlayLib CPU path stays the caller's
- *    (cpu_compute_pbbs); this library prints a notice and returns;
+ *  - use_cpu_only (not in debug mode, as in the reference): the forest is
+ *    computed on the host by the function registered with phem_set_cpu_mst
+ *    (the caller's ParlayLib cpu_compute_pbbs, MST/src/utils.cu:887, behind
+ *    an adapter), or by the library's host solver phem_cpu_msf when none is
+ *    registered; prints "pbbs execution time T microseconds";
+ *  - debug: the streamline modes run 10 times, printing "i th execution
+ *    time T" (microseconds) and "Average exec time T"
+ *    (MST/src/new_boruvka.cu:356-384, MST/src/kernels.cu:878-1138);
  *  - debug && save_weighted_graph: writes weighted.mtx and returns.
+ * Dispatch order as in the reference (MST/src/new_boruvka.cu:356-506).
  * Prints "Total weight W" (the exact forest weight).  When result_filename is
  * not "res.txt" the forest is written there as "u+1 v+1 w" lines. */
 void new_boruvka(std::string filename, std::string result_filename, bool debug,
@@ -90,14 +97,29 @@ void new_boruvka(std::string filename, std::string result_filename, bool debug,
                  bool use_cpu_only = false, bool use_cpu_gpu_streamline = false,
                  bool save_weighted_graph = false);
 
+/* The CPU minimum spanning forest new_boruvka(use_cpu_only) calls: the n,
+ * m edges (u[i], v[i], w[i]) with unsigned weights; writes the forest's edge
+ * ids (any order; n slots) to ids_out and their number to *count; returns 0
+ * on success.  No counterpart in the reference, which calls its ParlayLib
+ * cpu_compute_pbbs directly (MST/src/new_boruvka.cu:468-480): this library
+ * cannot name that function's ParlayLib argument type, so the caller
+ * registers a plain-array adapter around it.  NULL restores the library's
+ * host solver (phem_cpu_msf). */
+typedef int (*phem_cpu_mst_fn)(uint32_t n, uint64_t m, const uint32_t *u,
+                               const uint32_t *v, const uint32_t *w,
+                               uint32_t *ids_out, uint64_t *count);
+extern "C" void phem_set_cpu_mst(phem_cpu_mst_fn fn);
+
 /* Minimum spanning forest of device edges (MST/src/kernels.cu:507-689).
  * Reads d_edges[0 .. num_edges); appends the forest's edges to
  * d_final_msf + number_of_filled_edges; writes min-id component labels to
  * d_representatives; sets num_edges to 0 (every edge consumed).  The other
  * buffers (d_iter_edges, d_msf, d_new_representatives,
  * d_iter_representatives, d_outgoing_edges) are accepted for signature
- * compatibility and left untouched; the solver's workspace is allocated and
- * cached by the library. */
+ * compatibility and left untouched: the solver needs about 32 B per edge of
+ * workspace, more than those buffers hold, so its workspace is allocated and
+ * cached by the library (grown on demand, one per device) until
+ * phem_dropin_release_workspace frees it. */
 void device_boruvka(edge *d_edges, edge *d_final_msf, edge *d_iter_edges,
                     edge *d_msf, vertex *d_representatives,
                     vertex *d_new_representatives,
@@ -105,6 +127,12 @@ void device_boruvka(edge *d_edges, edge *d_final_msf, edge *d_iter_edges,
                     outgoing_edge_id *d_outgoing_edges, vertex &num_nodes,
                     vertex &num_edges, vertex &number_of_filled_edges,
                     hipStream_t kernel_stream = 0);
+
+/* Frees the workspace device_boruvka cached on the current device (no
+ * counterpart in the reference, whose device_boruvka works in the caller's
+ * buffers, MST/src/kernels.cu:507-518).  The next device_boruvka allocates
+ * it again. */
+extern "C" void phem_dropin_release_workspace(void);
 
 /* The reference's full parameter list (its four thrust::device_ptr views of
  * the buffers above are ignored). */

@@ -157,11 +157,16 @@ def par_cc(n, u, v, threads=0, forest=False):
 
 
 def wkey(w, wmode=1):
+    """Order key of weight bit patterns (or_wkey in phem_oracle.c): float32
+    -0.0 equals +0.0, every NaN is one value above +inf."""
     w = np.asarray(w, dtype=np.uint32)
     if wmode == 0:
         return w.copy()
     neg = (w & np.uint32(0x80000000)) != 0
-    return np.where(neg, ~w, w | np.uint32(0x80000000)).astype(np.uint32)
+    k = np.where(neg, ~w, w | np.uint32(0x80000000)).astype(np.uint32)
+    mag = w & np.uint32(0x7FFFFFFF)
+    k = np.where(mag == 0, np.uint32(0x80000000), k)
+    return np.where(mag > np.uint32(0x7F800000), np.uint32(0xFFFFFFFF), k).astype(np.uint32)
 
 
 # ---- reference CPU builds (oracle/_ref), present only where they were built ----
@@ -208,3 +213,18 @@ def ref_host_cc(n, u, v):
     sp = np.empty(n, np.uint64)
     _ref_cc.ref_host_cc(n, len(e), e, lab, sp)
     return lab, sp
+
+
+def ref_host_cc_time(n, edges):
+    """Seconds of the reference's HAST host_spanning_tree over the packed
+    (u << 32 | v) records `edges` (numpy uint64), inputs prepared outside the
+    timer; and the ParlayLib worker count it ran with."""
+    global _ref_cc
+    if _ref_cc is None:
+        ref_host_cc(1, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    L = _ref_cc
+    L.ref_host_cc_time.restype = ctypes.c_double
+    L.ref_host_cc_time.argtypes = [ctypes.c_int, ctypes.c_long, _u64p]
+    L.ref_host_cc_workers.restype = ctypes.c_int
+    e = np.ascontiguousarray(edges, dtype=np.uint64)
+    return L.ref_host_cc_time(n, len(e), e), L.ref_host_cc_workers()

@@ -18,8 +18,13 @@
 
 uint32_t or_wkey(uint32_t w, int wmode) {
   if (wmode == 0) return w;
-  /* IEEE-754 total order on the bit pattern: negative values reversed and
-   * placed below the positive ones. */
+  /* Numeric order of the reference's float comparator, (a.w < b.w) ||
+   * (a.w == b.w && a.id < b.id) (MST/src/utils.cu:801-802,
+   * MST/src/bench/serialMST.C:67-68): -0.0 equals +0.0 (ties by id);
+   * negative values reversed and placed below the positive ones.  NaN,
+   * unordered there, is defined as one value above +inf. */
+  if ((w & 0x7FFFFFFFu) == 0) return 0x80000000u;
+  if ((w & 0x7FFFFFFFu) > 0x7F800000u) return 0xFFFFFFFFu;
   return (w & 0x80000000u) ? ~w : (w | 0x80000000u);
 }
 

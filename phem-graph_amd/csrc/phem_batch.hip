@@ -281,23 +281,13 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
   uint32_t *first = (uint32_t *)(ws + L.first);
   uint32_t *emap = (uint32_t *)(ws + L.emap), *vmap = (uint32_t *)(ws + L.vmap);
   const uint32_t *gw = table ? (const uint32_t *)(ws + L.gw) : A.d_w;
-  // offsets (and pointer tables) go up in one copy from a pinned buffer
-  // owned by the calling thread, so the copy is asynchronous and the next
-  // call cannot overwrite it before the solve below has synchronized
-  struct Stage {
-    uint64_t *p = nullptr;
-    uint64_t cap = 0;
-  };
-  static thread_local Stage st_buf;
+  // offsets (and pointer tables) go up from a pinned buffer of the shared
+  // pool, held until the solve below has synchronized, so the copies are
+  // asynchronous
   const uint64_t words = 2ull * (k + 1) + (table ? 3ull * k : 0ull);
-  if (st_buf.cap < words) {
-    if (st_buf.p) (void)hipHostFree(st_buf.p);
-    st_buf.p = nullptr;
-    st_buf.cap = 0;
-    PHEM_HIP(hipHostMalloc((void **)&st_buf.p, 8 * words));
-    st_buf.cap = words;
-  }
-  uint64_t *hstage = st_buf.p;
+  PinnedLease stage(8 * words, s);
+  if (!stage.get()) return set_err(PHEM_ENOMEM, "pinned offset staging");
+  uint64_t *hstage = (uint64_t *)stage.get();
   std::memcpy(hstage, h_voff, 8ull * (k + 1));
   std::memcpy(hstage + (k + 1), h_eoff, 8ull * (k + 1));
   PHEM_HIP(hipMemcpyAsync(voff, hstage, 8ull * (k + 1), hipMemcpyHostToDevice, s));
@@ -342,6 +332,7 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
                         ws + L.mst_ws, L.mst_bytes, ids, glab, A.opts, &st, s,
                         groups);
   if (rc) return rc;
+  stage.done();  // the solve's counter reads synchronized the stream
   {
     PHEM_PROF("k_batch_counts", s);
     hipLaunchKernelGGL(k_batch_counts, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,

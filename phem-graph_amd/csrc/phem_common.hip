@@ -69,6 +69,56 @@ double cc_default_sample_factor() {
   return g_sample_factor.load(std::memory_order_relaxed);
 }
 
+// ---- shared pinned host buffers ---------------------------------------------
+
+namespace {
+struct PinnedBuf {
+  void *p;
+  size_t cap;
+};
+std::mutex g_pin_mu;
+std::vector<PinnedBuf> g_pin_free;  // buffers not in use
+std::vector<PinnedBuf> g_pin_used;  // buffers handed out
+}  // namespace
+
+void *pinned_acquire(size_t bytes) {
+  if (bytes < 4096) bytes = 4096;
+  {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    // the smallest free buffer that is large enough
+    size_t best = g_pin_free.size();
+    for (size_t i = 0; i < g_pin_free.size(); i++)
+      if (g_pin_free[i].cap >= bytes &&
+          (best == g_pin_free.size() || g_pin_free[i].cap < g_pin_free[best].cap))
+        best = i;
+    if (best < g_pin_free.size()) {
+      PinnedBuf b = g_pin_free[best];
+      g_pin_free.erase(g_pin_free.begin() + best);
+      g_pin_used.push_back(b);
+      return b.p;
+    }
+  }
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_pin_mu);
+  g_pin_used.push_back(PinnedBuf{p, bytes});
+  return p;
+}
+
+void pinned_release(void *p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_pin_mu);
+  for (size_t i = 0; i < g_pin_used.size(); i++)
+    if (g_pin_used[i].p == p) {
+      g_pin_free.push_back(g_pin_used[i]);
+      g_pin_used.erase(g_pin_used.begin() + i);
+      return;
+    }
+}
+
 // ---- per-kernel event profiler --------------------------------------------
 
 namespace {

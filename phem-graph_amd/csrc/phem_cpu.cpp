@@ -24,8 +24,12 @@
 namespace phem {
 namespace {
 
+// The device order key (phem_dev.h wkey): float32 -0.0 orders as +0.0 and
+// every NaN as one value above +inf.
 uint32_t order_key(uint32_t w, int wmode) {
   if (wmode == PHEM_W_U32) return w;
+  if ((w & 0x7FFFFFFFu) == 0) return 0x80000000u;
+  if ((w & 0x7FFFFFFFu) > 0x7F800000u) return 0xFFFFFFFFu;
   return (w & 0x80000000u) ? ~w : (w | 0x80000000u);
 }
 

@@ -12,10 +12,17 @@ constexpr int WAVE = 64;
 
 typedef unsigned long long ull;
 
-// Order key of a 32-bit weight pattern: identity for unsigned weights, the
-// IEEE-754 total order for float32 (negatives reversed below positives).
+// Order key of a 32-bit weight pattern: identity for unsigned weights; for
+// float32 the numeric order of the reference's comparator
+// ((a.w < b.w) || (a.w == b.w && a.id < b.id), MST/src/utils.cu:801-802,
+// MST/src/bench/serialMST.C:67-68): -0.0 and +0.0 are one weight (ties by
+// id), negatives below positives.  NaN, which that comparator leaves
+// unordered, is defined here as one weight above +inf.
 __device__ __forceinline__ uint32_t wkey(uint32_t w, int mode) {
-  return mode ? ((w & 0x80000000u) ? ~w : (w | 0x80000000u)) : w;
+  if (!mode) return w;
+  if ((w & 0x7FFFFFFFu) == 0) return 0x80000000u;           // +-0
+  if ((w & 0x7FFFFFFFu) > 0x7F800000u) return 0xFFFFFFFFu;  // NaN
+  return (w & 0x80000000u) ? ~w : (w | 0x80000000u);
 }
 __device__ __forceinline__ uint32_t wkey_inv(uint32_t k, int mode) {
   return mode ? ((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k) : k;

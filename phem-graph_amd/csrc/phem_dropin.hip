@@ -1,4 +1,5 @@
 // phem_dropin.hip -- the reference's C++ entry points over the C ABI.
+#include <chrono>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -69,10 +70,26 @@ void *cached_workspace(size_t bytes) {
   return w.p;
 }
 
+void release_workspace() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  if (dev < 0 || dev >= 64) dev = 0;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  CachedWs &w = g_ws[dev];
+  if (w.p) (void)hipFree(w.p);
+  w.p = nullptr;
+  w.cap = 0;
+}
+
 }  // namespace
 }  // namespace phem
 
 using namespace phem;
+
+extern "C" void phem_dropin_release_workspace(void) { release_workspace(); }
 
 extern "C" int phem_gather_edges_aos(const uint32_t *d_src, const uint32_t *d_ids,
                                      uint64_t count, uint32_t *d_out,
@@ -224,6 +241,96 @@ uint64_t report_forest(const edge *f, uint64_t k, const std::string &result) {
 
 }  // namespace
 
+namespace {
+
+// The caller's CPU MST for use_cpu_only (phem_set_cpu_mst); NULL: the
+// library's host solver.
+phem_cpu_mst_fn g_cpu_mst = nullptr;
+
+// The streamline paths: chunks of cs edges folded into a forest kept on the
+// device, F = MSF(F u chunk) (MST/src/kernels.cu:849-1154); with `hetero`
+// the last chunk is first reduced to its own forest on the CPU while the GPU
+// folds in the others (MST/src/kernels.cu:1155-1826).  Returns the forest
+// and the elapsed microseconds of the solve (reading excluded).
+std::vector<edge> streamline_forest(const phem_graph_file &g, uint64_t cs,
+                                    bool hetero, long long *usec) {
+  const uint32_t n = g.n;
+  const uint64_t m = g.m;
+  const auto t0 = std::chrono::steady_clock::now();
+  phem_stream_t *st = nullptr;
+  die_phem(phem_stream_create(n, cs, PHEM_W_U32, &st), "phem_stream_create");
+  uint32_t *d = nullptr;
+  die_hip(hipMalloc(&d, 12 * cs), "hipMalloc (chunk)");
+  auto apply = [&](const uint32_t *hu, const uint32_t *hv, const uint32_t *hw,
+                   uint64_t k) {
+    die_hip(hipMemcpy(d, hu, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
+    die_hip(hipMemcpy(d + cs, hv, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
+    die_hip(hipMemcpy(d + 2 * cs, hw, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
+    die_phem(phem_stream_apply(st, d, d + cs, d + 2 * cs, k, nullptr),
+             "phem_stream_apply");
+  };
+  const uint64_t nchunks = m ? (m + cs - 1) / cs : 0;
+  const uint64_t last = nchunks ? (nchunks - 1) * cs : 0;  // first edge of the last chunk
+  // heterogeneous: the CPU share is the first 10^8 edges of the last chunk,
+  // as the reference caps it; the rest of that chunk stays unfiltered
+  std::vector<uint32_t> cpu_ids;
+  uint64_t cpu_cnt = 0, cpu_span = 0;
+  std::thread cpu;
+  if (hetero && nchunks > 1) {
+    cpu_span = std::min<uint64_t>(m - last, 100000000ull);
+    cpu_ids.resize(n ? n : 1);
+    cpu = std::thread([&] {
+      die_phem(phem_cpu_msf(n, cpu_span, g.u + last, g.v + last, g.w + last,
+                            PHEM_W_U32, cpu_ids.data(), &cpu_cnt, 0),
+               "phem_cpu_msf");
+    });
+  }
+  for (uint64_t c = 0; c < nchunks; c++) {
+    const uint64_t off = c * cs, k = m - off < cs ? m - off : cs;
+    if (c + 1 < nchunks || !cpu.joinable()) {
+      apply(g.u + off, g.v + off, g.w + off, k);
+      continue;
+    }
+    cpu.join();
+    std::vector<uint32_t> fu, fv, fw;  // filtered chunk, input order kept
+    for (uint64_t i = 0; i < cpu_cnt; i++) {
+      const uint64_t e = last + cpu_ids[i];
+      fu.push_back(g.u[e]);
+      fv.push_back(g.v[e]);
+      fw.push_back(g.w[e]);
+    }
+    for (uint64_t e = last + cpu_span; e < m; e++) {
+      fu.push_back(g.u[e]);
+      fv.push_back(g.v[e]);
+      fw.push_back(g.w[e]);
+    }
+    std::printf("%llu cut down to %llu\n", (unsigned long long)cpu_span,
+                (unsigned long long)cpu_cnt);
+    apply(fu.data(), fv.data(), fw.data(), fu.size());
+  }
+  uint64_t fc = 0;
+  const uint32_t *fu, *fv, *fw;
+  const uint64_t *fg;
+  die_phem(phem_stream_forest(st, &fc, &fu, &fv, &fw, &fg), "phem_stream_forest");
+  std::vector<uint32_t> hu(fc), hv(fc), hw(fc);
+  if (fc) {
+    die_hip(hipMemcpy(hu.data(), fu, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
+    die_hip(hipMemcpy(hv.data(), fv, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
+    die_hip(hipMemcpy(hw.data(), fw, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
+  }
+  std::vector<edge> forest(fc);
+  for (uint64_t i = 0; i < fc; i++) forest[i] = edge{hu[i], hv[i], hw[i]};
+  die_hip(hipFree(d), "hipFree");
+  die_phem(phem_stream_destroy(st), "phem_stream_destroy");
+  *usec = std::chrono::duration_cast<std::chrono::microseconds>(
+              std::chrono::steady_clock::now() - t0).count();
+  return forest;
+}
+
+}  // namespace
+
+extern "C" void phem_set_cpu_mst(phem_cpu_mst_fn fn) { g_cpu_mst = fn; }
+
 void new_boruvka(std::string filename, std::string result_filename, bool debug,
                  vertex chunk_size, int num_chunks_ideal,
                  bool generate_random_weights, bool use_malloc_managed,
@@ -238,109 +345,82 @@ void new_boruvka(std::string filename, std::string result_filename, bool debug,
   std::printf("Total num edges %llu\nTotal num nodes %u\n", (unsigned long long)m, n);
   if (num_chunks_ideal > 0)  // MST/src/new_boruvka.cu:326-354
     chunk_size = (vertex)((float)m / ((float)num_chunks_ideal + 0.4f));
-  if (debug && save_weighted_graph) {
+  // Dispatch in the reference's order (MST/src/new_boruvka.cu:356-506):
+  // debug runs the streamline paths 10 times and reports each time; it
+  // checks save_weighted_graph and never use_cpu_only; otherwise
+  // use_cpu_only is checked after the streamline and managed modes.
+  if (use_streamline || use_cpu_gpu_streamline) {
+    const uint64_t cs = chunk_size ? chunk_size : (m ? m : 1);
+    std::printf("streamline chunk boruvka, chunk size %llu\n", (unsigned long long)cs);
+    const int iters = debug ? 10 : 1;  // execution_time_iterations, :358
+    std::vector<edge> forest;
+    long long total = 0;
+    for (int it = 0; it < iters; it++) {
+      long long us = 0;
+      forest = streamline_forest(g, cs, use_cpu_gpu_streamline, &us);
+      total += us;
+      if (debug) std::printf("%d th execution time %lld\n", it, us);
+    }
+    if (debug) {
+      std::printf("Average exec time %lld\n", total / iters);
+      std::printf("debug %d\n", (int)debug);
+    }
+    const uint64_t w = report_forest(forest.data(), forest.size(), result_filename);
+    std::printf("final weight by streamline approach %llu\n", (unsigned long long)w);
+    phem_graph_file_free(&g);
+    return;
+  }
+  if (use_malloc_managed)
+    std::printf("malloc managed: runs in device memory\n");
+  if (debug && save_weighted_graph && !use_malloc_managed) {
     die_phem(phem_write_mtx_weighted("weighted.mtx", &g), "phem_write_mtx_weighted");
     phem_graph_file_free(&g);
     return;
   }
-  if (use_cpu_only) {
-    std::printf("cpu only: the ParlayLib path (cpu_compute_pbbs) is the "
-                "caller's; nothing to do here\n");
+  if (!debug && use_cpu_only && !use_malloc_managed) {
+    // the CPU path (MST/src/new_boruvka.cu:468-492): the caller's solver
+    // registered with phem_set_cpu_mst (the ParlayLib cpu_compute_pbbs
+    // behind a plain-array adapter), else the library's host solver
+    std::vector<uint32_t> ids(n ? n : 1);
+    uint64_t cnt = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (g_cpu_mst) {
+      if (g_cpu_mst(n, m, g.u, g.v, g.w, ids.data(), &cnt) != 0 || cnt > n) {
+        std::fprintf(stderr, "new_boruvka: the registered CPU MST failed\n");
+        std::exit(EXIT_FAILURE);
+      }
+      std::sort(ids.begin(), ids.begin() + cnt);
+    } else {
+      die_phem(phem_cpu_msf(n, m, g.u, g.v, g.w, PHEM_W_U32, ids.data(), &cnt, 0),
+               "phem_cpu_msf");
+    }
+    const long long us = std::chrono::duration_cast<std::chrono::microseconds>(
+                             std::chrono::steady_clock::now() - t0).count();
+    std::vector<edge> forest(cnt);
+    for (uint64_t i = 0; i < cnt; i++) {
+      const uint32_t e = ids[i];
+      forest[i] = edge{g.u[e], g.v[e], g.w[e]};
+    }
+    const uint64_t w = report_forest(forest.data(), cnt, result_filename);
+    std::printf("pbbs execution time %lld microseconds\n", us);
+    std::printf("Total weight %llu\n", (unsigned long long)w);
     phem_graph_file_free(&g);
     return;
   }
-  std::vector<edge> forest;
-  if (use_streamline || use_cpu_gpu_streamline) {
-    const uint64_t cs = chunk_size ? chunk_size : (m ? m : 1);
-    std::printf("streamline chunk boruvka, chunk size %llu\n", (unsigned long long)cs);
-    phem_stream_t *st = nullptr;
-    die_phem(phem_stream_create(n, cs, PHEM_W_U32, &st), "phem_stream_create");
-    uint32_t *d = nullptr;
-    die_hip(hipMalloc(&d, 12 * cs), "hipMalloc (chunk)");
-    auto apply = [&](const uint32_t *hu, const uint32_t *hv, const uint32_t *hw,
-                     uint64_t k) {
-      die_hip(hipMemcpy(d, hu, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
-      die_hip(hipMemcpy(d + cs, hv, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
-      die_hip(hipMemcpy(d + 2 * cs, hw, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
-      die_phem(phem_stream_apply(st, d, d + cs, d + 2 * cs, k, nullptr),
-               "phem_stream_apply");
-    };
-    const uint64_t nchunks = m ? (m + cs - 1) / cs : 0;
-    const uint64_t last = nchunks ? (nchunks - 1) * cs : 0;  // first edge of the last chunk
-    // heterogeneous: the CPU reduces the last chunk to its own forest while
-    // the GPU folds in the others (the first 10^8 edges of the chunk, as the
-    // reference caps its CPU share; the rest stays unfiltered)
-    std::vector<uint32_t> cpu_ids;
-    uint64_t cpu_cnt = 0, cpu_span = 0;
-    std::thread cpu;
-    if (use_cpu_gpu_streamline && nchunks > 1) {
-      cpu_span = std::min<uint64_t>(m - last, 100000000ull);
-      cpu_ids.resize(n ? n : 1);
-      cpu = std::thread([&] {
-        const unsigned hw = std::thread::hardware_concurrency();
-        die_phem(phem_cpu_msf(n, cpu_span, g.u + last, g.v + last, g.w + last,
-                              PHEM_W_U32, cpu_ids.data(), &cpu_cnt,
-                              (int)std::min(hw ? hw : 1u, 16u)),
-                 "phem_cpu_msf");
-      });
-    }
-    for (uint64_t c = 0; c < nchunks; c++) {
-      const uint64_t off = c * cs, k = m - off < cs ? m - off : cs;
-      if (c + 1 < nchunks || !cpu.joinable()) {
-        apply(g.u + off, g.v + off, g.w + off, k);
-        continue;
-      }
-      cpu.join();
-      std::vector<uint32_t> fu, fv, fw;  // filtered chunk, input order kept
-      for (uint64_t i = 0; i < cpu_cnt; i++) {
-        const uint64_t e = last + cpu_ids[i];
-        fu.push_back(g.u[e]);
-        fv.push_back(g.v[e]);
-        fw.push_back(g.w[e]);
-      }
-      for (uint64_t e = last + cpu_span; e < m; e++) {
-        fu.push_back(g.u[e]);
-        fv.push_back(g.v[e]);
-        fw.push_back(g.w[e]);
-      }
-      std::printf("%llu cut down to %llu\n", (unsigned long long)cpu_span,
-                  (unsigned long long)cpu_cnt);
-      apply(fu.data(), fv.data(), fw.data(), fu.size());
-    }
-    uint64_t fc = 0;
-    const uint32_t *fu, *fv, *fw;
-    const uint64_t *fg;
-    die_phem(phem_stream_forest(st, &fc, &fu, &fv, &fw, &fg), "phem_stream_forest");
-    std::vector<uint32_t> hu(fc), hv(fc), hw(fc);
-    if (fc) {
-      die_hip(hipMemcpy(hu.data(), fu, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
-      die_hip(hipMemcpy(hv.data(), fv, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
-      die_hip(hipMemcpy(hw.data(), fw, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
-    }
-    forest.resize(fc);
-    for (uint64_t i = 0; i < fc; i++) forest[i] = edge{hu[i], hv[i], hw[i]};
-    die_hip(hipFree(d), "hipFree");
-    die_phem(phem_stream_destroy(st), "phem_stream_destroy");
-    const uint64_t w = report_forest(forest.data(), fc, result_filename);
-    std::printf("final weight by streamline approach %llu\n", (unsigned long long)w);
-  } else {
-    if (use_malloc_managed)
-      std::printf("malloc managed: runs in device memory\n");
-    if (m > PHEM_MAX_EDGES) {
-      std::fprintf(stderr, "new_boruvka: %llu edges beyond the 32-bit id range\n",
-                   (unsigned long long)m);
-      std::exit(EXIT_FAILURE);
-    }
-    std::vector<edge> edges(m ? m : 1);
-    for (uint64_t i = 0; i < m; i++) edges[i] = edge{g.u[i], g.v[i], g.w[i]};
-    forest.resize(n ? n : 1);
-    vertex filled = 0;
-    std::printf("monolithic GPU approach\n");
-    boruvka_for_edge_array(edges.data(), n, (vertex)m, forest.data(), filled,
-                           result_filename);
-    const uint64_t w = report_forest(forest.data(), filled, result_filename);
-    std::printf("Total weight %llu\n", (unsigned long long)w);
+  if (m > PHEM_MAX_EDGES) {
+    std::fprintf(stderr, "new_boruvka: %llu edges beyond the 32-bit id range\n",
+                 (unsigned long long)m);
+    std::exit(EXIT_FAILURE);
   }
+  std::vector<edge> edges(m ? m : 1);
+  for (uint64_t i = 0; i < m; i++) edges[i] = edge{g.u[i], g.v[i], g.w[i]};
+  std::vector<edge> forest(n ? n : 1);
+  vertex filled = 0;
+  std::printf("monolithic GPU approach\n");
+  boruvka_for_edge_array(edges.data(), n, (vertex)m, forest.data(), filled,
+                         result_filename);
+  const uint64_t w = report_forest(forest.data(), filled, result_filename);
+  std::printf("Total weight %llu\n", (unsigned long long)w);
   phem_graph_file_free(&g);
 }
 

@@ -68,6 +68,37 @@ int mst_solve_tail(uint32_t n, uint64_t m, const uint32_t *d_u,
                    uint32_t *d_labels, const phem_mst_opts *opts,
                    phem_mst_stats *stats, hipStream_t s, const MstTail &tail);
 
+// Pinned host buffers shared by every thread of the process: a free list
+// that short-lived threads (server workers) draw from and return to, so no
+// thread keeps a buffer of its own after it ends.  Buffers are never freed;
+// their number is bounded by the solves that ever ran at the same time.
+void *pinned_acquire(size_t bytes);  // nullptr if hipHostMalloc fails
+void pinned_release(void *p);
+
+// A pinned buffer held for one scope.  `s` is the stream whose pending
+// copies may still read or write it: the release waits for that stream
+// unless done() marked it idle (after the caller's own synchronisation).
+class PinnedLease {
+ public:
+  PinnedLease(size_t bytes, hipStream_t s)
+      : p_(pinned_acquire(bytes)), s_(s) {}
+  ~PinnedLease() {
+    if (!p_) return;
+    if (!idle_) (void)hipStreamSynchronize(s_);
+    pinned_release(p_);
+  }
+  PinnedLease(const PinnedLease &) = delete;
+  PinnedLease &operator=(const PinnedLease &) = delete;
+  void *get() const { return p_; }
+  void busy() { idle_ = false; }  // a copy into or out of it was enqueued
+  void done() { idle_ = true; }   // the stream has been synchronised since
+
+ private:
+  void *p_;
+  hipStream_t s_;
+  bool idle_ = false;
+};
+
 // Records HIP events around the launches in its scope when profiling is on.
 class ProfScope {
  public:

@@ -321,8 +321,10 @@ extern "C" int phem_read_ecl(const char *path, phem_graph_file *g) {
   int64_t V, E;
   std::memcpy(&V, mp.p, 8);
   std::memcpy(&E, mp.p + 8, 8);
-  if (V < 0 || E < 0 || (uint64_t)V > PHEM_MAX_VERTICES ||
-      mp.size != 16 + 8 * ((uint64_t)V + 1) + 4 * (uint64_t)E)
+  // sizes compared without multiplying E, so no E wraps to a matching size
+  const uint64_t head = 16 + 8 * ((uint64_t)V + 1);  // V <= 2^32 here
+  if (V < 0 || E < 0 || (uint64_t)V > PHEM_MAX_VERTICES || mp.size < head ||
+      (mp.size - head) % 4 != 0 || (mp.size - head) / 4 != (uint64_t)E)
     return set_err(PHEM_EINVAL, "%s: ECL sizes (V=%lld, E=%lld) do not match "
                    "the file size %zu", path, (long long)V, (long long)E, mp.size);
   const char *offp = mp.p + 16, *nbp = offp + 8 * ((uint64_t)V + 1);

@@ -52,6 +52,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <memory>
 #include <new>
 #include <utility>
 
@@ -1256,7 +1257,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                           phem_mst_stats *stats, hipStream_t s,
                           const GraphGroups *grp = nullptr,
                           const MstTail *tail = nullptr,
-                          const FixedRounds *fr = nullptr) {
+                          const FixedRounds *fr = nullptr,
+                          ull *host_sums = nullptr) {
   const bool groups = grp && grp->k > 1;
   if (m > PHEM_MAX_EDGES || n > PHEM_MAX_VERTICES)
     return set_err(PHEM_EINVAL, "n=%u m=%llu beyond limits", n,
@@ -1308,17 +1310,26 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   }
   PHEM_LAUNCHED("k_mst_init");
 
-  // Counter values for the host: summed on the device, copied to a pinned
-  // buffer (one per host thread, kept for the life of the process).
-  static thread_local ull *hpin = nullptr;
+  // Counter values for the host: summed on the device, copied to host_sums
+  // (the caller's host memory) or to a pinned buffer of the shared pool held
+  // until the solve returns.
+  std::unique_ptr<PinnedLease> lease;
+  ull *hpin = host_sums;
   auto read_sum_req = [&](const SumReq &q, ull *out) -> int {
-    if (!hpin) PHEM_HIP(hipHostMalloc((void **)&hpin, MAX_SUMS * sizeof(ull)));
+    if (!hpin) {
+      lease.reset(new (std::nothrow) PinnedLease(MAX_SUMS * sizeof(ull), s));
+      if (!lease || !lease->get())
+        return set_err(PHEM_ENOMEM, "pinned counter buffer");
+      hpin = (ull *)lease->get();
+    }
     hipLaunchKernelGGL(k_seg_sums, dim3(1), dim3(WAVE), 0, s, ctr, q,
                        ctr + C_SUMS);
     PHEM_LAUNCHED("k_seg_sums");
+    if (lease) lease->busy();
     PHEM_HIP(hipMemcpyAsync(hpin, ctr + C_SUMS, q.n * sizeof(ull),
                             hipMemcpyDeviceToHost, s));
     PHEM_HIP(hipStreamSynchronize(s));
+    if (lease) lease->done();  // no copy into it is pending
     for (int i = 0; i < q.n; i++) out[i] = hpin[i];
     return PHEM_OK;
   };
@@ -1935,3 +1946,4 @@ extern "C" int phem_mst_graph_destroy(phem_mst_graph_t *g) {
   mst_graph_free(g);
   return PHEM_OK;
 }
+

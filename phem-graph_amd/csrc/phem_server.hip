@@ -40,6 +40,7 @@ struct Req {
   phem_result res;
   uint64_t ticket;
   bool done = false;
+  bool claimed = false;  // a phem_server_wait is waiting for it
 };
 
 struct Slot {
@@ -291,10 +292,18 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
     (void)hipStreamSynchronize(S.stream);
   }
   const double t1 = now_s();
+  // the failure's message lives in this worker thread's error slot: each
+  // request of the batch carries a copy to its client
+  char err[sizeof(((phem_result *)nullptr)->error)] = "";
+  if (rc != PHEM_OK) {
+    std::strncpy(err, phem_last_error(), sizeof(err) - 1);
+    err[sizeof(err) - 1] = 0;
+  }
   std::lock_guard<std::mutex> lk(mu);
   for (uint32_t g = 0; g < k; g++) {
     Req *r = batch[g];
     r->res.status = rc;
+    std::memcpy(r->res.error, err, sizeof(err));
     r->res.forest_count = rc == PHEM_OK ? h_cnt[g] : 0;
     r->res.weight_fx = rc == PHEM_OK ? h_wfx[g] : 0;
     r->res.weight_exact = rc == PHEM_OK && h_bad[g] == 0;
@@ -427,9 +436,14 @@ extern "C" int phem_server_wait(phem_server *srv, uint64_t ticket,
   if (it == srv->reqs.end())
     return set_err(PHEM_ENOTFOUND, "unknown ticket %llu", (unsigned long long)ticket);
   Req *r = it->second.get();
+  if (r->claimed)
+    return set_err(PHEM_EBUSY, "ticket %llu is already being waited for",
+                   (unsigned long long)ticket);
+  // claimed: a poll or second wait meanwhile gets PHEM_EBUSY and leaves it
+  r->claimed = true;
   srv->cv_done.wait(lk, [&] { return r->done; });
   if (res) *res = r->res;
-  srv->reqs.erase(it);
+  srv->reqs.erase(ticket);  // by key: other submissions may have rehashed
   return PHEM_OK;
 }
 
@@ -440,6 +454,9 @@ extern "C" int phem_server_poll(phem_server *srv, uint64_t ticket,
   auto it = srv->reqs.find(ticket);
   if (it == srv->reqs.end())
     return set_err(PHEM_ENOTFOUND, "unknown ticket %llu", (unsigned long long)ticket);
+  if (it->second->claimed)
+    return set_err(PHEM_EBUSY, "ticket %llu is being waited for",
+                   (unsigned long long)ticket);
   *done = it->second->done ? 1 : 0;
   if (*done) {
     if (res) *res = it->second->res;

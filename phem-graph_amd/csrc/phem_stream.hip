@@ -39,6 +39,9 @@ struct phem_stream {
   unsigned long long *cc_forest = nullptr;
   unsigned long long *packed = nullptr;  // batch records (u << 32) | v
   phem_mst_stats last{};
+  // an apply failed after the batch's union-find was enqueued: the
+  // union-find state holds the batch, the forest does not
+  bool broken = false;
 };
 
 namespace phem {
@@ -163,6 +166,8 @@ extern "C" int phem_stream_apply(phem_stream_t *s, const uint32_t *d_u,
                                  const uint32_t *d_v, const uint32_t *d_w,
                                  uint64_t mb, void *stream) {
   if (!s) return set_err(PHEM_EINVAL, "stream handle is NULL");
+  if (s->broken)
+    return set_err(PHEM_EINVAL, "stream is unusable after a failed apply");
   if (mb > s->max_batch)
     return set_err(PHEM_EINVAL, "batch of %llu edges exceeds max_batch %llu",
                    (unsigned long long)mb, (unsigned long long)s->max_batch);
@@ -180,11 +185,6 @@ extern "C" int phem_stream_apply(phem_stream_t *s, const uint32_t *d_u,
   }
   PHEM_LAUNCHED("k_stream_stage");
   int rc;
-  if (mb) {
-    rc = phem_cc_union(s->n, (const uint64_t *)s->packed, mb, s->parent,
-                       (uint64_t *)s->cc_forest, stream);
-    if (rc) return rc;
-  }
   // The new forest (at most n - 1 edges) is gathered before the solve's
   // final counter read, so one wait covers the whole batch.
   const uint64_t total = s->f + mb;
@@ -199,10 +199,27 @@ extern "C" int phem_stream_apply(phem_stream_t *s, const uint32_t *d_u,
     PHEM_LAUNCHED("k_stream_keep");
     return PHEM_OK;
   };
+  // The batch's union-find is enqueued after the solve's kernels and before
+  // its final counter read (in the tail), so a solve that fails before that
+  // read leaves parent and union-find forest as they were.
+  bool unioned = false;
+  const MstTail tail = [&](const ull *d_count) -> int {
+    int trc = keep(d_count);
+    if (trc || !mb) return trc;
+    unioned = true;
+    return phem_cc_union(s->n, (const uint64_t *)s->packed, mb, s->parent,
+                         (uint64_t *)s->cc_forest, stream);
+  };
+  phem_mst_stats last{};
   rc = mst_solve_tail(s->n, total, s->u[c], s->v[c], s->w[c], s->wmode, s->ws,
-                      s->ws_bytes, s->ids, s->labels, nullptr, &s->last, st,
-                      keep);
-  if (rc) return rc;
+                      s->ws_bytes, s->ids, s->labels, nullptr, &last, st,
+                      tail);
+  if (rc) {
+    // once the union is enqueued the two states disagree: refuse later use
+    if (unioned) s->broken = true;
+    return rc;
+  }
+  s->last = last;
   const uint64_t count = s->last.mst_count;
   PHEM_HIP(hipStreamSynchronize(st));
   s->cur ^= 1;
@@ -305,6 +322,8 @@ extern "C" int phem_stream_snapshot_bytes(const phem_stream_t *s,
 extern "C" int phem_stream_save(const phem_stream_t *s, void *host,
                                 uint64_t bytes, void *stream) {
   if (!s || !host) return set_err(PHEM_EINVAL, "stream handle or buffer is NULL");
+  if (s->broken)
+    return set_err(PHEM_EINVAL, "stream is unusable after a failed apply");
   const uint64_t need = snap_bytes(s->n, s->f);
   if (bytes < need)
     return set_err(PHEM_EINVAL, "snapshot buffer of %llu B < %llu B",
@@ -357,12 +376,17 @@ extern "C" int phem_stream_restore(const void *host, uint64_t bytes,
   {
     const uint32_t *su = (const uint32_t *)p, *sv = su + f;
     const char *gp = p + 12 * f;
+    // the forest is kept in ascending global-id order (the tie order of
+    // later batches depends on it): ids must strictly increase
+    uint64_t prev = 0;
     for (uint64_t i = 0; i < f; i++) {
       uint64_t g;
       std::memcpy(&g, gp + 8 * i, 8);
-      if (su[i] >= n || sv[i] >= n || su[i] == sv[i] || g >= h.next_gid)
+      if (su[i] >= n || sv[i] >= n || su[i] == sv[i] || g >= h.next_gid ||
+          (i > 0 && g <= prev))
         return set_err(PHEM_EINVAL, "snapshot forest edge %llu is invalid",
                        (unsigned long long)i);
+      prev = g;
     }
     const char *pp = p + 20 * f, *fp = pp + 4 * n;
     for (uint64_t x = 0; x < n; x++) {

@@ -82,10 +82,13 @@ class Result(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("forest_count", ctypes.c_uint32),
                 ("weight_fx", ctypes.c_uint64), ("weight_exact", ctypes.c_int32),
                 ("batch_graphs", ctypes.c_uint32), ("arrival", ctypes.c_double),
-                ("start", ctypes.c_double), ("done", ctypes.c_double)]
+                ("start", ctypes.c_double), ("done", ctypes.c_double),
+                ("error", ctypes.c_char * 160)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["error"] = self.error.decode(errors="replace")
+        return d
 
 
 class ServerStats(ctypes.Structure):

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <thrust/device_ptr.h>
 
+#include "phem.h"
 #include "phem_dropin.h"
 
 extern "C" int harness_boruvka_for_edge_array(const uint32_t *h_aos, uint32_t n,
@@ -45,14 +46,36 @@ extern "C" int harness_device_boruvka(const uint32_t *h_aos, uint32_t n,
 }
 
 // new_boruvka on a MatrixMarket file; mode 0 plain, 1 streamline,
-// 2 heterogeneous streamline, 3 managed.
+// 2 heterogeneous streamline, 3 managed, 4 cpu only.
 extern "C" int harness_new_boruvka(const char *file, const char *result,
                                    uint32_t chunk_size, int num_chunks,
-                                   int generate_weights, int mode) {
-  new_boruvka(file, result, false, chunk_size, num_chunks, generate_weights != 0,
-              mode == 3, mode == 1, false, mode == 2, false);
+                                   int generate_weights, int mode, int debug) {
+  new_boruvka(file, result, debug != 0, chunk_size, num_chunks,
+              generate_weights != 0, mode == 3, mode == 1, mode == 4, mode == 2,
+              false);
   return 0;
 }
+
+// A caller's CPU MST registered with phem_set_cpu_mst: the library's host
+// solver, ids handed back in descending order, calls counted.
+static int g_cpu_calls = 0;
+static int cpu_mst_reversed(uint32_t n, uint64_t m, const uint32_t *u,
+                            const uint32_t *v, const uint32_t *w, uint32_t *ids,
+                            uint64_t *count) {
+  g_cpu_calls++;
+  int rc = phem_cpu_msf(n, m, u, v, w, PHEM_W_U32, ids, count, 2);
+  for (uint64_t i = 0, j = *count; i + 1 < j; i++, j--) {
+    uint32_t t = ids[i];
+    ids[i] = ids[j - 1];
+    ids[j - 1] = t;
+  }
+  return rc;
+}
+extern "C" void harness_register_cpu_mst(int on) {
+  phem_set_cpu_mst(on ? cpu_mst_reversed : nullptr);
+}
+extern "C" int harness_cpu_mst_calls() { return g_cpu_calls; }
+extern "C" void harness_release_workspace() { phem_dropin_release_workspace(); }
 
 extern "C" int harness_graph_info(const char *file, uint32_t *n, uint64_t *m) {
   std::ifstream f(file);

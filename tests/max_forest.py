@@ -6,14 +6,10 @@ import oracle as O
 
 def max_forest_oracle(n, u, v, w, wmode):
     """Maximum spanning forest under (weight, id) descending, by reduction to
-    the oracle's minimum forest: reversing the edge order and the weight
-    order turns (w, id) descending into (w', id') ascending."""
+    the oracle's minimum forest: reversing the edge order and the order key
+    turns (w, id) descending into (key', id') ascending; the reversed keys go
+    in as unsigned weights, whose order key is the identity."""
     m = len(u)
-    kr = ~O.wkey(w, wmode)[::-1]
-    if wmode == 0:
-        wr = kr
-    else:  # float bit patterns whose order key is kr
-        wr = np.where(kr & np.uint32(0x80000000), kr & np.uint32(0x7FFFFFFF), ~kr)
-    ids, lab = O.kruskal(n, u[::-1].copy(), v[::-1].copy(), wr.astype(np.uint32), wmode)
+    kr = (~O.wkey(w, wmode)[::-1]).astype(np.uint32)
+    ids, lab = O.kruskal(n, u[::-1].copy(), v[::-1].copy(), kr, 0)
     return np.sort((m - 1 - ids.astype(np.int64))).astype(np.uint32), lab
-

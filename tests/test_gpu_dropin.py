@@ -125,7 +125,7 @@ def test_reference_cc_driver_over_dropin(HAST_LIB, name, share, batch):
 
 def _new_boruvka_lib(H):
     H.harness_new_boruvka.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint32,
-                                      ctypes.c_int, ctypes.c_int, ctypes.c_int]
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     H.harness_graph_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32),
                                      ctypes.POINTER(ctypes.c_uint64)]
     H.harness_mmap_populate.argtypes = [ctypes.c_char_p, _u32p, ctypes.c_uint32,
@@ -143,15 +143,17 @@ def _mtx(tmp_path, n, a, b, w=None):
     return str(p)
 
 
-@pytest.mark.parametrize("mode,chunk,nchunks", [(0, 0, 0), (1, 7000, 0), (1, 0, 5),
-                                                (2, 0, 3), (2, 0, 1), (2, 9000, 0),
-                                                (3, 0, 0)])
+@pytest.mark.parametrize("mode,chunk,nchunks,debug", [
+    (0, 0, 0, 0), (1, 7000, 0, 0), (1, 0, 5, 0), (2, 0, 3, 0), (2, 0, 1, 0),
+    (2, 9000, 0, 0), (3, 0, 0, 0), (4, 0, 0, 0), (0, 0, 0, 1), (1, 0, 4, 1),
+    (2, 0, 3, 1)])
 @pytest.mark.parametrize("gen", [0, 1])
-def test_new_boruvka_matrix_market(H, tmp_path, mode, chunk, nchunks, gen):
+def test_new_boruvka_matrix_market(H, tmp_path, mode, chunk, nchunks, debug, gen):
     """The reference's MST driver on a MatrixMarket file: the forest written
     to the result file (u+1 v+1 w, ascending input order) equals the oracle's
     Kruskal forest of the graph the reader loaded, with integer weights (many
-    ties) or the generated-weight rule, whole or in chunks."""
+    ties) or the generated-weight rule, whole or in chunks, on the GPU, on the
+    CPU (mode 4) and in debug mode (the streamline paths repeated 10 times)."""
     import phem_native as P
     L = _new_boruvka_lib(H)
     rng = np.random.default_rng(3 + gen)
@@ -163,8 +165,10 @@ def test_new_boruvka_matrix_market(H, tmp_path, mode, chunk, nchunks, gen):
     g = P.read_mtx(path, generate_weights=bool(gen))
     ids, lab = O.kruskal(g["n"], g["u"], g["v"], g["w"], 0)
     res = str(tmp_path / "forest.txt")
-    print(f"new_boruvka mode={mode} chunk={chunk} nchunks={nchunks} gen={gen}", flush=True)
-    assert L.harness_new_boruvka(path.encode(), res.encode(), chunk, nchunks, gen, mode) == 0
+    print(f"new_boruvka mode={mode} chunk={chunk} nchunks={nchunks} gen={gen} "
+          f"debug={debug}", flush=True)
+    assert L.harness_new_boruvka(path.encode(), res.encode(), chunk, nchunks, gen, mode,
+                                 debug) == 0
     got = np.loadtxt(res, dtype=np.uint64, ndmin=2)
     want = np.stack([g["u"][ids] + 1, g["v"][ids] + 1, g["w"][ids]], axis=1).astype(np.uint64)
     assert got.shape == want.shape and np.array_equal(got, want)
@@ -175,3 +179,21 @@ def test_new_boruvka_matrix_market(H, tmp_path, mode, chunk, nchunks, gen):
     aos = np.zeros(3 * m, np.uint32)
     L.harness_mmap_populate(path.encode(), aos, m, n, gen)
     assert np.array_equal(aos.reshape(-1, 3), np.stack([g["u"], g["v"], g["w"]], axis=1))
+
+
+def test_device_boruvka_after_workspace_release(H):
+    """device_boruvka still gives the oracle's forest after its cached
+    workspace is released, and the release is repeatable."""
+    n, u, v, w, wmode = cases.make("rmat_small")
+    aos = np.ascontiguousarray(np.stack([u, v, w], axis=1).reshape(-1), np.uint32)
+    ids, rec, lab = expected_records(n, u, v, w, 0)
+    for _ in range(2):
+        H.harness_release_workspace()
+        out = np.zeros(3 * n, np.uint32)
+        labels = np.zeros(n, np.uint32)
+        filled = np.array([0], np.uint32)
+        m_after = np.array([7], np.uint32)
+        assert H.harness_device_boruvka(aos, n, len(u), out, labels, filled, m_after) == 0
+        assert filled[0] == len(ids)
+        assert np.array_equal(out[: 3 * len(ids)].reshape(-1, 3), rec)
+        assert np.array_equal(labels, lab)

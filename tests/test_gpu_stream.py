@@ -217,6 +217,7 @@ def test_stream_restore_rejects_bad_snapshots(P):
     st = P.Stream(1000, 100)
     u, v, w = S.uniform(1000, 100, 3)
     st.apply(dev(u), dev(v), dev(w))
+    f = st.stats()["mst_count"]
     snap = st.save()
     st.close()
     bad = snap.copy()
@@ -226,7 +227,15 @@ def test_stream_restore_rejects_bad_snapshots(P):
     # the union-find forest record of vertex n - 2 (the second-last 8 bytes):
     # u = 5 > v = 2 is no record the library writes
     corrupt[-16:-8] = np.array([(5 << 32) | 2], np.uint64).view(np.uint8)
-    for b in (bad, truncated, corrupt, snap[:10]):
+    # forest global ids out of order (the first two swapped): the tie order
+    # of later batches depends on ascending ids
+    assert f >= 2
+    head = len(snap) - 20 * f - 12 * 1000
+    g0 = head + 12 * f
+    unordered = snap.copy()
+    unordered[g0:g0 + 8], unordered[g0 + 8:g0 + 16] = snap[g0 + 8:g0 + 16], snap[g0:g0 + 8]
+    assert not np.array_equal(unordered, snap)
+    for b in (bad, truncated, corrupt, unordered, snap[:10]):
         with pytest.raises(P.PhemError) as e:
             P.Stream.restore(b, 100)
         assert e.value.code == P.EINVAL

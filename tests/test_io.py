@@ -82,6 +82,14 @@ def test_ecl_fixture(tmp_path):
     with pytest.raises(P.PhemError) as e:
         P.read_ecl(bad)
     assert e.value.code == P.EINVAL
+    # E = 2^62 makes 4 E wrap to 0 in 64 bits: a file of header and offsets
+    # only must still be refused (its offsets up to E would read past it)
+    huge = 1 << 62
+    wrap = struct.pack("<qq", 4, huge) + struct.pack("<5q", 0, 1, 2, 3, huge)
+    for name, d in (("wrap.egr", wrap), ("wrap1.egr", wrap + b"\0\0")):
+        with pytest.raises(P.PhemError) as e:
+            P.read_ecl(write(tmp_path, name, d, "wb"))
+        assert e.value.code == P.EINVAL
 
 
 def test_pbbs_weighted_fixture(tmp_path):

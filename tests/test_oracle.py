@@ -135,7 +135,7 @@ def test_oracle_labels_agree(name):
 @pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (no reference tree)")
 @pytest.mark.parametrize("name", ["uniform_c1", "rmat_small", "grid_small", "path_chords_small",
                                   "duplicates_equal_w", "equal_weight_path", "star",
-                                  "many_components"])
+                                  "many_components", "signed_floats"])
 def test_oracle_vs_reference_cpu_builds(name):
     n, u, v, w, wmode = cases.make(name)
     assert wmode == 1
@@ -213,15 +213,69 @@ def test_boruvka_partial_state(name):
     first = np.unique((best[best != np.iinfo(np.uint64).max] &
                        np.uint64(0xFFFFFFFF)).astype(np.uint32))
     prev = np.zeros(0, np.uint32)
-    for r in range(1, full["rounds"] + 2):
-        ids, lab = O.boruvka_rounds(n, u, v, w, wmode, r)
-        if r == 1:
-            assert np.array_equal(ids, first)
-        assert np.isin(prev, ids).all() and np.isin(ids, full["ids"]).all()
-        assert np.array_equal(lab, O.bfs_labels(n, u[ids], v[ids]))
-        prev = ids
+    # the round log follows from the pinned partial labels: round r starts
+    # with the valid edges joining two components of the first r rounds'
+    # forest (m_r) and the components those edges touch (c_r)
+    uu, vv = u[valid], v[valid]
+    lab = np.arange(n, dtype=np.uint32)
+    for r in range(0, full["rounds"] + 2):
+        if r > 0:
+            ids, lab = O.boruvka_rounds(n, u, v, w, wmode, r)
+            if r == 1:
+                assert np.array_equal(ids, first)
+            assert np.isin(prev, ids).all() and np.isin(ids, full["ids"]).all()
+            assert np.array_equal(lab, O.bfs_labels(n, u[ids], v[ids]))
+            prev = ids
+        alive = lab[uu] != lab[vv]
+        m_r = int(alive.sum())
+        c_r = len(np.unique(np.concatenate([lab[uu[alive]], lab[vv[alive]]])))
+        if r < full["rounds"]:
+            assert (m_r, c_r) == (int(full["log_m"][r]), int(full["log_c"][r])), r
+        else:
+            assert m_r == 0
     assert np.array_equal(prev, full["ids"])
     assert np.array_equal(lab, full["labels"])
+
+
+def test_float_order_signed_zero_and_nan():
+    """The float order is the reference comparator's numeric one
+    ((a.w < b.w) || (a.w == b.w && a.id < b.id), MST/src/utils.cu:801-802):
+    -0.0 and +0.0 tie and the smaller id wins.  NaN, which that comparator
+    leaves unordered, is heavier than +inf.  Expected forests come from
+    Kruskal over Python's own float comparisons, not from the oracle's key."""
+    import math
+    # two parallel edges: the tie between the zeros goes to the smaller id
+    for w in ([-0.0, 0.0], [0.0, -0.0]):
+        ww = np.array(w, np.float32).view(np.uint32)
+        ids, _ = O.kruskal(2, np.array([0, 0], np.uint32), np.array([1, 1], np.uint32), ww)
+        assert ids.tolist() == [0]
+    rng = np.random.default_rng(17)
+    pool = np.array([-1.5, -0.0, 0.0, 0.5, np.nan, np.inf, -np.inf, -np.nan], np.float32)
+    for t in range(200):
+        n = int(rng.integers(2, 9))
+        m = int(rng.integers(1, 16))
+        u = rng.integers(0, n, m).astype(np.uint32)
+        v = rng.integers(0, n, m).astype(np.uint32)
+        wf = pool[rng.integers(0, len(pool), m)]
+        order = sorted(range(m), key=lambda i: (math.isnan(float(wf[i])),
+                                                0.0 if math.isnan(float(wf[i])) else float(wf[i]),
+                                                i))
+        par = list(range(n))
+
+        def f(x):
+            while par[x] != x:
+                x = par[x]
+            return x
+        want = []
+        for i in order:
+            a, b = f(int(u[i])), f(int(v[i]))
+            if u[i] != v[i] and a != b:
+                par[a] = b
+                want.append(i)
+        ww = wf.view(np.uint32)
+        assert O.kruskal(n, u, v, ww)[0].tolist() == sorted(want)
+        assert O.boruvka(n, u, v, ww)["ids"].tolist() == sorted(want)
+        assert O.par_mst(n, u, v, ww, threads=2).tolist() == sorted(want)
 
 
 def test_weight_sum_exact():
