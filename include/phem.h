@@ -163,6 +163,22 @@ int phem_mst_solve_aos(uint32_t n, uint64_t m, const uint32_t *d_edges,
                        const phem_mst_opts *opts, phem_mst_stats *stats,
                        void *stream);
 
+/* phem_mst_solve with no allocation of any kind (no counterpart in the
+ * reference: for callers that cannot allocate, such as code under HIP graph
+ * capture, tight serving loops and harnesses that forbid hipMalloc).  Same
+ * arguments and results as phem_mst_solve, without d_ws / ws_bytes: the
+ * workspace is a static device array of 64 MiB (67108864 B) per device,
+ * held from the first call on that device until the process ends.  Graphs
+ * whose phem_mst_workspace_bytes exceeds it (for instance n = 2^17,
+ * m = 2^21; n = 2^17, m = 2^20 fits) return PHEM_ENOMEM.  Every call on one
+ * device uses the same array: calls that overlap on one device (other
+ * streams or threads) race and must be serialised by the caller. */
+int phem_mst_solve_noalloc(uint32_t n, uint64_t m, const uint32_t *d_u,
+                           const uint32_t *d_v, const uint32_t *d_w, int wmode,
+                           uint32_t *d_mst_ids, uint32_t *d_labels,
+                           const phem_mst_opts *opts, phem_mst_stats *stats,
+                           void *stream);
+
 /* A whole phem_mst_solve recorded once into a HIP graph and replayed.
  * Replaces the host-driven round loop of device_boruvka
  * (MST/src/kernels.cu:526-689: a count_if / remove_if and a host read of

@@ -53,6 +53,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <utility>
 
@@ -1947,3 +1948,67 @@ extern "C" int phem_mst_graph_destroy(phem_mst_graph_t *g) {
   return PHEM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// phem_mst_solve without any allocation: the workspace is a file-scope
+// device array of NOALLOC_BYTES, one per device that runs it, held until
+// the process ends.  Its address on each device comes from
+// hipGetSymbolAddress on the first call there (the array's name in host
+// code is not a device address); kernels get that address, as they get the
+// caller's workspace in phem_mst_solve.  The solve initialises every part of
+// the workspace it reads (k_mst_init clears the counters, parents, minima
+// and forest bitmap), so the previous call's contents do not matter; calls
+// that overlap on one device share the array and race.  Counter values come
+// back through host memory on the stack, not a pinned buffer.
+
+namespace phem {
+
+constexpr uint64_t NOALLOC_BYTES = 64ull << 20;
+static_assert(NOALLOC_BYTES % 256 == 0, "scratch size");
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64];  // device address of `scratch`, per device
+}  // namespace
+
+}  // namespace phem
+
+extern "C" int phem_mst_solve_noalloc(uint32_t n, uint64_t m,
+                                      const uint32_t *d_u, const uint32_t *d_v,
+                                      const uint32_t *d_w, int wmode,
+                                      uint32_t *d_mst_ids, uint32_t *d_labels,
+                                      const phem_mst_opts *opts,
+                                      phem_mst_stats *stats, void *stream) {
+  if (m > PHEM_MAX_EDGES || n > PHEM_MAX_VERTICES)
+    return set_err(PHEM_EINVAL, "n=%u m=%llu beyond limits", n,
+                   (unsigned long long)m);
+  const uint64_t need = mst_layout(n, m).total;
+  if (need > NOALLOC_BYTES)
+    return set_err(PHEM_ENOMEM,
+                   "n=%u m=%llu needs %llu B of workspace, beyond the %llu B "
+                   "static scratch of phem_mst_solve_noalloc (use "
+                   "phem_mst_solve with a workspace)",
+                   n, (unsigned long long)m, (unsigned long long)need,
+                   (unsigned long long)NOALLOC_BYTES);
+  if (m && (!d_u || !d_v || !d_w))
+    return set_err(PHEM_EINVAL, "edge arrays are NULL");
+  int dev = 0;
+  PHEM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(PHEM_EINVAL, "device %d beyond 64", dev);
+  void *ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void *p = nullptr;
+      PHEM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      if (!p) return set_err(PHEM_EHIP, "scratch has no device address");
+      g_scratch_ptr[dev] = p;
+    }
+    ws = g_scratch_ptr[dev];
+  }
+  ull sums[MAX_SUMS];
+  const bool vec = aligned16(d_u) && aligned16(d_v) && aligned16(d_w);
+  return mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w, vec}, wmode, ws,
+                        NOALLOC_BYTES, d_mst_ids, d_labels, opts, stats,
+                        (hipStream_t)stream, nullptr, nullptr, nullptr, sums);
+}

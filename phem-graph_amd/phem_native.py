@@ -134,6 +134,8 @@ def lib():
         "phem_mst_workspace_bytes": (i32, [u32, u64, c.POINTER(c.c_uint64)]),
         "phem_mst_solve": (i32, [u32, u64, vp, vp, vp, i32, vp, u64, vp, vp,
                                  c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
+        "phem_mst_solve_noalloc": (i32, [u32, u64, vp, vp, vp, i32, vp, vp,
+                                         c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
         "phem_mst_solve_aos": (i32, [u32, u64, vp, i32, vp, u64, vp, vp,
                                      c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
         "phem_mst_graph_create": (i32, [u32, u64, vp, vp, vp, i32, vp, u64, vp, vp,
@@ -345,6 +347,18 @@ class MstGraph:
             self.close()
         except Exception:
             pass
+
+
+def mst_solve_noalloc(n, u, v, w, ids, labels=None, wmode=W_F32, light_factor=None,
+                      maximum=False, stream=None):
+    """phem_mst_solve_noalloc into the caller's device tensors ids (n slots)
+    and labels (n slots or None); returns the stats dict."""
+    st = MstStats()
+    _check(lib().phem_mst_solve_noalloc(n, u.numel(), _p(u), _p(v), _p(w), wmode, _p(ids),
+                                        _p(labels), _opts(light_factor, maximum),
+                                        ctypes.byref(st), _stream(stream)),
+           "phem_mst_solve_noalloc")
+    return st.as_dict()
 
 
 def mst_solve(n, u, v, w, wmode=W_F32, labels=True, light_factor=None, maximum=False):

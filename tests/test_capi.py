@@ -41,3 +41,23 @@ def test_workspace_query_without_device():
                                         ctypes.byref(b)) == -1
     lib.phem_last_error.restype = ctypes.c_char_p
     assert b"beyond limits" in lib.phem_last_error()
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libphem.so not built")
+def test_noalloc_refuses_graphs_beyond_its_scratch():
+    """The size check comes before any device call: a graph whose workspace
+    exceeds the 64 MiB static scratch gets PHEM_ENOMEM with a message."""
+    lib = ctypes.CDLL(LIB)
+    lib.phem_last_error.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    lib.phem_mst_solve_noalloc.argtypes = [ctypes.c_uint32, ctypes.c_uint64, vp, vp, vp,
+                                           ctypes.c_int, vp, vp, vp, vp, vp]
+    fake = vp(0x1000)  # never dereferenced: the call fails before any use
+    rc = lib.phem_mst_solve_noalloc(1 << 17, 1 << 21, fake, fake, fake, 1, fake, None,
+                                    None, None, None)
+    assert rc == -2
+    assert b"static scratch" in lib.phem_last_error()
+    b = ctypes.c_uint64(0)
+    assert lib.phem_mst_workspace_bytes(ctypes.c_uint32(1 << 17), ctypes.c_uint64(1 << 20),
+                                        ctypes.byref(b)) == 0
+    assert b.value <= 64 << 20
