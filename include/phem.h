@@ -56,15 +56,39 @@ extern "C" {
  * environment variables PHEM_BLOCKS_PER_CU, PHEM_MST_LIGHT_FACTOR and
  * PHEM_CC_SAMPLE_FACTOR at first use, then phem_set_tuning.  Set them between
  * solves, not while one runs.  Results do not depend on them: the forest,
- * labels and weights are the same for every setting. */
+ * labels and weights are the same for every setting.  The kernel shapes
+ * select among kernels compiled for each allowed value (the reference fixes
+ * its block size at compile time, BLOCKSIZE 1024, MST/src/utils.cu:12, and
+ * tb_size 1024, CC/ExternalSpanningTree.cu:18); workspace sizes do not
+ * depend on them. */
 typedef struct {
-  uint32_t blocks_per_cu;  /* resident blocks per CU of the looping kernels
-                              (1..64; 0 in phem_set_tuning: unchanged) */
+  uint32_t blocks_per_cu;  /* resident 256-thread blocks per CU of the looping
+                              kernels (1..64; 0 in phem_set_tuning: unchanged;
+                              512-thread blocks count twice) */
   uint32_t reserved;
   double mst_light_factor; /* default light_factor (NaN: unchanged) */
   double cc_sample_factor; /* CC sampling phase: the first f*n records,
                               then a second giant after 3*f*n records
                               (default 1.0; <= 0: one phase; NaN: unchanged) */
+  /* kernel shapes; 0 (NaN for mst_split_ratio) in phem_set_tuning: unchanged */
+  uint32_t mst_block;       /* threads per block of the MST input passes
+                               (light and heavy filter): 256 (default), 512 */
+  uint32_t mst_ept;         /* their edges per thread: 4, 8 (default), 16;
+                               mst_block x mst_ept <= 4096 */
+  uint32_t mst_ept_compact; /* edges per thread of the compaction rounds of
+                               graphs with m >= 2^mst_compact_log2: 8, 16
+                               (default; smaller graphs use 8) */
+  uint32_t mst_compact_log2; /* that edge-count threshold (1..40, default 22) */
+  double mst_split_ratio;   /* the light/heavy split runs when
+                               m > ratio * light_factor * n (default 1.5;
+                               > 1) */
+  uint32_t cc_block;        /* threads per block of the CC unions: 256
+                               (default), 512 */
+  uint32_t cc_ept;          /* records per thread of the unfiltered union:
+                               8, 16 (default), 32 */
+  uint32_t cc_ept_filter;   /* records per thread of the filtered unions:
+                               4, 8 (default), 16 */
+  uint32_t reserved2;
 } phem_tuning;
 int phem_get_tuning(phem_tuning *t);
 int phem_set_tuning(const phem_tuning *t);

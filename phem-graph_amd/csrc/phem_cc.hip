@@ -17,15 +17,13 @@
 
 namespace phem {
 
-constexpr int CBLK = 256;
-// Records per thread per tile: the unfiltered union (every record a union)
-// measured faster with more records in flight per thread (16: 0.72 ms
-// against 0.80 ms for the first n records at n = 2^24), the filtered pass
-// with fewer registers and more resident waves (8: 2.09 ms against 2.46 ms).
-constexpr int CEPT_ALL = 16;
-constexpr int CEPT_FILTER = 8;
-template <bool FILTER>
-constexpr int cept() { return FILTER ? CEPT_FILTER : CEPT_ALL; }
+constexpr int CBLK = 256;  // threads per block of the small passes
+// The unions run with B threads per block and E records per thread per tile
+// (the cc_block, cc_ept and cc_ept_filter knobs).  Built-in shapes: 256
+// threads; 16 records per thread in the unfiltered union (every record a
+// union: more records in flight per thread measured faster, 0.72 ms against
+// 0.80 ms for the first n records at n = 2^24), 8 in the filtered passes
+// (fewer registers and more resident waves: 2.09 ms against 2.46 ms).
 constexpr uint32_t GSUM_CC = 4096;  // giant-summary words in LDS (16 KB)
 
 __global__ __launch_bounds__(CBLK) void k_cc_init(uint32_t n, uint32_t *par,
@@ -48,23 +46,22 @@ __device__ __forceinline__ void cc_edge(ull e, uint32_t n, uint32_t *par,
 
 // FILTER: edges whose endpoints are both in the giant component of the
 // sampling phase (bitmap `giant`) are skipped without touching par.
-template <bool FILTER>
-__global__ __launch_bounds__(CBLK) void k_cc_union(const ull *__restrict__ edges,
-                                                   uint64_t m, uint32_t n,
-                                                   uint32_t *par, ull *forest,
-                                                   const uint32_t *__restrict__ giant,
-                                                   const uint32_t *gsum) {
+template <bool FILTER, int B, int CEPT>
+__global__ __launch_bounds__(B) void k_cc_union(const ull *__restrict__ edges,
+                                                uint64_t m, uint32_t n,
+                                                uint32_t *par, ull *forest,
+                                                const uint32_t *__restrict__ giant,
+                                                const uint32_t *gsum) {
   __shared__ uint32_t s_sum[FILTER ? GSUM_CC : 1];
   uint32_t covered = 0;
-  if (FILTER) covered = gsum_load<CBLK, GSUM_CC>(s_sum, gsum, n);
-  constexpr int CEPT = cept<FILTER>();
-  constexpr int CTILE = CBLK * CEPT;
+  if (FILTER) covered = gsum_load<B, GSUM_CC>(s_sum, gsum, n);
+  constexpr int CTILE = B * CEPT;
   for (uint64_t tile = blockIdx.x; tile * CTILE < m; tile += gridDim.x) {
     const uint64_t base = tile * CTILE + threadIdx.x;
     ull e[CEPT];
 #pragma unroll
     for (int j = 0; j < CEPT; j++) {
-      uint64_t i = base + (uint64_t)j * CBLK;
+      uint64_t i = base + (uint64_t)j * B;
       e[j] = i < m ? __builtin_nontemporal_load(&edges[i]) : KEY_NONE;
     }
     // Fast path: the parents of both endpoints of all edges are loaded
@@ -118,6 +115,38 @@ __global__ __launch_bounds__(CBLK) void k_cc_merge(uint32_t n,
 
 }  // namespace phem
 
+namespace phem {
+
+// Launches the union of records [0, m) of `edges` with the knobs' shape.
+template <bool FILTER>
+static void launch_cc_union(const ull *edges, uint64_t m, uint32_t n,
+                            uint32_t *par, ull *forest, const uint32_t *giant,
+                            const uint32_t *gsum, hipStream_t s) {
+  const KernelShape k = kernel_shape();
+  const uint32_t B = k.cc_block, E = FILTER ? k.cc_ept_filter : k.cc_ept;
+  const unsigned grid = grid_for_threads(m, (uint64_t)B * E, B);
+#define PHEM_CC_UNION(BB, EE)                                                  \
+  if (B == BB && E == EE) {                                                    \
+    hipLaunchKernelGGL((k_cc_union<FILTER, BB, EE>), dim3(grid), dim3(BB), 0, \
+                       s, edges, m, n, par, forest, giant, gsum);              \
+    return;                                                                    \
+  }
+  if constexpr (FILTER) {
+    PHEM_CC_UNION(256, 4) PHEM_CC_UNION(256, 8) PHEM_CC_UNION(256, 16)
+    PHEM_CC_UNION(512, 4) PHEM_CC_UNION(512, 8) PHEM_CC_UNION(512, 16)
+  } else {
+    PHEM_CC_UNION(256, 8) PHEM_CC_UNION(256, 16) PHEM_CC_UNION(256, 32)
+    PHEM_CC_UNION(512, 8) PHEM_CC_UNION(512, 16) PHEM_CC_UNION(512, 32)
+  }
+#undef PHEM_CC_UNION
+  // phem_set_tuning admits no other shape; the built-in one as a backstop
+  hipLaunchKernelGGL((k_cc_union<FILTER, 256, FILTER ? 8 : 16>),
+                     dim3(grid_for(m, 256ull * (FILTER ? 8 : 16))), dim3(256),
+                     0, s, edges, m, n, par, forest, giant, gsum);
+}
+
+}  // namespace phem
+
 using namespace phem;
 
 extern "C" int phem_cc_init(uint32_t n, uint32_t *d_parent, uint64_t *d_forest,
@@ -143,11 +172,8 @@ extern "C" int phem_cc_union(uint32_t n, const uint64_t *d_edges, uint64_t m,
   if (!m || !n) return PHEM_OK;
   {
     PHEM_PROF("k_cc_union", (hipStream_t)stream);
-    hipLaunchKernelGGL(k_cc_union<false>, dim3(grid_for(m, CBLK * cept<false>())),
-                       dim3(CBLK),
-                       0, (hipStream_t)stream, (const ull *)d_edges, m, n,
-                       d_parent, (ull *)d_forest, (const uint32_t *)nullptr,
-                       (const uint32_t *)nullptr);
+    launch_cc_union<false>((const ull *)d_edges, m, n, d_parent, (ull *)d_forest,
+                           nullptr, nullptr, (hipStream_t)stream);
   }
   PHEM_LAUNCHED("k_cc_union");
   return PHEM_OK;
@@ -166,9 +192,6 @@ extern "C" int phem_cc_labels(uint32_t n, const uint32_t *d_parent,
   PHEM_LAUNCHED("k_cc_labels");
   return PHEM_OK;
 }
-
-namespace phem {
-}  // namespace phem
 
 // Sampling phases when m is well above n (the sampling idea of Afforest,
 // Sutton et al., IPDPS 2018): union the first m1 = f*n records (f: the
@@ -231,11 +254,8 @@ extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
     if (a1 <= a0) return PHEM_OK;
     {
       PHEM_PROF("k_cc_union_filtered", s);
-      hipLaunchKernelGGL(k_cc_union<true>,
-                         dim3(grid_for(a1 - a0, CBLK * cept<true>())),
-                         dim3(CBLK), 0, s, (const ull *)d_edges + a0, a1 - a0, n,
-                         d_parent, (ull *)d_forest, (const uint32_t *)bits,
-                         (const uint32_t *)gsum);
+      launch_cc_union<true>((const ull *)d_edges + a0, a1 - a0, n, d_parent,
+                            (ull *)d_forest, bits, gsum, s);
     }
     PHEM_LAUNCHED("k_cc_union_filtered");
     return PHEM_OK;

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <cmath>
+#include <initializer_list>
 #include <limits>
 #include <mutex>
 #include <string>
@@ -22,6 +23,17 @@ std::atomic<unsigned> g_blocks_per_cu{0};  // 0: not initialised
 std::atomic<double> g_light_factor{std::numeric_limits<double>::quiet_NaN()};
 std::atomic<double> g_sample_factor{std::numeric_limits<double>::quiet_NaN()};
 std::once_flag g_tune_once;
+// kernel shapes (built-in defaults: the shapes measured fastest at RMAT
+// n = 2^24, m = 2^28)
+std::atomic<uint32_t> g_mst_block{256}, g_mst_ept{8}, g_mst_ept_compact{16},
+    g_mst_compact_log2{22}, g_cc_block{256}, g_cc_ept{16}, g_cc_ept_filter{8};
+std::atomic<double> g_mst_split_ratio{1.5};
+
+bool one_of(uint32_t x, std::initializer_list<uint32_t> allowed) {
+  for (uint32_t a : allowed)
+    if (x == a) return true;
+  return false;
+}
 
 void tuning_init() {
   std::call_once(g_tune_once, [] {
@@ -67,6 +79,19 @@ double mst_default_light_factor() {
 double cc_default_sample_factor() {
   tuning_init();
   return g_sample_factor.load(std::memory_order_relaxed);
+}
+
+KernelShape kernel_shape() {
+  KernelShape k;
+  k.mst_block = g_mst_block.load(std::memory_order_relaxed);
+  k.mst_ept = g_mst_ept.load(std::memory_order_relaxed);
+  k.mst_ept_compact = g_mst_ept_compact.load(std::memory_order_relaxed);
+  k.mst_compact_log2 = g_mst_compact_log2.load(std::memory_order_relaxed);
+  k.mst_split_ratio = g_mst_split_ratio.load(std::memory_order_relaxed);
+  k.cc_block = g_cc_block.load(std::memory_order_relaxed);
+  k.cc_ept = g_cc_ept.load(std::memory_order_relaxed);
+  k.cc_ept_filter = g_cc_ept_filter.load(std::memory_order_relaxed);
+  return k;
 }
 
 // ---- shared pinned host buffers ---------------------------------------------
@@ -253,14 +278,60 @@ extern "C" int phem_get_tuning(phem_tuning *t) {
   t->reserved = 0;
   t->mst_light_factor = phem::g_light_factor.load();
   t->cc_sample_factor = phem::g_sample_factor.load();
+  const phem::KernelShape k = phem::kernel_shape();
+  t->mst_block = k.mst_block;
+  t->mst_ept = k.mst_ept;
+  t->mst_ept_compact = k.mst_ept_compact;
+  t->mst_compact_log2 = k.mst_compact_log2;
+  t->mst_split_ratio = k.mst_split_ratio;
+  t->cc_block = k.cc_block;
+  t->cc_ept = k.cc_ept;
+  t->cc_ept_filter = k.cc_ept_filter;
+  t->reserved2 = 0;
   return PHEM_OK;
 }
 
 extern "C" int phem_set_tuning(const phem_tuning *t) {
   if (!t) return phem::set_err(PHEM_EINVAL, "t is NULL");
+  using phem::one_of;
   if (t->blocks_per_cu > 64)
     return phem::set_err(PHEM_EINVAL, "blocks_per_cu %u beyond 64", t->blocks_per_cu);
+  if (t->mst_block && !one_of(t->mst_block, {256, 512}))
+    return phem::set_err(PHEM_EINVAL, "mst_block %u: 256 or 512", t->mst_block);
+  if (t->mst_ept && !one_of(t->mst_ept, {4, 8, 16}))
+    return phem::set_err(PHEM_EINVAL, "mst_ept %u: 4, 8 or 16", t->mst_ept);
+  {
+    // at most 4096 edges per tile (the segment capacities' bound)
+    const uint32_t b = t->mst_block ? t->mst_block : phem::g_mst_block.load();
+    const uint32_t e = t->mst_ept ? t->mst_ept : phem::g_mst_ept.load();
+    if (b * e > 4096)
+      return phem::set_err(PHEM_EINVAL, "mst_block %u x mst_ept %u beyond 4096 edges",
+                           b, e);
+  }
+  if (t->mst_ept_compact && !one_of(t->mst_ept_compact, {8, 16}))
+    return phem::set_err(PHEM_EINVAL, "mst_ept_compact %u: 8 or 16",
+                         t->mst_ept_compact);
+  if (t->mst_compact_log2 > 40)
+    return phem::set_err(PHEM_EINVAL, "mst_compact_log2 %u beyond 40",
+                         t->mst_compact_log2);
+  if (!std::isnan(t->mst_split_ratio) && !(t->mst_split_ratio > 1.0))
+    return phem::set_err(PHEM_EINVAL, "mst_split_ratio must be above 1");
+  if (t->cc_block && !one_of(t->cc_block, {256, 512}))
+    return phem::set_err(PHEM_EINVAL, "cc_block %u: 256 or 512", t->cc_block);
+  if (t->cc_ept && !one_of(t->cc_ept, {8, 16, 32}))
+    return phem::set_err(PHEM_EINVAL, "cc_ept %u: 8, 16 or 32", t->cc_ept);
+  if (t->cc_ept_filter && !one_of(t->cc_ept_filter, {4, 8, 16}))
+    return phem::set_err(PHEM_EINVAL, "cc_ept_filter %u: 4, 8 or 16",
+                         t->cc_ept_filter);
   phem::tuning_init();
+  if (t->mst_block) phem::g_mst_block.store(t->mst_block);
+  if (t->mst_ept) phem::g_mst_ept.store(t->mst_ept);
+  if (t->mst_ept_compact) phem::g_mst_ept_compact.store(t->mst_ept_compact);
+  if (t->mst_compact_log2) phem::g_mst_compact_log2.store(t->mst_compact_log2);
+  if (!std::isnan(t->mst_split_ratio)) phem::g_mst_split_ratio.store(t->mst_split_ratio);
+  if (t->cc_block) phem::g_cc_block.store(t->cc_block);
+  if (t->cc_ept) phem::g_cc_ept.store(t->cc_ept);
+  if (t->cc_ept_filter) phem::g_cc_ept_filter.store(t->cc_ept_filter);
   if (t->blocks_per_cu) phem::g_blocks_per_cu.store(t->blocks_per_cu);
   if (!std::isnan(t->mst_light_factor)) phem::g_light_factor.store(t->mst_light_factor);
   if (!std::isnan(t->cc_sample_factor)) phem::g_sample_factor.store(t->cc_sample_factor);

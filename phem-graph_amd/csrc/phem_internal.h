@@ -25,12 +25,30 @@ unsigned resident_blocks();
 double mst_default_light_factor();
 double cc_default_sample_factor();
 
+// Kernel shapes selected at launch (phem_tuning's mst_* and cc_* shapes).
+struct KernelShape {
+  uint32_t mst_block, mst_ept, mst_ept_compact, mst_compact_log2;
+  double mst_split_ratio;
+  uint32_t cc_block, cc_ept, cc_ept_filter;
+};
+KernelShape kernel_shape();
+
 // Grid size for `work` items at `per_block` items per block, capped at the
 // resident block count: kernels loop over their work, so blocks stay alive
 // across many tiles instead of the dispatcher launching one block per tile.
 inline unsigned grid_for(uint64_t work, uint64_t per_block) {
   const uint64_t g = ceil_div(work ? work : 1, per_block);
   const uint64_t cap = resident_blocks();
+  return (unsigned)(g < cap ? g : cap);
+}
+
+// grid_for for blocks of `threads` threads: the resident cap counts
+// 256-thread blocks, so larger blocks get proportionally fewer.
+inline unsigned grid_for_threads(uint64_t work, uint64_t per_block,
+                                 unsigned threads) {
+  const uint64_t g = ceil_div(work ? work : 1, per_block);
+  uint64_t cap = (uint64_t)resident_blocks() * 256 / threads;
+  if (!cap) cap = 1;
   return (unsigned)(g < cap ? g : cap);
 }
 

@@ -55,6 +55,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <type_traits>
 #include <utility>
 
 #include "phem_dev.h"
@@ -66,12 +67,13 @@ namespace phem {
 constexpr int BLK = 256;       // threads per block (4 waves)
 constexpr int EPT = 8;         // edges per thread per tile (input passes)
 constexpr int TILE = BLK * EPT;
-constexpr int EPT_E = 16;      // edges per thread per tile (compaction rounds
-constexpr int TILE_E = BLK * EPT_E;  // of large graphs, see wide_edges)
-
-// Compaction rounds of graphs with at least 2^22 edges use EPT_E edges per
-// thread; smaller graphs keep EPT, whose smaller tiles need less segment room.
-static inline bool wide_edges(uint64_t m) { return m >= (1ull << 22); }
+// The input passes (light and heavy filter) run with mst_block threads and
+// mst_ept edges per thread (256 x 8 built in); the compaction rounds of
+// graphs with m >= 2^mst_compact_log2 with mst_ept_compact edges per thread
+// (16 built in), smaller graphs with EPT.  Segment capacities are sized for
+// the largest tile of any allowed shape, so the workspace does not depend on
+// the knobs.
+constexpr int MAX_EDGE_TILE = 4096;  // 256 x 16 or 512 x 8 edges
 constexpr int WPT = 8;         // bitmap words per thread
 constexpr int WPB = BLK * WPT; // bitmap words per block
 constexpr int IPT = 4;         // vertices or components per thread (compress,
@@ -438,8 +440,8 @@ struct EdgeTaus {
   uint32_t k;
 };
 
-template <int MODE, class Src, int NE_ = EPT>
-__global__ __launch_bounds__(BLK) void k_pass(
+template <int MODE, class Src, int NE_ = EPT, int B_ = BLK>
+__global__ __launch_bounds__(B_) void k_pass(
     Src src, const uint32_t *__restrict__ ia, const uint32_t *__restrict__ ib,
     const ull *__restrict__ ik, const ull *in_cnt, uint64_t in_cap,
     uint64_t m0, uint32_t n, int ord, const uint32_t *__restrict__ par,
@@ -448,17 +450,17 @@ __global__ __launch_bounds__(BLK) void k_pass(
     ull *__restrict__ ok, ull *out_cnt, uint64_t out_cap, ull *valid_parts,
     ull *best, ull *gmin, const uint32_t *gsum, EdgeTaus et, int two_step) {
   // tile shape: the compaction rounds of large graphs measured faster with
-  // EPT_E edges per thread (16: 1.43 against 1.52 ms of rounds at n = 2^24,
+  // 16 edges per thread (1.43 against 1.52 ms of rounds at n = 2^24,
   // m = 2^28), the input passes with EPT (8; 16 made k_light and k_heavy
   // 0.15-0.4 ms slower)
   constexpr int NE = NE_;
-  constexpr int KT = BLK * NE;
+  constexpr int KT = B_ * NE;
   __shared__ SegView sv;
   __shared__ uint32_t s_sum[MODE == P_HEAVY ? GSUM_HEAVY : 1];
   constexpr bool INPUT = MODE != P_EDGES;
   if (!INPUT) seg_load<KT>(sv, in_cnt);
   uint32_t covered = 0;
-  if (MODE == P_HEAVY) covered = gsum_load<BLK, GSUM_HEAVY>(s_sum, gsum, n);
+  if (MODE == P_HEAVY) covered = gsum_load<B_, GSUM_HEAVY>(s_sum, gsum, n);
   const ull tau = (MODE == P_LIGHT || MODE == P_HEAVY) ? *tau_ptr : 0ull;
   const uint32_t ntiles_in = INPUT ? (uint32_t)ceil_div_dev(m0, KT) : 0u;
   uint32_t nval = 0;  // valid input edges seen by this thread
@@ -484,24 +486,24 @@ __global__ __launch_bounds__(BLK) void k_pass(
       if (full) {
 #pragma unroll
         for (int g = 0; g < NE / 4; g++)
-          src.load4_full(start + ((uint64_t)g * BLK + threadIdx.x) * 4, a + 4 * g,
+          src.load4_full(start + ((uint64_t)g * B_ + threadIdx.x) * 4, a + 4 * g,
                          b + 4 * g, wv + 4 * g);
       } else {
 #pragma unroll
         for (int g = 0; g < NE / 4; g++)
-          src.load4(start + ((uint64_t)g * BLK + threadIdx.x) * 4, end, a + 4 * g,
+          src.load4(start + ((uint64_t)g * B_ + threadIdx.x) * 4, end, a + 4 * g,
                     b + 4 * g, wv + 4 * g);
       }
 #pragma unroll
       for (int j = 0; j < NE; j++) {
-        const uint64_t i = start + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
+        const uint64_t i = start + ((uint64_t)(j / 4) * B_ + threadIdx.x) * 4 + (j % 4);
         key[j] = make_key(wv[j], i, ord);
         val[j] = (a[j] < n) & (b[j] < n) & (a[j] != b[j]);
       }
     } else if (full) {
 #pragma unroll
       for (int g = 0; g < NE / 4; g++) {
-        const uint64_t i4 = start + ((uint64_t)g * BLK + threadIdx.x) * 4;
+        const uint64_t i4 = start + ((uint64_t)g * B_ + threadIdx.x) * 4;
         u32x4 x = __builtin_nontemporal_load((const u32x4 *)(ia + i4));
         u32x4 y = __builtin_nontemporal_load((const u32x4 *)(ib + i4));
         u64x2 k0 = __builtin_nontemporal_load((const u64x2 *)(ik + i4));
@@ -516,7 +518,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
     } else {
 #pragma unroll
       for (int j = 0; j < NE; j++) {
-        const uint64_t i = start + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
+        const uint64_t i = start + ((uint64_t)(j / 4) * B_ + threadIdx.x) * 4 + (j % 4);
         const bool in = i < end;
         a[j] = in ? ia[i] : 0u;
         b[j] = in ? ib[i] : 0u;
@@ -540,7 +542,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
     }
     auto tau_of = [&](int j) -> ull {
       if (one_graph) return ttau;
-      const uint64_t i = start + ((uint64_t)(j / 4) * BLK + threadIdx.x) * 4 + (j % 4);
+      const uint64_t i = start + ((uint64_t)(j / 4) * B_ + threadIdx.x) * 4 + (j % 4);
       return et.taus[graph_of(et.emap, et.eoff, et.k, i < end ? i : start)];
     };
     if (MODE == P_LIGHT) {
@@ -631,7 +633,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
     // output segment (one atomic per tile)
     const uint32_t seg = v % KSEG;
     uint64_t slot[NE];
-    block_append<BLK, NE>(keep, slot, out_cnt + seg * CSTR,
+    block_append<B_, NE>(keep, slot, out_cnt + seg * CSTR,
                            (uint64_t)seg * out_cap);
 #pragma unroll
     for (int j = 0; j < NE; j++) {
@@ -643,7 +645,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
     }
   }
   if (INPUT && valid_parts) {
-    __shared__ ull s_w[BLK / WAVE];
+    __shared__ ull s_w[B_ / WAVE];
     ull x = nval;
 #pragma unroll
     for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
@@ -651,7 +653,7 @@ __global__ __launch_bounds__(BLK) void k_pass(
     __syncthreads();
     if (threadIdx.x == 0) {
       ull tot = 0;
-      for (int w = 0; w < BLK / WAVE; w++) tot += s_w[w];
+      for (int w = 0; w < B_ / WAVE; w++) tot += s_w[w];
       if (tot) atomicAdd(valid_parts + (blockIdx.x % KSEG) * CSTR, tot);
     }
   }
@@ -1168,16 +1170,17 @@ static MstLayout mst_layout(uint32_t n, uint64_t m) {
   L.nb = ceil_div(L.words, WPB);
   // Segment capacities: a producer pass over at most ceil(x / T) + KSEG tiles
   // of T items sends at most ceil(ceil(x / T) / KSEG) + 1 tiles to one
-  // segment, at most x / KSEG + 2 T items; the bound with the compaction
-  // rounds' tile (TILE_E for wide_edges graphs) covers the input passes'
-  // TILE too.
+  // segment, at most x / KSEG + 2 T items; the bound with the largest edge
+  // tile of any shape (MAX_EDGE_TILE) covers every smaller tile too.
   // The heavy pass and k_giant_emit (LTILE vertices per tile) append to the
   // same list, so its segments hold both producers' worst case.
   // No list ever holds more than m + n alive edges (heavy survivors plus one
   // giant edge per component), so no segment does either: small graphs get
   // segments of m + n items instead of the tile-based bound.
-  static_assert(TILE_E >= TILE, "segment capacity uses the largest edge tile");
-  const uint64_t TE = wide_edges(m) ? TILE_E : TILE;
+  static_assert(MAX_EDGE_TILE >= TILE && MAX_EDGE_TILE >= 256 * 16 &&
+                    MAX_EDGE_TILE >= 512 * 8,
+                "segment capacity uses the largest edge tile");
+  const uint64_t TE = MAX_EDGE_TILE;
   L.ecap = (ceil_div(ceil_div(m ? m : 1, TE), KSEG) + 2) * TE +
            ceil_div(ceil_div(n ? n : 1, LTILE), KSEG) * LTILE;
   L.ecap = std::min<uint64_t>(L.ecap, align_up(m + n + 1, 4));
@@ -1245,7 +1248,29 @@ static uint64_t mst_light_edges(uint32_t n, uint64_t m, double light_factor) {
 // light_edges); sparser graphs run the canonical rounds.
 static bool mst_split(uint32_t n, uint64_t m, double light_factor) {
   const uint64_t le = mst_light_edges(n, m, light_factor);
-  return le < m && 2 * m > 3 * le;
+  return le < m && (double)m > kernel_shape().mst_split_ratio * (double)le;
+}
+
+// Launches k_pass<MODE> over the m input edges with the knobs' shape
+// (mst_block x mst_ept) for structure-of-arrays inputs; the array-of-records
+// layout keeps the built-in shape.
+template <int MODE, class Src, class... A>
+static void launch_input_pass(uint64_t m, hipStream_t s, A... args) {
+  if constexpr (std::is_same<Src, SrcSoA>::value) {
+    const KernelShape k = kernel_shape();
+#define PHEM_INPUT_PASS(BB, EE)                                                \
+    if (k.mst_block == BB && k.mst_ept == EE) {                                \
+      hipLaunchKernelGGL((k_pass<MODE, Src, EE, BB>),                          \
+                         dim3(grid_for_threads(m, (uint64_t)BB * EE, BB)),     \
+                         dim3(BB), 0, s, args...);                             \
+      return;                                                                  \
+    }
+    PHEM_INPUT_PASS(256, 4) PHEM_INPUT_PASS(256, 8) PHEM_INPUT_PASS(256, 16)
+    PHEM_INPUT_PASS(512, 4) PHEM_INPUT_PASS(512, 8)
+#undef PHEM_INPUT_PASS
+  }
+  hipLaunchKernelGGL((k_pass<MODE, Src, EPT, BLK>), dim3(grid_for(m, TILE)),
+                     dim3(BLK), 0, s, args...);
 }
 
 // grp: the graphs of a batch (giant component per graph), or NULL.
@@ -1273,7 +1298,10 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   const int ord = wmode | ((flags & PHEM_MST_MAXIMUM) ? ORD_MAX : 0);
   if (n && !d_mst_ids) return set_err(PHEM_EINVAL, "d_mst_ids is NULL");
   const MstLayout L = mst_layout(n, m);
-  const bool wide = wide_edges(m);  // compaction tile, as in the layout
+  const KernelShape shape = kernel_shape();
+  // compaction rounds: 16 or 8 edges per thread
+  const bool wide = m >= (1ull << shape.mst_compact_log2) &&
+                    shape.mst_ept_compact == 16;
   if (!d_ws || ws_bytes < L.total)
     return set_err(PHEM_ENOMEM, "workspace %llu B < required %llu B",
                    (unsigned long long)ws_bytes, (unsigned long long)L.total);
@@ -1413,8 +1441,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
         {
           PHEM_PROF("k_compact", s);
           if (wide)
-            hipLaunchKernelGGL((k_pass<P_EDGES, Src, EPT_E>),
-                               dim3(grid_for(ceil_div(m_r, TILE_E) + KSEG, 1)),
+            hipLaunchKernelGGL((k_pass<P_EDGES, Src, 16>),
+                               dim3(grid_for(ceil_div(m_r, BLK * 16) + KSEG, 1)),
                                dim3(BLK), 0, s, src, Ea[c], Eb[c], Ek[c],
                                ctr + cE(rr), L.ecap, 0ull, n, ord, par, noull,
                                nou, Ea[c ^ 1], Eb[c ^ 1], Ek[c ^ 1],
@@ -1550,8 +1578,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     PHEM_LAUNCHED("k_threshold");
     {
       PHEM_PROF("k_light", s);
-      hipLaunchKernelGGL((k_pass<P_LIGHT, Src>), dim3(grid_for(m, TILE)),
-                         dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
+      launch_input_pass<P_LIGHT, Src>(m, s, src, nou, nou, noull, noull, 0ull, m,
                          n, ord, par, ctr + C_TAU, nou, Ea[0], Eb[0], Ek[0],
                          ctr + cE(0), L.ecap, ctr + cPart(0), best,
                          (ull *)nullptr, nou, et, 0);
@@ -1617,8 +1644,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     PHEM_LAUNCHED("k_giant_summary");
     {
       PHEM_PROF("k_heavy", s);
-      hipLaunchKernelGGL((k_pass<P_HEAVY, Src>), dim3(grid_for(m, TILE)),
-                         dim3(BLK), 0, s, src, nou, nou, noull, noull, 0ull, m,
+      launch_input_pass<P_HEAVY, Src>(m, s, src, nou, nou, noull, noull, 0ull, m,
                          n, ord, par, ctr + C_TAU, gbits, Ea[cur], Eb[cur],
                          Ek[cur], ctr + cE(r), L.ecap, (ull *)nullptr, best,
                          gmin, gsum, et, 0);

@@ -111,7 +111,19 @@ class GraphFile(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
-                ("mst_light_factor", ctypes.c_double), ("cc_sample_factor", ctypes.c_double)]
+                ("mst_light_factor", ctypes.c_double), ("cc_sample_factor", ctypes.c_double),
+                ("mst_block", ctypes.c_uint32), ("mst_ept", ctypes.c_uint32),
+                ("mst_ept_compact", ctypes.c_uint32), ("mst_compact_log2", ctypes.c_uint32),
+                ("mst_split_ratio", ctypes.c_double),
+                ("cc_block", ctypes.c_uint32), ("cc_ept", ctypes.c_uint32),
+                ("cc_ept_filter", ctypes.c_uint32), ("reserved2", ctypes.c_uint32)]
+
+
+# the knobs of phem_tuning, in order (the reserved words excluded)
+TUNING_KNOBS = ("blocks_per_cu", "mst_light_factor", "cc_sample_factor", "mst_block",
+                "mst_ept", "mst_ept_compact", "mst_compact_log2", "mst_split_ratio",
+                "cc_block", "cc_ept", "cc_ept_filter")
+_FLOAT_KNOBS = ("mst_light_factor", "cc_sample_factor", "mst_split_ratio")
 
 
 _lib = None
@@ -730,15 +742,22 @@ def cpu_msf(n, u, v, w, wmode=W_F32, threads=0):
 def get_tuning():
     t = Tuning()
     _check(lib().phem_get_tuning(ctypes.byref(t)), "phem_get_tuning")
-    return {"blocks_per_cu": t.blocks_per_cu, "mst_light_factor": t.mst_light_factor,
-            "cc_sample_factor": t.cc_sample_factor}
+    return {k: getattr(t, k) for k in TUNING_KNOBS}
 
 
-def set_tuning(blocks_per_cu=0, mst_light_factor=None, cc_sample_factor=None):
-    """Sets the process-wide knobs (None / 0: unchanged)."""
-    nan = float("nan")
-    t = Tuning(blocks_per_cu or 0, 0, nan if mst_light_factor is None else mst_light_factor,
-               nan if cc_sample_factor is None else cc_sample_factor)
+def set_tuning(**knobs):
+    """Sets the process-wide knobs named in TUNING_KNOBS (a knob left out, or
+    given as None / 0, is unchanged)."""
+    bad = set(knobs) - set(TUNING_KNOBS)
+    if bad:
+        raise PhemError(f"unknown tuning knobs {sorted(bad)}", EINVAL)
+    t = Tuning()
+    for k in TUNING_KNOBS:
+        x = knobs.get(k)
+        if k in _FLOAT_KNOBS:
+            setattr(t, k, float("nan") if x is None else float(x))
+        else:
+            setattr(t, k, int(x or 0))
     _check(lib().phem_set_tuning(ctypes.byref(t)), "phem_set_tuning")
 
 

@@ -6,7 +6,14 @@
   solve (edges below the filter threshold ~ factor * n), i.e. where the
   filter-Kruskal split falls;
 - cc_sample_factor: size of the sampling phase of the connected-components
-  solve (first factor * n records).
+  solve (first factor * n records);
+- kernel shapes, each a choice among kernels compiled for it: mst_block and
+  mst_ept (threads per block and edges per thread of the MST input passes),
+  mst_ept_compact and mst_compact_log2 (edges per thread of the compaction
+  rounds and the edge count from which they use it), mst_split_ratio (the
+  light/heavy split runs when m > ratio * light_factor * n), cc_block,
+  cc_ept and cc_ept_filter (the CC unions' threads per block and records per
+  thread, unfiltered and filtered).
 
 tune() builds a synthetic graph of the caller's shape on the device (RMAT or
 uniform, n vertices, m edges), then improves one knob at a time (coordinate
@@ -30,9 +37,25 @@ CACHE = os.environ.get("PHEM_TUNE_CACHE",
 
 GRID = {
     "blocks_per_cu": [4, 8, 16, 32],
+    "mst_block": [256, 512],
+    "mst_ept": [4, 8, 16],
+    "mst_ept_compact": [8, 16],
+    "mst_compact_log2": [18, 22, 26],
     "mst_light_factor": [0.25, 0.5, 1.0, 2.0],
+    "mst_split_ratio": [1.25, 1.5, 2.0, 3.0],
+    "cc_block": [256, 512],
+    "cc_ept": [8, 16, 32],
+    "cc_ept_filter": [4, 8, 16],
     "cc_sample_factor": [0.5, 1.0, 1.5, 2.0],
 }
+# knobs timed on the MST solve alone and on the CC solve alone (the rest on
+# both), in the order coordinate descent visits them
+MST_KNOBS = ("mst_block", "mst_ept", "mst_ept_compact", "mst_compact_log2",
+             "mst_light_factor", "mst_split_ratio")
+CC_KNOBS = ("cc_block", "cc_ept", "cc_ept_filter", "cc_sample_factor")
+ORDER = ("blocks_per_cu",) + MST_KNOBS + CC_KNOBS
+# swept together
+JOINT = (("mst_block", "mst_ept"), ("cc_block", "cc_ept", "cc_ept_filter"))
 
 
 def _device_name():
@@ -131,22 +154,42 @@ def tune(n, m, graph="rmat", seed=1, reps=3, grid=None, device="cuda", verbose=F
     log = []
 
     def cost(knob):
-        if knob == "mst_light_factor":
+        if knob in MST_KNOBS:
             return timed(run_mst, "mst")
-        if knob == "cc_sample_factor":
+        if knob in CC_KNOBS:
             return timed(run_cc, "cc")
         return timed(run_mst, "mst") + timed(run_cc, "cc")
 
+    def sweep(knobs):
+        """All combinations of the grid values of `knobs` (jointly: a block
+        size and a per-thread count interact through occupancy)."""
+        combos = [{}]
+        for k in knobs:
+            combos = [{**c, k: v} for c in combos for v in grid[k]]
+        scores = []
+        for c in combos:
+            trial = {**best, **c}
+            if trial["mst_block"] * trial["mst_ept"] > 4096:
+                continue  # a tile beyond 4096 edges is not compiled
+            P.set_tuning(**trial)
+            ms = cost(knobs[0])
+            scores.append((ms, c))
+            log.append({"knob": ",".join(knobs), "value": [c[k] for k in knobs],
+                        "ms": round(ms, 4)})
+            if verbose:
+                print("# " + " ".join(f"{k}={c[k]}" for k in knobs) + f": {ms:.3f} ms",
+                      flush=True)
+        best.update(min(scores, key=lambda t: t[0])[1])
+
     try:
-        for knob in ("blocks_per_cu", "mst_light_factor", "cc_sample_factor"):
-            scores = {}
-            for val in grid[knob]:
-                P.set_tuning(**{**best, knob: val})
-                scores[val] = cost(knob)
-                log.append({"knob": knob, "value": val, "ms": round(scores[val], 4)})
-                if verbose:
-                    print(f"# {knob}={val}: {scores[val]:.3f} ms", flush=True)
-            best[knob] = min(scores, key=scores.get)
+        done = set()
+        for knob in ORDER:
+            if knob not in grid or knob in done:
+                continue
+            group = next((g for g in JOINT if knob in g), (knob,))
+            group = tuple(k for k in group if k in grid)
+            sweep(group)
+            done.update(group)
         P.set_tuning(**best)
         total = timed(run_mst, "mst") + timed(run_cc, "cc")
     except Exception:

@@ -22,12 +22,14 @@ def test_tune_small_graph(P, tmp_path):
     import phem_tune as T
     old = P.get_tuning()
     grid = {"blocks_per_cu": [4, 16], "mst_light_factor": [0.5, 1.0],
-            "cc_sample_factor": [1.0, 2.0]}
+            "cc_sample_factor": [1.0, 2.0], "mst_block": [256, 512], "mst_ept": [4, 8],
+            "cc_block": [256, 512], "cc_ept": [16], "cc_ept_filter": [8]}
     try:
         r = T.tune(1 << 16, 1 << 20, "rmat", seed=3, reps=2, grid=grid)
         for k, vals in grid.items():
             assert r["config"][k] in vals
-        assert len(r["log"]) == 6
+        # 2 + 4 (block x ept jointly) + 2 + 2 (cc shapes jointly) + 2 settings
+        assert len(r["log"]) == 12
         path = str(tmp_path / "tune.json")
         T.save(r, path)
         assert T.load(1 << 16, 1 << 20, path) == r["config"]

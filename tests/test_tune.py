@@ -12,13 +12,26 @@ def test_set_get_tuning_round_trip():
     old = P.get_tuning()
     try:
         P.set_tuning(blocks_per_cu=8, mst_light_factor=0.5, cc_sample_factor=3.0)
-        assert P.get_tuning() == {"blocks_per_cu": 8, "mst_light_factor": 0.5,
-                                  "cc_sample_factor": 3.0}
-        P.set_tuning(mst_light_factor=2.0)  # the others unchanged
-        assert P.get_tuning()["blocks_per_cu"] == 8
-        assert P.get_tuning()["mst_light_factor"] == 2.0
-        with pytest.raises(P.PhemError):
-            P.set_tuning(blocks_per_cu=65)
+        t = P.get_tuning()
+        assert (t["blocks_per_cu"], t["mst_light_factor"], t["cc_sample_factor"]) == (8, 0.5, 3.0)
+        # built-in kernel shapes
+        assert {k: t[k] for k in ("mst_block", "mst_ept", "mst_ept_compact",
+                                  "mst_compact_log2", "mst_split_ratio", "cc_block",
+                                  "cc_ept", "cc_ept_filter")} == {
+            "mst_block": 256, "mst_ept": 8, "mst_ept_compact": 16, "mst_compact_log2": 22,
+            "mst_split_ratio": 1.5, "cc_block": 256, "cc_ept": 16, "cc_ept_filter": 8}
+        P.set_tuning(mst_light_factor=2.0, mst_block=512, cc_ept=32)  # the others unchanged
+        t = P.get_tuning()
+        assert t["blocks_per_cu"] == 8 and t["mst_light_factor"] == 2.0
+        assert (t["mst_block"], t["mst_ept"], t["cc_ept"]) == (512, 8, 32)
+        for bad in ({"blocks_per_cu": 65}, {"mst_block": 384}, {"mst_ept": 32},
+                    {"mst_ept_compact": 4}, {"mst_compact_log2": 41},
+                    {"mst_ept": 16},  # 512 x 16 edges per tile: beyond 4096
+                    {"mst_split_ratio": 1.0}, {"cc_block": 1024}, {"cc_ept": 4},
+                    {"cc_ept_filter": 32}, {"no_such_knob": 1}):
+            with pytest.raises(P.PhemError):
+                P.set_tuning(**bad)
+        assert P.get_tuning() == t  # refused settings change nothing
     finally:
         P.set_tuning(**old)
 
