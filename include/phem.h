@@ -74,11 +74,14 @@ typedef struct {
   uint32_t mst_block;       /* threads per block of the MST input passes
                                (light and heavy filter): 256 (default), 512 */
   uint32_t mst_ept;         /* their edges per thread: 4, 8 (default), 16;
-                               mst_block x mst_ept <= 4096 */
+                               mst_block x mst_ept <= 4096; graphs of fewer
+                               than 2^22 edges use at most 2048 edges per
+                               tile (per thread: 2048 / mst_block) */
   uint32_t mst_ept_compact; /* edges per thread of the compaction rounds of
                                graphs with m >= 2^mst_compact_log2: 8, 16
                                (default; smaller graphs use 8) */
-  uint32_t mst_compact_log2; /* that edge-count threshold (1..40, default 22) */
+  uint32_t mst_compact_log2; /* that edge-count threshold (1..40, default 22;
+                                below 22 it acts as 22) */
   double mst_split_ratio;   /* the light/heavy split runs when
                                m > ratio * light_factor * n (default 1.5;
                                > 1) */

@@ -71,9 +71,15 @@ constexpr int TILE = BLK * EPT;
 // mst_ept edges per thread (256 x 8 built in); the compaction rounds of
 // graphs with m >= 2^mst_compact_log2 with mst_ept_compact edges per thread
 // (16 built in), smaller graphs with EPT.  Segment capacities are sized for
-// the largest tile of any allowed shape, so the workspace does not depend on
-// the knobs.
+// the largest tile any shape can use on the graph: 4096 edges for graphs of
+// at least 2^22 edges (WIDE_LOG2), 2048 below, where a larger shape runs
+// with its block size and 2048 edges per tile.  So the workspace does not
+// depend on the knobs, and small graphs keep small segments.
 constexpr int MAX_EDGE_TILE = 4096;  // 256 x 16 or 512 x 8 edges
+constexpr int WIDE_LOG2 = 22;
+static inline uint64_t max_edge_tile(uint64_t m) {
+  return m >= (1ull << WIDE_LOG2) ? MAX_EDGE_TILE : TILE;
+}
 constexpr int WPT = 8;         // bitmap words per thread
 constexpr int WPB = BLK * WPT; // bitmap words per block
 constexpr int IPT = 4;         // vertices or components per thread (compress,
@@ -1177,10 +1183,10 @@ static MstLayout mst_layout(uint32_t n, uint64_t m) {
   // No list ever holds more than m + n alive edges (heavy survivors plus one
   // giant edge per component), so no segment does either: small graphs get
   // segments of m + n items instead of the tile-based bound.
-  static_assert(MAX_EDGE_TILE >= TILE && MAX_EDGE_TILE >= 256 * 16 &&
-                    MAX_EDGE_TILE >= 512 * 8,
+  static_assert(MAX_EDGE_TILE >= 256 * 16 && MAX_EDGE_TILE >= 512 * 8 &&
+                    TILE >= 512 * 4 && TILE >= 256 * 8,
                 "segment capacity uses the largest edge tile");
-  const uint64_t TE = MAX_EDGE_TILE;
+  const uint64_t TE = max_edge_tile(m);
   L.ecap = (ceil_div(ceil_div(m ? m : 1, TE), KSEG) + 2) * TE +
            ceil_div(ceil_div(n ? n : 1, LTILE), KSEG) * LTILE;
   L.ecap = std::min<uint64_t>(L.ecap, align_up(m + n + 1, 4));
@@ -1258,8 +1264,11 @@ template <int MODE, class Src, class... A>
 static void launch_input_pass(uint64_t m, hipStream_t s, A... args) {
   if constexpr (std::is_same<Src, SrcSoA>::value) {
     const KernelShape k = kernel_shape();
+    const uint32_t B = k.mst_block;
+    uint32_t E = k.mst_ept;
+    if ((uint64_t)B * E > max_edge_tile(m)) E = (uint32_t)(max_edge_tile(m) / B);
 #define PHEM_INPUT_PASS(BB, EE)                                                \
-    if (k.mst_block == BB && k.mst_ept == EE) {                                \
+    if (B == BB && E == EE) {                                                  \
       hipLaunchKernelGGL((k_pass<MODE, Src, EE, BB>),                          \
                          dim3(grid_for_threads(m, (uint64_t)BB * EE, BB)),     \
                          dim3(BB), 0, s, args...);                             \
@@ -1300,7 +1309,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   const MstLayout L = mst_layout(n, m);
   const KernelShape shape = kernel_shape();
   // compaction rounds: 16 or 8 edges per thread
-  const bool wide = m >= (1ull << shape.mst_compact_log2) &&
+  const bool wide = m >= (1ull << std::max<uint32_t>(shape.mst_compact_log2,
+                                                    WIDE_LOG2)) &&
                     shape.mst_ept_compact == 16;
   if (!d_ws || ws_bytes < L.total)
     return set_err(PHEM_ENOMEM, "workspace %llu B < required %llu B",

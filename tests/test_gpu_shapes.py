@@ -37,11 +37,28 @@ def host(t):
     return t.cpu().numpy().view(np.uint32)
 
 
+_BIG = {}
+
+
+def big_graph(P):
+    """RMAT n = 2^18, m = 2^22: large enough for the 4096-edge tiles (graphs
+    of fewer than 2^22 edges run at most 2048 edges per tile)."""
+    if not _BIG:
+        u, v, w = P.gen_rmat(18, 1 << 22, 7, positive=True)
+        torch.cuda.synchronize()
+        _BIG["g"] = (1 << 18, host(u).copy(), host(v).copy(), host(w).copy(), 1)
+        _BIG["k"] = O.kruskal(*_BIG["g"])
+    return _BIG["g"], _BIG["k"]
+
+
 def check_mst(P, name, lf):
-    n, u, v, w, wmode = cases.make(name)
+    if name == "big":
+        (n, u, v, w, wmode), (k_ids, k_lab) = big_graph(P)
+    else:
+        n, u, v, w, wmode = cases.make(name)
+        k_ids, k_lab = O.kruskal(n, u, v, w, wmode)
     ids, lab, st = P.mst_solve(n, dev(u), dev(v), dev(w), wmode, light_factor=lf)
     torch.cuda.synchronize()
-    k_ids, k_lab = O.kruskal(n, u, v, w, wmode)
     assert np.array_equal(host(ids), k_ids), name
     assert np.array_equal(host(lab), k_lab), name
     ws, exact = O.weight_sum(k_ids, w, wmode)
@@ -50,15 +67,17 @@ def check_mst(P, name, lf):
 
 @pytest.mark.parametrize("block,ept", [(256, 4), (256, 8), (256, 16), (512, 4), (512, 8)])
 def test_mst_input_pass_shapes(P, block, ept):
-    # compaction at 16 per thread from 2^10 edges, so both tiles run too
+    # both compaction tiles: 8 per thread everywhere, 16 from 2^22 edges; the
+    # big graph runs the 4096-edge tiles (smaller graphs at most 2048)
     for compact in (8, 16):
-        P.set_tuning(mst_block=block, mst_ept=ept, mst_ept_compact=compact,
-                     mst_compact_log2=10)
+        P.set_tuning(mst_block=block, mst_ept=ept, mst_ept_compact=compact)
         print(f"mst shape block={block} ept={ept} compact={compact}", flush=True)
-        for name in CASES:
+        for name in CASES + ["big"]:
             for lf in (0.25, 1.0, 0.0):
                 check_mst(P, name, lf)
-    P.set_tuning(mst_block=256, mst_ept=8, mst_ept_compact=16, mst_compact_log2=22)
+    P.set_tuning(mst_block=256, mst_ept=8, mst_ept_compact=16)
+    with pytest.raises(P.PhemError):  # 512 x 16 is beyond the largest compiled tile
+        P.set_tuning(mst_block=512, mst_ept=16)
 
 
 @pytest.mark.parametrize("ratio", [1.25, 3.0, 40.0])
@@ -66,7 +85,7 @@ def test_mst_split_ratio(P, ratio):
     """The split threshold only moves the choice between the two schedules."""
     P.set_tuning(mst_split_ratio=ratio)
     try:
-        for name in CASES:
+        for name in CASES + ["big"]:
             check_mst(P, name, 1.0)
     finally:
         P.set_tuning(mst_split_ratio=1.5)
@@ -76,7 +95,7 @@ def test_mst_split_ratio(P, ratio):
 def test_batch_with_shapes(P, block, ept):
     """A batch of graphs of different density (per-graph thresholds and
     giants) with a non-default input-pass shape."""
-    P.set_tuning(mst_block=block, mst_ept=ept, mst_compact_log2=10)
+    P.set_tuning(mst_block=block, mst_ept=ept)
     try:
         rng = np.random.default_rng(block + ept)
         graphs = []
@@ -94,7 +113,7 @@ def test_batch_with_shapes(P, block, ept):
             assert np.array_equal(r["labels"][g].numpy().view(np.uint32), k_lab), g
             assert r["weight_fx"][g] == O.weight_sum(k_ids, w, 1)[0], g
     finally:
-        P.set_tuning(mst_block=256, mst_ept=8, mst_compact_log2=22)
+        P.set_tuning(mst_block=256, mst_ept=8)
 
 
 @pytest.mark.parametrize("block", [256, 512])
