@@ -382,6 +382,41 @@ int phem_batch_solve_ptrs(uint32_t k, const uint64_t *h_voff,
                           const phem_mst_opts *opts, phem_mst_stats *stats,
                           void *stream);
 
+/* A batched solve recorded once into a HIP graph and replayed (the batched
+ * counterpart of phem_mst_graph_*; no counterpart in the reference, whose
+ * round loop is host-driven, MST/src/kernels.cu:526-689).
+ * phem_batch_graph_create records a batch of up to k_cap - 1 graphs with at
+ * most n_cap vertices and m_cap edges in total, edges given per graph as in
+ * phem_batch_solve_ptrs, with `rounds` Boruvka rounds per phase (0: 12) and
+ * no counter read.  d_ws (>= phem_batch_workspace_bytes(n_cap, m_cap,
+ * k_cap)) and the outputs (d_ids, d_labels: n_cap slots; d_counts,
+ * d_weights, d_inexact: k_cap) stay the caller's and must outlive the graph.
+ * phem_batch_graph_launch copies a batch's offsets and pointer tables (as
+ * for phem_batch_solve_ptrs: k <= k_cap - 1 graphs that fit the
+ * capacities) and enqueues one replay on `stream`; the batch is padded with
+ * one graph of no valid edge that takes the remaining vertices and edges.
+ * Graph g's results are where phem_batch_solve_ptrs puts them.  Launches
+ * must not overlap (replays share the workspace); the edge arrays must stay
+ * valid until the replay is done.  phem_batch_graph_stats waits for the last
+ * replay, fills stats for the whole padded solve, and returns PHEM_ENOCONV
+ * when a phase needed more rounds than recorded: the results are then
+ * incomplete and the batch must be solved another way (phem_batch_solve). */
+typedef struct phem_batch_graph phem_batch_graph_t;
+int phem_batch_graph_create(uint32_t k_cap, uint64_t n_cap, uint64_t m_cap,
+                            int wmode, void *d_ws, uint64_t ws_bytes,
+                            uint32_t *d_ids, uint32_t *d_counts,
+                            uint32_t *d_labels, uint64_t *d_weights,
+                            uint64_t *d_inexact, const phem_mst_opts *opts,
+                            int rounds, phem_batch_graph_t **out);
+int phem_batch_graph_launch(phem_batch_graph_t *g, uint32_t k,
+                            const uint64_t *h_voff, const uint64_t *h_eoff,
+                            const uint32_t *const *h_pu,
+                            const uint32_t *const *h_pv,
+                            const uint32_t *const *h_pw, void *stream);
+int phem_batch_graph_stats(phem_batch_graph_t *g, phem_mst_stats *stats,
+                           void *stream);
+int phem_batch_graph_destroy(phem_batch_graph_t *g);
+
 /* ---------------------------------------------------------------------------
  * Serving: a request loop for MST + CC queries on independent graphs.
  * No counterpart in the reference (its drivers solve one graph per process

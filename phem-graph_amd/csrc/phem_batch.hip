@@ -13,6 +13,7 @@
 // rebase pass reads either and writes the global endpoint arrays the solve
 // runs on, so scattered requests are gathered in the same single pass.
 #include <cstring>
+#include <new>
 
 #include "phem_batch_core.h"
 #include "phem_dev.h"
@@ -62,8 +63,14 @@ __global__ __launch_bounds__(BBLK) void k_batch_rebase(
         const ull base = voff[g], ng = voff[g + 1] - base;
         uint32_t a, b;
         if (TABLE) {
+          const uint32_t *pug = (const uint32_t *)pu[g];
+          if (!pug) {  // a recorded batch's padding graph: no valid edge
+            gu[i] = gv[i] = U32_NONE;
+            gw[i] = 0;
+            continue;
+          }
           const uint64_t j = i - eoff[g];
-          a = ((const uint32_t *)pu[g])[j];
+          a = pug[j];
           b = ((const uint32_t *)pv[g])[j];
           gw[i] = ((const uint32_t *)pw[g])[j];
         } else {
@@ -78,10 +85,13 @@ __global__ __launch_bounds__(BBLK) void k_batch_rebase(
 }
 
 // per-graph forest sizes: ids is ascending, graph g owns [eoff[g], eoff[g+1])
+// count: the forest size, or *d_count when d_count is set (a recorded solve
+// leaves it on the device; count is then only an upper bound).
 __global__ __launch_bounds__(BBLK) void k_batch_counts(
-    const uint32_t *__restrict__ ids, uint64_t count,
+    const uint32_t *__restrict__ ids, uint64_t count, const ull *d_count,
     const ull *__restrict__ eoff, uint32_t k, uint32_t *__restrict__ first,
     uint32_t *__restrict__ counts) {
+  if (d_count) count = *d_count;
   for (uint64_t g = (uint64_t)blockIdx.x * BBLK + threadIdx.x; g < k;
        g += (uint64_t)gridDim.x * BBLK) {
     auto lower = [&](ull x) {  // first index with ids[i] >= x
@@ -100,10 +110,11 @@ __global__ __launch_bounds__(BBLK) void k_batch_counts(
 
 // forest edge i (global id) -> local id at out[voff[g] + (i - first[g])]
 __global__ __launch_bounds__(BBLK) void k_batch_split_ids(
-    const uint32_t *__restrict__ ids, uint64_t count,
+    const uint32_t *__restrict__ ids, uint64_t count, const ull *d_count,
     const ull *__restrict__ voff, const ull *__restrict__ eoff,
     const uint32_t *__restrict__ emap, uint32_t k,
     const uint32_t *__restrict__ first, uint32_t *__restrict__ out) {
+  if (d_count) count = *d_count;
   for (uint64_t i = (uint64_t)blockIdx.x * BBLK + threadIdx.x; i < count;
        i += (uint64_t)gridDim.x * BBLK) {
     const uint32_t e = ids[i];
@@ -129,10 +140,11 @@ __global__ __launch_bounds__(BBLK) void k_batch_split_labels(
 // across the wave) so that one atomic per graph run of a wave is left.
 constexpr int WCH = 8;
 __global__ __launch_bounds__(BBLK) void k_batch_weights(
-    const uint32_t *__restrict__ ids, uint64_t count,
+    const uint32_t *__restrict__ ids, uint64_t count, const ull *d_count,
     const ull *__restrict__ eoff, const uint32_t *__restrict__ emap, uint32_t k,
     const uint32_t *__restrict__ w, int wmode, ull *__restrict__ weights,
     ull *__restrict__ inexact) {
+  if (d_count) count = *d_count;
   const uint64_t nthreads = ceil_div_u64(count, WCH);
   const uint64_t stride = (uint64_t)gridDim.x * BBLK;
   // every lane of a wave runs the same number of iterations (the shuffles)
@@ -238,9 +250,131 @@ int batch_workspace_bytes(uint64_t n_total, uint64_t m_total, uint32_t k,
   return PHEM_OK;
 }
 
-int batch_solve(const BatchArgs &A, hipStream_t s) {
-  const uint32_t k = A.k;
-  const uint64_t *h_voff = A.h_voff, *h_eoff = A.h_eoff;
+// Workspace pointers of a batch (the layout of batch_layout).
+struct BatchDev {
+  uint32_t *gu, *gv, *gw_own, *ids, *glab, *first, *emap, *vmap, *groot;
+  ull *voff, *eoff, *taus;
+  uint64_t *ptrs;
+  char *mst_ws;
+  uint64_t mst_bytes;
+};
+
+static BatchDev batch_dev(char *ws, const BatchLayout &L) {
+  BatchDev D;
+  D.gu = (uint32_t *)(ws + L.gu);
+  D.gv = (uint32_t *)(ws + L.gv);
+  D.gw_own = (uint32_t *)(ws + L.gw);
+  D.ids = (uint32_t *)(ws + L.ids);
+  D.glab = (uint32_t *)(ws + L.glab);
+  D.first = (uint32_t *)(ws + L.first);
+  D.emap = (uint32_t *)(ws + L.emap);
+  D.vmap = (uint32_t *)(ws + L.vmap);
+  D.groot = (uint32_t *)(ws + L.groot);
+  D.voff = (ull *)(ws + L.voff);
+  D.eoff = (ull *)(ws + L.eoff);
+  D.taus = (ull *)(ws + L.taus);
+  D.ptrs = (uint64_t *)(ws + L.ptrs);
+  D.mst_ws = ws + L.mst_ws;
+  D.mst_bytes = L.mst_bytes;
+  return D;
+}
+
+// The device work of a batched solve whose offsets (and pointer tables) are
+// already in the workspace: tile maps, rebase, the solve over the disjoint
+// union, then the per-graph split of the results.  fr: the fixed-round
+// schedule of a recording (the forest size then stays on the device and the
+// result kernels read it there; their grids come from upper bounds).
+static int batch_device(const BatchDev &D, uint32_t k, uint64_t n_total,
+                        uint64_t m_total, bool table, const uint32_t *d_u,
+                        const uint32_t *d_v, const uint32_t *d_w, int wmode,
+                        uint32_t *d_ids, uint32_t *d_counts, uint32_t *d_labels,
+                        uint64_t *d_weights, uint64_t *d_inexact,
+                        const phem_mst_opts *opts, phem_mst_stats *stats,
+                        hipStream_t s, const FixedRounds *fr) {
+  const uint32_t *gw = table ? D.gw_own : d_w;
+  {
+    PHEM_PROF("k_tile_map", s);
+    if (m_total)
+      hipLaunchKernelGGL(k_tile_map, dim3(grid_for(ceil_div(m_total, MT), BBLK)),
+                         dim3(BBLK), 0, s, (const ull *)D.eoff, k, m_total, D.emap);
+    if (n_total)
+      hipLaunchKernelGGL(k_tile_map, dim3(grid_for(ceil_div(n_total, MT), BBLK)),
+                         dim3(BBLK), 0, s, (const ull *)D.voff, k, n_total, D.vmap);
+  }
+  PHEM_LAUNCHED("k_tile_map");
+  if (m_total) {
+    PHEM_PROF("k_batch_rebase", s);
+    if (table)
+      hipLaunchKernelGGL(k_batch_rebase<true>, dim3(grid_for(m_total, RB_PT * BBLK)),
+                         dim3(BBLK), 0, s, nullptr, nullptr, D.ptrs, D.ptrs + k,
+                         D.ptrs + 2ull * k, m_total, (const ull *)D.voff,
+                         (const ull *)D.eoff, D.emap, k, D.gu, D.gv, D.gw_own);
+    else
+      hipLaunchKernelGGL(k_batch_rebase<false>, dim3(grid_for(m_total, RB_PT * BBLK)),
+                         dim3(BBLK), 0, s, d_u, d_v, nullptr, nullptr, nullptr,
+                         m_total, (const ull *)D.voff, (const ull *)D.eoff, D.emap,
+                         k, D.gu, D.gv, nullptr);
+  }
+  PHEM_LAUNCHED("k_batch_rebase");
+  phem_mst_stats st{};
+  const GraphGroups groups{k, D.voff, D.eoff, D.vmap, D.emap, D.taus, D.groot};
+  int rc = mst_solve_groups((uint32_t)n_total, m_total, D.gu, D.gv, gw, wmode,
+                            D.mst_ws, D.mst_bytes, D.ids, D.glab, opts, &st, s,
+                            groups, fr);
+  if (rc) return rc;
+  // forest size: from the stats, or (recorded) on the device with n_total
+  // as its bound
+  const ull *d_count = fr ? solve_count_word(D.mst_ws, (uint32_t)n_total, m_total)
+                          : nullptr;
+  const uint64_t count = fr ? n_total : (uint64_t)st.mst_count;
+  {
+    PHEM_PROF("k_batch_counts", s);
+    hipLaunchKernelGGL(k_batch_counts, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,
+                       D.ids, count, d_count, (const ull *)D.eoff, k, D.first,
+                       d_counts);
+  }
+  PHEM_LAUNCHED("k_batch_counts");
+  if (count) {
+    PHEM_PROF("k_batch_split_ids", s);
+    hipLaunchKernelGGL(k_batch_split_ids, dim3(grid_for(count, BBLK)),
+                       dim3(BBLK), 0, s, D.ids, count, d_count,
+                       (const ull *)D.voff, (const ull *)D.eoff, D.emap, k,
+                       D.first, d_ids);
+  }
+  PHEM_LAUNCHED("k_batch_split_ids");
+  if (d_labels && n_total) {
+    PHEM_PROF("k_batch_split_labels", s);
+    hipLaunchKernelGGL(k_batch_split_labels, dim3(grid_for(n_total, BBLK)),
+                       dim3(BBLK), 0, s, D.glab, n_total, (const ull *)D.voff,
+                       D.vmap, k, d_labels);
+  }
+  PHEM_LAUNCHED("k_batch_split_labels");
+  if (d_weights) {
+    {
+      PHEM_PROF("k_batch_zero", s);
+      hipLaunchKernelGGL(k_batch_zero, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,
+                         (ull *)d_weights, (uint64_t)k);
+      hipLaunchKernelGGL(k_batch_zero, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,
+                         (ull *)d_inexact, (uint64_t)k);
+    }
+    PHEM_LAUNCHED("k_batch_zero");
+    if (count) {
+      PHEM_PROF("k_batch_weights", s);
+      hipLaunchKernelGGL(k_batch_weights,
+                         dim3(grid_for(ceil_div(count, WCH), BBLK)), dim3(BBLK),
+                         0, s, D.ids, count, d_count, (const ull *)D.eoff,
+                         D.emap, k, gw, wmode, (ull *)d_weights,
+                         (ull *)d_inexact);
+    }
+    PHEM_LAUNCHED("k_batch_weights");
+  }
+  if (stats && !fr) *stats = st;
+  return PHEM_OK;
+}
+
+// Offsets checks shared by the eager and the recorded batch.
+static int check_offsets(uint32_t k, const uint64_t *h_voff,
+                         const uint64_t *h_eoff) {
   if (k == 0 || !h_voff || !h_eoff)
     return set_err(PHEM_EINVAL, "empty batch or NULL offsets");
   if (h_voff[0] != 0 || h_eoff[0] != 0)
@@ -248,20 +382,35 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
   for (uint32_t g = 0; g < k; g++)
     if (h_voff[g + 1] < h_voff[g] || h_eoff[g + 1] < h_eoff[g])
       return set_err(PHEM_EINVAL, "offsets of graph %u decrease", g);
+  return PHEM_OK;
+}
+
+static int check_tables(uint32_t k, const uint64_t *h_eoff,
+                        const uint32_t *const *h_pu, const uint32_t *const *h_pv,
+                        const uint32_t *const *h_pw) {
+  if (!h_pu || !h_pv || !h_pw)
+    return set_err(PHEM_EINVAL, "pointer tables must all be given");
+  for (uint32_t g = 0; g < k; g++)
+    if (h_eoff[g + 1] > h_eoff[g] && (!h_pu[g] || !h_pv[g] || !h_pw[g]))
+      return set_err(PHEM_EINVAL, "graph %u has edges but a NULL array", g);
+  return PHEM_OK;
+}
+
+int batch_solve(const BatchArgs &A, hipStream_t s) {
+  const uint32_t k = A.k;
+  const uint64_t *h_voff = A.h_voff, *h_eoff = A.h_eoff;
+  int rc = check_offsets(k, h_voff, h_eoff);
+  if (rc) return rc;
   const uint64_t n_total = h_voff[k], m_total = h_eoff[k];
   uint64_t need = 0;
-  int rc = batch_workspace_bytes(n_total, m_total, k, &need);
+  rc = batch_workspace_bytes(n_total, m_total, k, &need);
   if (rc) return rc;
   if (!A.d_ws || A.ws_bytes < need)
     return set_err(PHEM_ENOMEM, "batch workspace %llu B < required %llu B",
                    (unsigned long long)A.ws_bytes, (unsigned long long)need);
   const bool table = A.h_pu != nullptr;
   if (table) {
-    if (!A.h_pv || !A.h_pw)
-      return set_err(PHEM_EINVAL, "pointer tables must all be given");
-    for (uint32_t g = 0; g < k; g++)
-      if (h_eoff[g + 1] > h_eoff[g] && (!A.h_pu[g] || !A.h_pv[g] || !A.h_pw[g]))
-        return set_err(PHEM_EINVAL, "graph %u has edges but a NULL array", g);
+    if ((rc = check_tables(k, h_eoff, A.h_pu, A.h_pv, A.h_pw))) return rc;
   } else if (m_total && (!A.d_u || !A.d_v || !A.d_w)) {
     return set_err(PHEM_EINVAL, "NULL edge array");
   }
@@ -273,14 +422,7 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
   rc = phem_mst_workspace_bytes((uint32_t)n_total, m_total, &mb);
   if (rc) return rc;
   const BatchLayout L = batch_layout(n_total, m_total, k, mb);
-  char *ws = (char *)A.d_ws;
-  uint32_t *gu = (uint32_t *)(ws + L.gu), *gv = (uint32_t *)(ws + L.gv);
-  uint32_t *ids = (uint32_t *)(ws + L.ids), *glab = (uint32_t *)(ws + L.glab);
-  ull *voff = (ull *)(ws + L.voff), *eoff = (ull *)(ws + L.eoff);
-  uint64_t *ptrs = (uint64_t *)(ws + L.ptrs);
-  uint32_t *first = (uint32_t *)(ws + L.first);
-  uint32_t *emap = (uint32_t *)(ws + L.emap), *vmap = (uint32_t *)(ws + L.vmap);
-  const uint32_t *gw = table ? (const uint32_t *)(ws + L.gw) : A.d_w;
+  const BatchDev D = batch_dev((char *)A.d_ws, L);
   // offsets (and pointer tables) go up from a pinned buffer of the shared
   // pool, held until the solve below has synchronized, so the copies are
   // asynchronous
@@ -290,8 +432,8 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
   uint64_t *hstage = (uint64_t *)stage.get();
   std::memcpy(hstage, h_voff, 8ull * (k + 1));
   std::memcpy(hstage + (k + 1), h_eoff, 8ull * (k + 1));
-  PHEM_HIP(hipMemcpyAsync(voff, hstage, 8ull * (k + 1), hipMemcpyHostToDevice, s));
-  PHEM_HIP(hipMemcpyAsync(eoff, hstage + (k + 1), 8ull * (k + 1),
+  PHEM_HIP(hipMemcpyAsync(D.voff, hstage, 8ull * (k + 1), hipMemcpyHostToDevice, s));
+  PHEM_HIP(hipMemcpyAsync(D.eoff, hstage + (k + 1), 8ull * (k + 1),
                           hipMemcpyHostToDevice, s));
   if (table) {
     uint64_t *t = hstage + 2ull * (k + 1);
@@ -300,79 +442,13 @@ int batch_solve(const BatchArgs &A, hipStream_t s) {
       t[k + g] = (uint64_t)(uintptr_t)A.h_pv[g];
       t[2ull * k + g] = (uint64_t)(uintptr_t)A.h_pw[g];
     }
-    PHEM_HIP(hipMemcpyAsync(ptrs, t, 24ull * k, hipMemcpyHostToDevice, s));
+    PHEM_HIP(hipMemcpyAsync(D.ptrs, t, 24ull * k, hipMemcpyHostToDevice, s));
   }
-  {
-    PHEM_PROF("k_tile_map", s);
-    if (m_total)
-      hipLaunchKernelGGL(k_tile_map, dim3(grid_for(ceil_div(m_total, MT), BBLK)),
-                         dim3(BBLK), 0, s, eoff, k, m_total, emap);
-    if (n_total)
-      hipLaunchKernelGGL(k_tile_map, dim3(grid_for(ceil_div(n_total, MT), BBLK)),
-                         dim3(BBLK), 0, s, voff, k, n_total, vmap);
-  }
-  PHEM_LAUNCHED("k_tile_map");
-  if (m_total) {
-    PHEM_PROF("k_batch_rebase", s);
-    if (table)
-      hipLaunchKernelGGL(k_batch_rebase<true>, dim3(grid_for(m_total, RB_PT * BBLK)),
-                         dim3(BBLK), 0, s, nullptr, nullptr, ptrs, ptrs + k,
-                         ptrs + 2ull * k, m_total, voff, eoff, emap, k, gu, gv,
-                         (uint32_t *)(ws + L.gw));
-    else
-      hipLaunchKernelGGL(k_batch_rebase<false>, dim3(grid_for(m_total, RB_PT * BBLK)),
-                         dim3(BBLK), 0, s, A.d_u, A.d_v, nullptr, nullptr,
-                         nullptr, m_total, voff, eoff, emap, k, gu, gv, nullptr);
-  }
-  PHEM_LAUNCHED("k_batch_rebase");
-  phem_mst_stats st;
-  const GraphGroups groups{k,    voff, eoff, vmap, emap, (ull *)(ws + L.taus),
-                           (uint32_t *)(ws + L.groot)};
-  rc = mst_solve_groups((uint32_t)n_total, m_total, gu, gv, gw, A.wmode,
-                        ws + L.mst_ws, L.mst_bytes, ids, glab, A.opts, &st, s,
-                        groups);
+  rc = batch_device(D, k, n_total, m_total, table, A.d_u, A.d_v, A.d_w, A.wmode,
+                    A.d_ids, A.d_counts, A.d_labels, A.d_weights, A.d_inexact,
+                    A.opts, A.stats, s, nullptr);
   if (rc) return rc;
   stage.done();  // the solve's counter reads synchronized the stream
-  {
-    PHEM_PROF("k_batch_counts", s);
-    hipLaunchKernelGGL(k_batch_counts, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,
-                       ids, (uint64_t)st.mst_count, eoff, k, first, A.d_counts);
-  }
-  PHEM_LAUNCHED("k_batch_counts");
-  if (st.mst_count) {
-    PHEM_PROF("k_batch_split_ids", s);
-    hipLaunchKernelGGL(k_batch_split_ids, dim3(grid_for(st.mst_count, BBLK)),
-                       dim3(BBLK), 0, s, ids, (uint64_t)st.mst_count, voff,
-                       eoff, emap, k, first, A.d_ids);
-  }
-  PHEM_LAUNCHED("k_batch_split_ids");
-  if (A.d_labels && n_total) {
-    PHEM_PROF("k_batch_split_labels", s);
-    hipLaunchKernelGGL(k_batch_split_labels, dim3(grid_for(n_total, BBLK)),
-                       dim3(BBLK), 0, s, glab, n_total, voff, vmap, k,
-                       A.d_labels);
-  }
-  PHEM_LAUNCHED("k_batch_split_labels");
-  if (A.d_weights) {
-    {
-      PHEM_PROF("k_batch_zero", s);
-      hipLaunchKernelGGL(k_batch_zero, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,
-                         (ull *)A.d_weights, (uint64_t)k);
-      hipLaunchKernelGGL(k_batch_zero, dim3(grid_for(k, BBLK)), dim3(BBLK), 0, s,
-                         (ull *)A.d_inexact, (uint64_t)k);
-    }
-    PHEM_LAUNCHED("k_batch_zero");
-    if (st.mst_count) {
-      PHEM_PROF("k_batch_weights", s);
-      hipLaunchKernelGGL(k_batch_weights,
-                         dim3(grid_for(ceil_div(st.mst_count, WCH), BBLK)),
-                         dim3(BBLK), 0, s, ids, (uint64_t)st.mst_count, eoff,
-                         emap, k, gw, A.wmode, (ull *)A.d_weights,
-                         (ull *)A.d_inexact);
-    }
-    PHEM_LAUNCHED("k_batch_weights");
-  }
-  if (A.stats) *A.stats = st;
   return PHEM_OK;
 }
 
@@ -439,4 +515,177 @@ extern "C" int phem_batch_solve_ptrs(
   A.opts = opts;
   A.stats = stats;
   return batch_solve(A, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// A batched solve recorded once into a HIP graph for capacities (k_cap
+// graphs, n_cap vertices, m_cap edges) and replayed for any batch that fits:
+// the batch is padded with one graph of no valid edge (a NULL pointer
+// table entry, which the rebase turns into invalid endpoints) that takes
+// the remaining vertices and edges, and by empty graphs after it.  Offsets
+// and pointer tables are copied from pinned staging before each replay.
+
+struct phem_batch_graph {
+  uint32_t k_cap = 0;
+  uint64_t n_cap = 0, m_cap = 0;
+  bool split = false;
+  bool launched = false;
+  phem::FixedRounds fr{0, 0};
+  phem::BatchDev D{};
+  uint64_t *h_stage = nullptr;  // pinned: voff, eoff (k_cap + 1 each), 3 tables
+  hipEvent_t staged = nullptr;  // the last launch's staging copies are done
+  unsigned long long *d_sums = nullptr, *h_sums = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void batch_graph_free(phem_batch_graph *g) {
+  if (!g) return;
+  if (g->staged) (void)hipEventSynchronize(g->staged);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->staged) (void)hipEventDestroy(g->staged);
+  (void)hipFree(g->d_sums);
+  (void)hipHostFree(g->h_sums);
+  (void)hipHostFree(g->h_stage);
+  delete g;
+}
+
+extern "C" int phem_batch_graph_create(uint32_t k_cap, uint64_t n_cap,
+                                       uint64_t m_cap, int wmode, void *d_ws,
+                                       uint64_t ws_bytes, uint32_t *d_ids,
+                                       uint32_t *d_counts, uint32_t *d_labels,
+                                       uint64_t *d_weights, uint64_t *d_inexact,
+                                       const phem_mst_opts *opts, int rounds,
+                                       phem_batch_graph_t **out) {
+  if (!out) return set_err(PHEM_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (k_cap < 2)
+    return set_err(PHEM_EINVAL, "k_cap %u < 2 (one graph slot pads the batch)", k_cap);
+  if (rounds < 0) return set_err(PHEM_EINVAL, "rounds %d < 0", rounds);
+  if (!rounds) rounds = 12;
+  uint64_t need = 0;
+  int rc = batch_workspace_bytes(n_cap, m_cap, k_cap, &need);
+  if (rc) return rc;
+  if (!d_ws || ws_bytes < need)
+    return set_err(PHEM_ENOMEM, "batch workspace %llu B < required %llu B",
+                   (unsigned long long)ws_bytes, (unsigned long long)need);
+  if ((n_cap && !d_ids) || !d_counts)
+    return set_err(PHEM_EINVAL, "NULL output array");
+  if ((d_weights != nullptr) != (d_inexact != nullptr))
+    return set_err(PHEM_EINVAL, "weights and inexact go together");
+  if (wmode != PHEM_W_U32 && wmode != PHEM_W_F32)
+    return set_err(PHEM_EINVAL, "unknown weight mode %d", wmode);
+  const bool split = mst_uses_split((uint32_t)n_cap, m_cap, opts);
+  const FixedRounds fr{rounds, split ? rounds : 0};
+  if ((split ? fr.light + fr.heavy : 1 + fr.light) > PHEM_MAX_ROUNDS)
+    return set_err(PHEM_EINVAL, "%d rounds per phase exceed the %d-round limit",
+                   rounds, PHEM_MAX_ROUNDS);
+  phem_batch_graph *g = new (std::nothrow) phem_batch_graph();
+  if (!g) return set_err(PHEM_ENOMEM, "host allocation failed");
+  g->k_cap = k_cap;
+  g->n_cap = n_cap;
+  g->m_cap = m_cap;
+  g->split = split;
+  g->fr = fr;
+  uint64_t mb = 0;
+  (void)phem_mst_workspace_bytes((uint32_t)n_cap, m_cap, &mb);
+  g->D = batch_dev((char *)d_ws, batch_layout(n_cap, m_cap, k_cap, mb));
+  const size_t stage_bytes = 8ull * (2ull * (k_cap + 1) + 3ull * k_cap);
+  if (hipHostMalloc((void **)&g->h_stage, stage_bytes) != hipSuccess ||
+      hipEventCreateWithFlags(&g->staged, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc((void **)&g->d_sums, sizeof(ull) * fixed_sums_words()) != hipSuccess ||
+      hipHostMalloc((void **)&g->h_sums, sizeof(ull) * fixed_sums_words()) != hipSuccess) {
+    (void)hipGetLastError();
+    batch_graph_free(g);
+    return set_err(PHEM_ENOMEM, "recorded batch buffers");
+  }
+  // record on a stream of our own; the launch sizing reads the device's
+  // CU count before recording starts
+  (void)resident_blocks();
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+    batch_graph_free(g);
+    return set_err(PHEM_EHIP, "hipStreamCreateWithFlags failed");
+  }
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = set_err(PHEM_EHIP, "hipStreamBeginCapture failed");
+  } else {
+    rc = batch_device(g->D, k_cap, n_cap, m_cap, true, nullptr, nullptr, nullptr,
+                      wmode, d_ids, d_counts, d_labels, d_weights, d_inexact, opts,
+                      nullptr, cs, &g->fr);
+    const hipError_t e = hipStreamEndCapture(cs, &g->graph);
+    if (!rc && e != hipSuccess)
+      rc = set_err(PHEM_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  (void)hipStreamDestroy(cs);
+  if (!rc && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) !=
+                 hipSuccess)
+    rc = set_err(PHEM_EHIP, "hipGraphInstantiate failed");
+  if (rc) {
+    batch_graph_free(g);
+    return rc;
+  }
+  *out = g;
+  return PHEM_OK;
+}
+
+extern "C" int phem_batch_graph_launch(phem_batch_graph_t *g, uint32_t k,
+                                       const uint64_t *h_voff,
+                                       const uint64_t *h_eoff,
+                                       const uint32_t *const *h_pu,
+                                       const uint32_t *const *h_pv,
+                                       const uint32_t *const *h_pw,
+                                       void *stream) {
+  if (!g) return set_err(PHEM_EINVAL, "graph handle is NULL");
+  int rc = check_offsets(k, h_voff, h_eoff);
+  if (rc) return rc;
+  if (k >= g->k_cap || h_voff[k] > g->n_cap || h_eoff[k] > g->m_cap)
+    return set_err(PHEM_EINVAL,
+                   "batch of %u graphs, %llu vertices, %llu edges exceeds the "
+                   "recorded %u, %llu, %llu (one graph slot pads)", k,
+                   (unsigned long long)h_voff[k], (unsigned long long)h_eoff[k],
+                   g->k_cap - 1, (unsigned long long)g->n_cap,
+                   (unsigned long long)g->m_cap);
+  if ((rc = check_tables(k, h_eoff, h_pu, h_pv, h_pw))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // the previous launch's copies must have read the staging first
+  if (g->launched) PHEM_HIP(hipEventSynchronize(g->staged));
+  const uint32_t K = g->k_cap;
+  uint64_t *vo = g->h_stage, *eo = vo + (K + 1), *t = eo + (K + 1);
+  for (uint32_t i = 0; i <= k; i++) {
+    vo[i] = h_voff[i];
+    eo[i] = h_eoff[i];
+  }
+  for (uint32_t i = k + 1; i <= K; i++) {  // graph k pads, the rest are empty
+    vo[i] = g->n_cap;
+    eo[i] = g->m_cap;
+  }
+  for (uint32_t i = 0; i < K; i++) {
+    const bool real = i < k;
+    t[i] = real ? (uint64_t)(uintptr_t)h_pu[i] : 0;
+    t[K + i] = real ? (uint64_t)(uintptr_t)h_pv[i] : 0;
+    t[2ull * K + i] = real ? (uint64_t)(uintptr_t)h_pw[i] : 0;
+  }
+  PHEM_HIP(hipMemcpyAsync(g->D.voff, vo, 8ull * (K + 1), hipMemcpyHostToDevice, s));
+  PHEM_HIP(hipMemcpyAsync(g->D.eoff, eo, 8ull * (K + 1), hipMemcpyHostToDevice, s));
+  PHEM_HIP(hipMemcpyAsync(g->D.ptrs, t, 24ull * K, hipMemcpyHostToDevice, s));
+  PHEM_HIP(hipEventRecord(g->staged, s));
+  PHEM_HIP(hipGraphLaunch(g->exec, s));
+  g->launched = true;
+  return PHEM_OK;
+}
+
+extern "C" int phem_batch_graph_stats(phem_batch_graph_t *g,
+                                      phem_mst_stats *stats, void *stream) {
+  if (!g) return set_err(PHEM_EINVAL, "graph handle is NULL");
+  if (!g->launched) return set_err(PHEM_EINVAL, "graph not launched yet");
+  return fixed_round_stats(g->D.mst_ws, (uint32_t)g->n_cap, g->m_cap, g->split,
+                           g->fr, g->d_sums, g->h_sums, stats,
+                           (hipStream_t)stream);
+}
+
+extern "C" int phem_batch_graph_destroy(phem_batch_graph_t *g) {
+  batch_graph_free(g);
+  return PHEM_OK;
 }

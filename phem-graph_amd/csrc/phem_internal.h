@@ -66,13 +66,46 @@ struct GraphGroups {
   uint32_t *groot;
 };
 
+// Rounds launched per phase when a solve is recorded for replay: the first
+// phase (canonical rounds after round 0, or the light phase) and the heavy
+// phase.  Nothing is read back between rounds; a round launched after its
+// phase converged finds no alive edge and does nothing.
+struct FixedRounds {
+  int light, heavy;
+};
+
 // phem_mst_solve over a batch's disjoint union (structure-of-arrays input).
+// fr: the fixed-round schedule of a recorded solve (no counter is read, stats
+// stay unwritten, the forest size stays in the device word solve_count_word
+// names), or NULL.
 int mst_solve_groups(uint32_t n, uint64_t m, const uint32_t *d_u,
                      const uint32_t *d_v, const uint32_t *d_w, int wmode,
                      void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
                      uint32_t *d_labels, const phem_mst_opts *opts,
                      phem_mst_stats *stats, hipStream_t s,
-                     const GraphGroups &groups);
+                     const GraphGroups &groups,
+                     const FixedRounds *fr = nullptr);
+
+// Device addresses in a solve's workspace (d_ws of phem_mst_solve for n, m):
+// the word holding the forest size after the solve, and the counter area
+// that fixed_round_stats reads.
+const unsigned long long *solve_count_word(const void *d_ws, uint32_t n,
+                                           uint64_t m);
+
+// Statistics of a fixed-round solve of (n, m) over workspace d_ws, with the
+// schedule fr and light_factor its recording used: enqueues one summing
+// kernel and one copy to h_sums (host, pinned or not) via d_sums (device,
+// fixed_sums_words() words), waits for them on s, fills *stats as
+// phem_mst_solve would and returns PHEM_ENOCONV when a phase still had alive
+// edges after its recorded rounds.
+int fixed_sums_words();
+int fixed_round_stats(const void *d_ws, uint32_t n, uint64_t m, bool split,
+                      const FixedRounds &fr, unsigned long long *d_sums,
+                      unsigned long long *h_sums, phem_mst_stats *stats,
+                      hipStream_t s);
+
+// The light/heavy split decision of a solve of (n, m) with these options.
+bool mst_uses_split(uint32_t n, uint64_t m, const phem_mst_opts *opts);
 
 // Work a caller enqueues after the forest ids are written and before the
 // solve's final counter read, so that read's wait covers it: d_count is the

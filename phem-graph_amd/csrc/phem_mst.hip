@@ -1129,6 +1129,28 @@ __global__ __launch_bounds__(WAVE) void k_phase_gate(const ull *e_cnt, ull *tau,
   }
 }
 
+// k_phase_gate for a batch's graphs, each with its own light threshold:
+// every graph's threshold becomes all ones.
+__global__ __launch_bounds__(BLK) void k_phase_gate_groups(const ull *e_cnt,
+                                                           ull *taus,
+                                                           uint32_t k,
+                                                           ull *gate) {
+  __shared__ ull s_x;
+  if (threadIdx.x < WAVE) {
+    const uint32_t lane = threadIdx.x;
+    ull x = (int)lane < KSEG ? e_cnt[lane * CSTR] : 0ull;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d);
+    if (lane == 0) {
+      s_x = x;
+      if (x) *gate = 1ull;
+    }
+  }
+  __syncthreads();
+  if (s_x)
+    for (uint32_t g = threadIdx.x; g < k; g += BLK) taus[g] = KEY_NONE;
+}
+
 // Counters of a fixed-round solve for the host, in one copy: forest size,
 // tau, gate, the four partial sums, then E(r), A(r) for r = 0 .. r_end.
 constexpr int FS_HEAD = 7;
@@ -1231,14 +1253,6 @@ extern "C" int phem_mst_workspace_bytes(uint32_t n, uint64_t m,
 }
 
 namespace phem {
-
-// Rounds launched per phase when the solve is recorded for replay: the first
-// phase (canonical rounds after round 0, or the light phase) and the heavy
-// phase.  Nothing is read back between rounds; a round launched after its
-// phase converged finds no alive edge and does nothing.
-struct FixedRounds {
-  int light, heavy;
-};
 
 static double mst_light_factor(const phem_mst_opts *opts) {
   return opts && !std::isnan(opts->light_factor) ? opts->light_factor
@@ -1601,8 +1615,13 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                         fr ? fr->light : 0);
     if (rc) return rc;
     if (fr) {
-      hipLaunchKernelGGL(k_phase_gate, dim3(1), dim3(WAVE), 0, s,
-                         (const ull *)(ctr + cE(r)), ctr + C_TAU, ctr + C_GATE);
+      if (groups)
+        hipLaunchKernelGGL(k_phase_gate_groups, dim3(1), dim3(BLK), 0, s,
+                           (const ull *)(ctr + cE(r)), grp->taus, grp->k,
+                           ctr + C_GATE);
+      else
+        hipLaunchKernelGGL(k_phase_gate, dim3(1), dim3(WAVE), 0, s,
+                           (const ull *)(ctr + cE(r)), ctr + C_TAU, ctr + C_GATE);
       PHEM_LAUNCHED("k_phase_gate");
     } else if (stats) {
       stats->tau = hv[0];
@@ -1752,7 +1771,7 @@ int mst_solve_groups(uint32_t n, uint64_t m, const uint32_t *d_u,
                      void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
                      uint32_t *d_labels, const phem_mst_opts *opts,
                      phem_mst_stats *stats, hipStream_t s,
-                     const GraphGroups &groups) {
+                     const GraphGroups &groups, const FixedRounds *fr) {
   if (m && (!d_u || !d_v || !d_w))
     return set_err(PHEM_EINVAL, "edge arrays are NULL");
   if (groups.k > 1 && (!groups.voff || !groups.eoff || !groups.vmap ||
@@ -1760,7 +1779,87 @@ int mst_solve_groups(uint32_t n, uint64_t m, const uint32_t *d_u,
     return set_err(PHEM_EINVAL, "graph ranges are NULL");
   const bool vec = aligned16(d_u) && aligned16(d_v) && aligned16(d_w);
   return mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w, vec}, wmode, d_ws, ws_bytes,
-                        d_mst_ids, d_labels, opts, stats, s, &groups);
+                        d_mst_ids, d_labels, opts, fr ? nullptr : stats, s,
+                        &groups, nullptr, fr);
+}
+
+const unsigned long long *solve_count_word(const void *d_ws, uint32_t n,
+                                           uint64_t m) {
+  return (const ull *)((const char *)d_ws + mst_layout(n, m).ctr) + C_MSTCNT;
+}
+
+bool mst_uses_split(uint32_t n, uint64_t m, const phem_mst_opts *opts) {
+  return m > 0 && n > 1 && mst_split(n, m, mst_light_factor(opts));
+}
+
+int fixed_sums_words() { return FS_HEAD + 2 * (PHEM_MAX_ROUNDS + 1); }
+
+int fixed_round_stats(const void *d_ws, uint32_t n, uint64_t m, bool split,
+                      const FixedRounds &fr, ull *d_sums, ull *h_sums,
+                      phem_mst_stats *stats, hipStream_t s) {
+  const bool work = m > 0 && n > 1;
+  const int r_end = work ? (split ? fr.light + fr.heavy : 1 + fr.light) : 0;
+  const ull *ctr = (const ull *)((const char *)d_ws + mst_layout(n, m).ctr);
+  const int nv = FS_HEAD + 2 * (r_end + 1);
+  hipLaunchKernelGGL(k_fixed_sums, dim3(1), dim3(WAVE), 0, s, ctr, r_end,
+                     d_sums);
+  PHEM_LAUNCHED("k_fixed_sums");
+  PHEM_HIP(hipMemcpyAsync(h_sums, d_sums, sizeof(ull) * nv,
+                          hipMemcpyDeviceToHost, s));
+  PHEM_HIP(hipStreamSynchronize(s));
+  const ull *h = h_sums;
+  auto E = [&](int r) { return h[FS_HEAD + 2 * r]; };
+  auto A = [&](int r) { return h[FS_HEAD + 2 * r + 1]; };
+  phem_mst_stats t;
+  std::memset(&t, 0, sizeof(t));
+  t.mst_count = (uint32_t)h[0];
+  t.valid_edges = h[3];
+  t.weight_fx = h[4];
+  t.weight_exact = h[5] == 0;
+  bool converged = true;
+  if (work) {
+    int r = 0;
+    // the rounds that started with an alive edge are a prefix of each phase
+    auto phase = [&](int k0, int k1) {
+      for (int k = k0; k < k1 && E(k) > 0; k++, r++) {
+        t.m_log[r] = E(k);
+        t.c_log[r] = A(k);
+      }
+    };
+    if (!split) {
+      if (h[3]) {  // round 0 sweeps the input
+        t.m_log[0] = h[3];
+        t.c_log[0] = A(0);
+        r = 1;
+        phase(1, 1 + fr.light);
+      }
+      t.light_edges = h[3];
+      t.tau = KEY_NONE;
+      converged = E(1 + fr.light) == 0;
+      t.light_rounds = (uint32_t)r;
+    } else {
+      phase(0, fr.light);
+      t.light_rounds = (uint32_t)r;
+      const int rh = fr.light;
+      phase(rh, rh + fr.heavy);
+      t.tau = h[1];
+      t.light_edges = E(0);
+      t.heavy_edges = E(rh);
+      t.giant_edges = h[6];
+      converged = h[2] == 0 && E(rh + fr.heavy) == 0;
+    }
+    t.rounds = (uint32_t)r;
+    for (int k = 0; k < r; k++) {
+      t.sum_m += t.m_log[k];
+      t.sum_c += t.c_log[k];
+    }
+  }
+  if (stats) *stats = t;
+  if (!converged)
+    return set_err(PHEM_ENOCONV,
+                   "a phase needed more than its %d recorded rounds",
+                   split ? (fr.light > fr.heavy ? fr.light : fr.heavy) : fr.light);
+  return PHEM_OK;
 }
 
 int mst_solve_tail(uint32_t n, uint64_t m, const uint32_t *d_u,
@@ -1812,8 +1911,7 @@ struct phem_mst_graph {
   bool split = false;
   bool launched = false;
   phem::FixedRounds fr{0, 0};
-  int r_end = 0;  // last round whose counters the stats read
-  unsigned long long *ctr = nullptr;
+  void *ws = nullptr;  // the caller's workspace (its counters give the stats)
   unsigned long long *d_sums = nullptr, *h_sums = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1863,8 +1961,7 @@ extern "C" int phem_mst_graph_create(uint32_t n, uint64_t m, const uint32_t *d_u
   g->m = m;
   g->split = split;
   g->fr = fr;
-  g->r_end = r_end;
-  g->ctr = (unsigned long long *)((char *)d_ws + mst_layout(n, m).ctr);
+  g->ws = d_ws;
   const size_t sums = sizeof(ull) * (FS_HEAD + 2 * (PHEM_MAX_ROUNDS + 1));
   if (hipMalloc((void **)&g->d_sums, sums) != hipSuccess ||
       hipHostMalloc((void **)&g->h_sums, sums) != hipSuccess) {
@@ -1915,68 +2012,8 @@ extern "C" int phem_mst_graph_stats(phem_mst_graph_t *g, phem_mst_stats *stats,
                                     void *stream) {
   if (!g) return set_err(PHEM_EINVAL, "graph handle is NULL");
   if (!g->launched) return set_err(PHEM_EINVAL, "graph not launched yet");
-  hipStream_t s = (hipStream_t)stream;
-  const int nv = FS_HEAD + 2 * (g->r_end + 1);
-  hipLaunchKernelGGL(k_fixed_sums, dim3(1), dim3(WAVE), 0, s,
-                     (const ull *)g->ctr, g->r_end, g->d_sums);
-  PHEM_LAUNCHED("k_fixed_sums");
-  PHEM_HIP(hipMemcpyAsync(g->h_sums, g->d_sums, sizeof(ull) * nv,
-                          hipMemcpyDeviceToHost, s));
-  PHEM_HIP(hipStreamSynchronize(s));
-  const ull *h = g->h_sums;
-  auto E = [&](int r) { return h[FS_HEAD + 2 * r]; };
-  auto A = [&](int r) { return h[FS_HEAD + 2 * r + 1]; };
-  phem_mst_stats t;
-  std::memset(&t, 0, sizeof(t));
-  t.mst_count = (uint32_t)h[0];
-  t.valid_edges = h[3];
-  t.weight_fx = h[4];
-  t.weight_exact = h[5] == 0;
-  bool converged = true;
-  if (g->m > 0 && g->n > 1) {
-    int r = 0;
-    // the rounds that started with an alive edge are a prefix of each phase
-    auto phase = [&](int k0, int k1) {
-      for (int k = k0; k < k1 && E(k) > 0; k++, r++) {
-        t.m_log[r] = E(k);
-        t.c_log[r] = A(k);
-      }
-    };
-    if (!g->split) {
-      if (h[3]) {  // round 0 sweeps the input
-        t.m_log[0] = h[3];
-        t.c_log[0] = A(0);
-        r = 1;
-        phase(1, 1 + g->fr.light);
-      }
-      t.light_edges = h[3];
-      t.tau = KEY_NONE;
-      converged = E(1 + g->fr.light) == 0;
-      t.light_rounds = (uint32_t)r;
-    } else {
-      phase(0, g->fr.light);
-      t.light_rounds = (uint32_t)r;
-      const int rh = g->fr.light;
-      phase(rh, rh + g->fr.heavy);
-      t.tau = h[1];
-      t.light_edges = E(0);
-      t.heavy_edges = E(rh);
-      t.giant_edges = h[6];
-      converged = h[2] == 0 && E(rh + g->fr.heavy) == 0;
-    }
-    t.rounds = (uint32_t)r;
-    for (int k = 0; k < r; k++) {
-      t.sum_m += t.m_log[k];
-      t.sum_c += t.c_log[k];
-    }
-  }
-  if (stats) *stats = t;
-  if (!converged)
-    return set_err(PHEM_ENOCONV,
-                   "a phase needed more than its %d recorded rounds",
-                   g->split ? (g->fr.light > g->fr.heavy ? g->fr.light : g->fr.heavy)
-                            : g->fr.light);
-  return PHEM_OK;
+  return fixed_round_stats(g->ws, g->n, g->m, g->split, g->fr, g->d_sums,
+                           g->h_sums, stats, (hipStream_t)stream);
 }
 
 extern "C" int phem_mst_graph_destroy(phem_mst_graph_t *g) {

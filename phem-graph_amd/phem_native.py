@@ -185,6 +185,13 @@ def lib():
                                         c.POINTER(vp), c.POINTER(vp), c.POINTER(vp),
                                         i32, vp, u64, vp, vp, vp, vp, vp,
                                         c.POINTER(MstOpts), c.POINTER(MstStats), vp]),
+        "phem_batch_graph_create": (i32, [u32, u64, u64, i32, vp, u64, vp, vp, vp, vp, vp,
+                                          c.POINTER(MstOpts), i32, c.POINTER(vp)]),
+        "phem_batch_graph_launch": (i32, [vp, u32, c.POINTER(c.c_uint64),
+                                          c.POINTER(c.c_uint64), c.POINTER(vp),
+                                          c.POINTER(vp), c.POINTER(vp), vp]),
+        "phem_batch_graph_stats": (i32, [vp, c.POINTER(MstStats), vp]),
+        "phem_batch_graph_destroy": (i32, [vp]),
         "phem_server_clock": (c.c_double, []),
         "phem_server_create": (i32, [c.POINTER(ServerOpts), c.POINTER(vp)]),
         "phem_server_destroy": (i32, [vp]),
@@ -504,6 +511,85 @@ def batch_solve(voff, eoff, u, v, w, wmode=W_F32, labels=True, weights=True,
         out["weight_fx"] = [x & 0xFFFFFFFFFFFFFFFF for x in wfx.cpu().tolist()]
         out["inexact"] = bad.cpu().tolist()
     return out
+
+
+class BatchGraph:
+    """phem_batch_graph_*: a batched solve recorded for up to k_cap - 1 graphs
+    of n_cap vertices and m_cap edges in total, replayed by launch() for any
+    batch that fits; results() waits for the replay and returns the same
+    dict as batch_solve (PhemError with PHEM_ENOCONV when a phase needed more
+    than the recorded rounds)."""
+
+    def __init__(self, k_cap, n_cap, m_cap, wmode=W_F32, rounds=0, light_factor=None,
+                 labels=True, weights=True, device="cuda"):
+        dev = torch.device(device)
+        need = ctypes.c_uint64(0)
+        _check(lib().phem_batch_workspace_bytes(n_cap, m_cap, k_cap, ctypes.byref(need)),
+               "phem_batch_workspace_bytes")
+        self.k_cap, self.n_cap, self.m_cap = k_cap, n_cap, m_cap
+        self.ws = torch.empty(max(need.value, 1), dtype=torch.uint8, device=dev)
+        self.ids = torch.empty(max(n_cap, 1), dtype=torch.int32, device=dev)
+        self.cnt = torch.empty(k_cap, dtype=torch.int32, device=dev)
+        self.lab = torch.empty(max(n_cap, 1), dtype=torch.int32, device=dev) if labels else None
+        self.wfx = torch.empty(k_cap, dtype=torch.int64, device=dev) if weights else None
+        self.bad = torch.empty(k_cap, dtype=torch.int64, device=dev) if weights else None
+        h = ctypes.c_void_p()
+        _check(lib().phem_batch_graph_create(k_cap, n_cap, m_cap, wmode, _p(self.ws),
+                                             self.ws.numel(), _p(self.ids), _p(self.cnt),
+                                             _p(self.lab), _p(self.wfx), _p(self.bad),
+                                             _opts(light_factor, False), rounds,
+                                             ctypes.byref(h)), "phem_batch_graph_create")
+        self.h = h
+        self.voff = None
+        self.keep = None
+
+    def launch(self, graphs, stream=None):
+        """graphs: list of (n, u, v, w) with int32 device tensors u, v, w."""
+        k = len(graphs)
+        voff, eoff = [0], [0]
+        for n, u, _, _ in graphs:
+            voff.append(voff[-1] + n)
+            eoff.append(eoff[-1] + u.numel())
+        tab = [(ctypes.c_void_p * max(k, 1))(*[g[i].data_ptr() if g[1].numel() else None
+                                              for g in graphs]) for i in (1, 2, 3)]
+        hv = (ctypes.c_uint64 * (k + 1))(*voff)
+        he = (ctypes.c_uint64 * (k + 1))(*eoff)
+        _check(lib().phem_batch_graph_launch(self.h, k, hv, he, tab[0], tab[1], tab[2],
+                                             _stream(stream)), "phem_batch_graph_launch")
+        self.voff, self.keep = voff, graphs
+
+    def stats(self, stream=None):
+        st = MstStats()
+        _check(lib().phem_batch_graph_stats(self.h, ctypes.byref(st), _stream(stream)),
+               "phem_batch_graph_stats")
+        return st.as_dict()
+
+    def results(self, stream=None):
+        st = self.stats(stream)
+        k = len(self.voff) - 1
+        cnt = self.cnt[:k].cpu().tolist()
+        ids_h, lab_h = self.ids.cpu(), (self.lab.cpu() if self.lab is not None else None)
+        out = {"ids": [], "labels": [], "stats": st, "counts": cnt}
+        for g in range(k):
+            a = self.voff[g]
+            out["ids"].append(ids_h[a: a + cnt[g]])
+            if lab_h is not None:
+                out["labels"].append(lab_h[a: self.voff[g + 1]])
+        if self.wfx is not None:
+            out["weight_fx"] = [x & 0xFFFFFFFFFFFFFFFF for x in self.wfx[:k].cpu().tolist()]
+            out["inexact"] = self.bad[:k].cpu().tolist()
+        return out
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib().phem_batch_graph_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Stream:
