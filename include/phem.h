@@ -442,6 +442,12 @@ typedef struct {
   uint32_t window_us;       /* an idle worker that finds fewer than max_graphs
                                requests waits up to this long for more */
   int32_t wmode;            /* PHEM_W_U32 or PHEM_W_F32 for every request */
+  uint32_t record_rounds;   /* > 0: batches replay a batched solve recorded
+                               (phem_batch_graph_*) for the batch's capacity
+                               bucket with this many rounds per phase; a
+                               batch that needs more falls back to the eager
+                               solve.  0: every batch is solved eagerly */
+  uint32_t reserved;
 } phem_server_opts;
 
 #define PHEM_REQ_DEVICE 1u /* u, v, w are device pointers */
@@ -475,6 +481,10 @@ typedef struct {
   double lat_p50, lat_p99, lat_max, lat_mean; /* done - arrival, seconds */
   double mean_batch_graphs;
   uint64_t slot_bytes, slot_edges;          /* capacity of one slot */
+  uint64_t recorded_batches;  /* batches that replayed a recorded solve */
+  uint64_t record_fallbacks;  /* of those, solved again eagerly (more rounds
+                                 needed than recorded) */
+  uint64_t recordings;        /* recorded solves created (capture + instantiate) */
 } phem_server_stats;
 
 double phem_server_clock(void);

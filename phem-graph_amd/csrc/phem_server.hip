@@ -1,9 +1,9 @@
 // phem_server.hip -- request loop with continuous batching (host code).
 //
 // Requests wait in one arrival-ordered queue.  Each slot owns a worker thread,
-// a HIP stream, one block of device memory and pinned host staging buffers
-// (the input one on the slot's first request from host memory), so serving
-// allocates nothing per request.  An idle worker
+// a HIP stream, one block of device memory, pinned host staging buffers (the
+// input one on the slot's first request from host memory) and a small pool
+// of copy threads, so serving allocates nothing per request.  An idle worker
 // takes the queued requests in order while the batch's device footprint (the
 // batched solve's workspace, staged host inputs and outputs) fits its memory
 // and the graph and edge caps hold, runs them as one batched solve and copies
@@ -11,12 +11,21 @@
 // their own streams, so the H2D staging and host-side scatter of one batch
 // overlap the kernels of another; requests that arrive meanwhile form the
 // next batch.
+//
+// With record_rounds set, a batch replays a batched solve recorded for its
+// capacity bucket (graph count rounded up to a power of two, vertex and edge
+// counts to a quarter octave) instead of launching the solve eagerly; each
+// slot keeps its most recently used recordings.  Host inputs are copied into
+// pinned staging by the slot's copy threads in chunks, each chunk's
+// host-to-device copy running while the next chunk is filled.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -43,6 +52,98 @@ struct Req {
   bool claimed = false;  // a phem_server_wait is waiting for it
 };
 
+// A fixed set of threads that run fn(0 .. count-1) together with the caller.
+class CopyPool {
+ public:
+  explicit CopyPool(unsigned n) {
+    for (unsigned i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  void run(uint64_t count, const std::function<void(uint64_t)> &fn) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      fn_ = &fn;
+      count_ = count;
+      next_.store(0);
+      pending_ = count;
+      gen_++;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [&] { return pending_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void work() {
+    for (;;) {
+      const uint64_t i = next_.fetch_add(1);
+      if (i >= count_) return;
+      (*fn_)(i);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--pending_ == 0) done_cv_.notify_all();
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || (gen_ != seen && fn_); });
+        if (stop_) return;
+        seen = gen_;
+      }
+      work();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(uint64_t)> *fn_ = nullptr;
+  uint64_t count_ = 0, pending_ = 0, gen_ = 0;
+  std::atomic<uint64_t> next_{0};
+  bool stop_ = false;
+};
+
+// A batch shape a recording is made for.
+struct Bucket {
+  uint32_t k_cap;
+  uint64_t n_cap, m_cap;
+  bool operator==(const Bucket &o) const {
+    return k_cap == o.k_cap && n_cap == o.n_cap && m_cap == o.m_cap;
+  }
+};
+
+// x rounded up to a quarter octave (at least lo): at most 25 % padding.
+uint64_t quarter_octave(uint64_t x, uint64_t lo) {
+  if (x < lo) x = lo;
+  uint64_t e = 63 - __builtin_clzll(x);
+  const uint64_t step = e >= 2 ? (1ull << (e - 2)) : 1;
+  return (x + step - 1) / step * step;
+}
+
+Bucket bucket_of(uint32_t k, uint64_t n, uint64_t m) {
+  uint32_t kc = 8;
+  while (kc < k + 1) kc <<= 1;
+  return Bucket{kc, quarter_octave(n, 4096), quarter_octave(m, 65536)};
+}
+
+struct Recorded {
+  Bucket b;
+  phem_batch_graph_t *g = nullptr;
+  uint64_t used = 0;
+};
+
+constexpr size_t MAX_RECORDINGS = 16;  // per slot
+
 struct Slot {
   hipStream_t stream = nullptr;
   void *dev = nullptr;
@@ -50,9 +151,18 @@ struct Slot {
   char *hout = nullptr;     // pinned: results on their way to the requests
   uint64_t hout_bytes = 0;
   std::thread th;
+  std::unique_ptr<CopyPool> pool;
+  std::vector<Recorded> rec;
+  uint64_t tick = 0;
 };
 
 uint64_t a256(uint64_t x) { return align_up(x ? x : 1, 256); }
+
+// Device bytes of the outputs and staging of a batch (capacities k, n; mh
+// edges from host memory) after its workspace.
+uint64_t io_bytes(uint64_t n, uint64_t mh, uint32_t k) {
+  return 3 * a256(4 * mh) + 2 * a256(4 * n) + a256(4ull * k) + 2 * a256(8ull * k);
+}
 
 }  // namespace
 }  // namespace phem
@@ -72,20 +182,28 @@ struct phem_server {
   bool stop = false;
   // statistics since the last reset
   uint64_t submitted = 0, completed = 0, refused = 0, batches = 0, edges = 0,
-           batch_graphs = 0;
+           batch_graphs = 0, recorded_batches = 0, record_fallbacks = 0,
+           recordings = 0;
   double first_arrival = 0, last_done = 0;
   std::vector<double> lat;
 
   // Device bytes of a batch of k graphs with n vertices, m edges of which
-  // mh come from host memory.
+  // mh come from host memory, solved eagerly (exact sizes) or replaying the
+  // recording of its capacity bucket.
   int footprint(uint64_t n, uint64_t m, uint64_t mh, uint32_t k,
                 uint64_t *bytes) const {
     uint64_t ws = 0;
     int rc = batch_workspace_bytes(n, m, k, &ws);
     if (rc) return rc;
-    *bytes = a256(ws) + 3 * a256(4 * mh) + 2 * a256(4 * n) + a256(4ull * k) +
-             2 * a256(8ull * k);
+    *bytes = a256(ws) + io_bytes(n, mh, k);
     return PHEM_OK;
+  }
+  bool recorded_fits(uint64_t n, uint64_t m, uint64_t mh, uint32_t k) const {
+    const Bucket b = bucket_of(k, n, m);
+    if (b.n_cap > PHEM_MAX_VERTICES || b.m_cap > PHEM_MAX_EDGES) return false;
+    uint64_t ws = 0;
+    if (batch_workspace_bytes(b.n_cap, b.m_cap, b.k_cap, &ws) != PHEM_OK) return false;
+    return a256(ws) + io_bytes(b.n_cap, mh, b.k_cap) <= slot_bytes;
   }
   bool fits(uint64_t n, uint64_t m, uint64_t mh, uint32_t k) const {
     if (k > o.max_graphs || m > o.max_batch_edges || n > o.max_batch_edges)
@@ -94,8 +212,20 @@ struct phem_server {
     if (footprint(n, m, mh, k, &b) != PHEM_OK) return false;
     return b <= slot_bytes;
   }
+  // Batches are formed so that they can replay their bucket's recording; a
+  // request too large for that alone still runs, eagerly.
+  bool admits(uint64_t n, uint64_t m, uint64_t mh, uint32_t k) const {
+    if (!fits(n, m, mh, k)) return false;
+    return !o.record_rounds || k == 1 || recorded_fits(n, m, mh, k);
+  }
   void worker(int si);
   void run_batch(Slot &S, std::vector<Req *> &batch);
+  int stage_inputs(Slot &S, const std::vector<Req *> &batch,
+                   const std::vector<uint64_t> &hoff, uint64_t mh,
+                   uint32_t *d_su, uint32_t *d_sv, uint32_t *d_sw);
+  phem_batch_graph_t *recording(Slot &S, const Bucket &b, char *ws,
+                                uint64_t ws_bytes, uint32_t *ids, uint32_t *cnt,
+                                uint32_t *lab, uint64_t *wfx, uint64_t *bad);
 };
 
 void phem_server::worker(int si) {
@@ -120,7 +250,7 @@ void phem_server::worker(int si) {
         Req *r = queue.front();
         const uint64_t rn = r->q.n, rm = r->q.m;
         const uint64_t rmh = (r->q.flags & PHEM_REQ_DEVICE) ? 0 : rm;
-        if (!fits(n + rn, m + rm, mh + rmh, (uint32_t)batch.size() + 1)) break;
+        if (!admits(n + rn, m + rm, mh + rmh, (uint32_t)batch.size() + 1)) break;
         n += rn;
         m += rm;
         mh += rmh;
@@ -131,6 +261,92 @@ void phem_server::worker(int si) {
     }
     if (!batch.empty()) run_batch(S, batch);
   }
+}
+
+// Host edges of the batch -> pinned staging -> device, in chunks of the
+// staging arrays: the slot's copy threads fill chunk c + 1 while chunk c's
+// host-to-device copy runs.  Staging index i of a field belongs to the host
+// request g with hoff[g] <= i < hoff[g + 1].
+int phem_server::stage_inputs(Slot &S, const std::vector<Req *> &batch,
+                              const std::vector<uint64_t> &hoff, uint64_t mh,
+                              uint32_t *d_su, uint32_t *d_sv, uint32_t *d_sw) {
+  if (!mh) return PHEM_OK;
+  if (!S.hin) {
+    hipError_t e = hipHostMalloc((void **)&S.hin, 12 * o.max_batch_edges);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      S.hin = nullptr;
+      return set_err(PHEM_ENOMEM, "pinned input staging: %s", hipGetErrorString(e));
+    }
+  }
+  const uint32_t k = (uint32_t)batch.size();
+  constexpr uint64_t CHUNK = 4ull << 20;  // elements (16 MB) per chunk
+  constexpr uint64_t PIECE = 1ull << 18;  // elements per copy-thread task
+  uint32_t *hst[3] = {S.hin, S.hin + mh, S.hin + 2 * mh};
+  uint32_t *dst[3] = {d_su, d_sv, d_sw};
+  for (int f = 0; f < 3; f++) {
+    for (uint64_t c0 = 0; c0 < mh; c0 += CHUNK) {
+      const uint64_t c1 = std::min(mh, c0 + CHUNK);
+      const uint64_t pieces = (c1 - c0 + PIECE - 1) / PIECE;
+      S.pool->run(pieces, [&](uint64_t p) {
+        uint64_t i = c0 + p * PIECE;
+        const uint64_t end = std::min(c1, i + PIECE);
+        // the host request holding staging index i
+        uint32_t g = (uint32_t)(std::upper_bound(hoff.begin(), hoff.begin() + k + 1, i) -
+                                hoff.begin()) - 1;
+        while (i < end) {
+          while (hoff[g + 1] <= i) g++;
+          const phem_request &q = batch[g]->q;
+          const uint32_t *src = f == 0 ? q.u : f == 1 ? q.v : q.w;
+          const uint64_t j = i - hoff[g];
+          const uint64_t len = std::min(end, hoff[g + 1]) - i;
+          std::memcpy(hst[f] + i, src + j, 4 * len);
+          i += len;
+        }
+      });
+      const hipError_t e = hipMemcpyAsync(dst[f] + c0, hst[f] + c0, 4 * (c1 - c0),
+                                          hipMemcpyHostToDevice, S.stream);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(PHEM_EHIP, "staging copy: %s", hipGetErrorString(e));
+      }
+    }
+  }
+  return PHEM_OK;
+}
+
+// The slot's recording for bucket b (made on first use; the least recently
+// used one is dropped beyond MAX_RECORDINGS), or NULL if recording failed.
+phem_batch_graph_t *phem_server::recording(Slot &S, const Bucket &b, char *ws,
+                                           uint64_t ws_bytes, uint32_t *ids,
+                                           uint32_t *cnt, uint32_t *lab,
+                                           uint64_t *wfx, uint64_t *bad) {
+  S.tick++;
+  for (auto &r : S.rec)
+    if (r.b == b) {
+      r.used = S.tick;
+      return r.g;
+    }
+  if (S.rec.size() >= MAX_RECORDINGS) {
+    auto lru = std::min_element(S.rec.begin(), S.rec.end(),
+                                [](const Recorded &x, const Recorded &y) {
+                                  return x.used < y.used;
+                                });
+    (void)hipStreamSynchronize(S.stream);  // no replay of it is in flight
+    phem_batch_graph_destroy(lru->g);
+    S.rec.erase(lru);
+  }
+  phem_batch_graph_t *g = nullptr;
+  if (phem_batch_graph_create(b.k_cap, b.n_cap, b.m_cap, o.wmode, ws, ws_bytes,
+                              ids, cnt, lab, wfx, bad, nullptr,
+                              (int)o.record_rounds, &g) != PHEM_OK)
+    return nullptr;
+  S.rec.push_back(Recorded{b, g, S.tick});
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    recordings++;
+  }
+  return g;
 }
 
 void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
@@ -144,8 +360,11 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
     hoff[g + 1] = hoff[g] + ((q.flags & PHEM_REQ_DEVICE) ? 0 : q.m);
   }
   const uint64_t n = voff[k], m = eoff[k], mh = hoff[k];
+  // a recorded replay when the batch's bucket fits the slot, else eager
+  const bool rec = o.record_rounds && recorded_fits(n, m, mh, k);
+  const Bucket b = rec ? bucket_of(k, n, m) : Bucket{k, n, m};
   uint64_t ws = 0;
-  int rc = batch_workspace_bytes(n, m, k, &ws);
+  int rc = batch_workspace_bytes(b.n_cap, b.m_cap, b.k_cap, &ws);
   // carve the slot's device memory
   char *base = (char *)S.dev;
   uint64_t off = 0;
@@ -155,59 +374,13 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
     return p;
   };
   char *d_ws = take(ws);
+  uint32_t *d_ids = (uint32_t *)take(4 * b.n_cap), *d_lab = (uint32_t *)take(4 * b.n_cap);
+  uint32_t *d_cnt = (uint32_t *)take(4ull * b.k_cap);
+  uint64_t *d_wfx = (uint64_t *)take(8ull * b.k_cap), *d_bad = (uint64_t *)take(8ull * b.k_cap);
   uint32_t *d_su = (uint32_t *)take(4 * mh), *d_sv = (uint32_t *)take(4 * mh),
            *d_sw = (uint32_t *)take(4 * mh);
-  uint32_t *d_ids = (uint32_t *)take(4 * n), *d_lab = (uint32_t *)take(4 * n);
-  uint32_t *d_cnt = (uint32_t *)take(4ull * k);
-  uint64_t *d_wfx = (uint64_t *)take(8ull * k), *d_bad = (uint64_t *)take(8ull * k);
-  // host inputs: into pinned staging (several threads for large batches),
-  // then one copy per field
+  if (rc == PHEM_OK) rc = stage_inputs(S, batch, hoff, mh, d_su, d_sv, d_sw);
   std::vector<const uint32_t *> pu(k), pv(k), pw(k);
-  if (rc == PHEM_OK && mh && !S.hin) {
-    hipError_t e = hipHostMalloc((void **)&S.hin, 12 * o.max_batch_edges);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      S.hin = nullptr;
-      rc = set_err(PHEM_ENOMEM, "pinned input staging: %s", hipGetErrorString(e));
-    }
-  }
-  if (rc == PHEM_OK && mh) {
-    uint32_t *hu = S.hin, *hv = S.hin + mh, *hw = S.hin + 2 * mh;
-    auto stage = [&](uint32_t g0, uint32_t g1) {
-      for (uint32_t g = g0; g < g1; g++) {
-        const phem_request &q = batch[g]->q;
-        if ((q.flags & PHEM_REQ_DEVICE) || !q.m) continue;
-        std::memcpy(hu + hoff[g], q.u, 4 * q.m);
-        std::memcpy(hv + hoff[g], q.v, 4 * q.m);
-        std::memcpy(hw + hoff[g], q.w, 4 * q.m);
-      }
-    };
-    const uint64_t bytes = 12 * mh;
-    const unsigned nth = bytes < (8u << 20) ? 1u : 4u;
-    if (nth == 1 || k == 1) {
-      stage(0, k);
-    } else {
-      // contiguous runs of graphs with about the same edge count per thread
-      std::vector<std::thread> ts;
-      uint32_t g = 0;
-      for (unsigned t = 0; t < nth; t++) {
-        const uint64_t target = mh * (t + 1) / nth;
-        uint32_t h = (t + 1 == nth) ? k : g;
-        while (h < k && hoff[h] < target) h++;
-        if (h > g) {
-          ts.emplace_back(stage, g, h);
-          g = h;
-        }
-      }
-      for (auto &t : ts) t.join();
-    }
-    hipError_t e = hipMemcpyAsync(d_su, hu, 4 * mh, hipMemcpyHostToDevice, S.stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_sv, hv, 4 * mh, hipMemcpyHostToDevice, S.stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_sw, hw, 4 * mh, hipMemcpyHostToDevice, S.stream);
-    if (e != hipSuccess) rc = set_err(PHEM_EHIP, "staging copy: %s", hipGetErrorString(e));
-  }
   for (uint32_t g = 0; g < k; g++) {
     const phem_request &q = batch[g]->q;
     if (q.flags & PHEM_REQ_DEVICE) {
@@ -220,8 +393,23 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
       pw[g] = d_sw + hoff[g];
     }
   }
-  phem_mst_stats st{};
-  if (rc == PHEM_OK) {
+  bool replayed = false, fell_back = false;
+  if (rc == PHEM_OK && rec) {
+    phem_mst_stats st{};
+    phem_batch_graph_t *g = recording(S, b, d_ws, ws, d_ids, d_cnt, d_lab, d_wfx, d_bad);
+    if (g) {
+      rc = phem_batch_graph_launch(g, k, voff.data(), eoff.data(), pu.data(),
+                                   pv.data(), pw.data(), S.stream);
+      if (rc == PHEM_OK) rc = phem_batch_graph_stats(g, &st, S.stream);
+      replayed = true;
+      if (rc == PHEM_ENOCONV) {  // more rounds than recorded: solve eagerly
+        fell_back = true;
+        rc = PHEM_OK;
+      }
+    }
+  }
+  if (rc == PHEM_OK && (!replayed || fell_back)) {
+    phem_mst_stats st{};
     BatchArgs A{};
     A.k = k;
     A.h_voff = voff.data();
@@ -243,10 +431,10 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
   // results: one copy per field into the pinned buffer, then to each request
   char *h = S.hout;
   uint32_t *h_cnt = (uint32_t *)h;
-  uint64_t *h_wfx = (uint64_t *)(h + a256(4ull * k));
-  uint64_t *h_bad = (uint64_t *)((char *)h_wfx + a256(8ull * k));
-  uint32_t *h_ids = (uint32_t *)((char *)h_bad + a256(8ull * k));
-  uint32_t *h_lab = (uint32_t *)((char *)h_ids + a256(4 * n));
+  uint64_t *h_wfx = (uint64_t *)(h + a256(4ull * o.max_graphs));
+  uint64_t *h_bad = (uint64_t *)((char *)h_wfx + a256(8ull * o.max_graphs));
+  uint32_t *h_ids = (uint32_t *)((char *)h_bad + a256(8ull * o.max_graphs));
+  uint32_t *h_lab = (uint32_t *)((char *)h_ids + a256(4 * o.max_batch_edges));
   if (rc == PHEM_OK) {
     hipError_t e = hipMemcpyAsync(h_cnt, d_cnt, 4ull * k, hipMemcpyDeviceToHost, S.stream);
     if (e == hipSuccess)
@@ -269,24 +457,23 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
         if (q.labels && q.n) std::memcpy(q.labels, h_lab + voff[g], 4ull * q.n);
       }
     };
-    // several threads for large batches, contiguous runs of graphs with
-    // about the same vertex count each
-    const unsigned nth = n < (2u << 20) ? 1u : 4u;
-    if (nth == 1 || k == 1) {
+    // the slot's copy threads for large batches, contiguous runs of graphs
+    // with about the same vertex count each
+    if (n < (2u << 20) || k == 1) {
       scatter(0, k);
     } else {
-      std::vector<std::thread> ts;
+      constexpr uint32_t PARTS = 8;
+      std::vector<uint32_t> cut(PARTS + 1, k);
+      cut[0] = 0;
       uint32_t g = 0;
-      for (unsigned t = 0; t < nth; t++) {
-        const uint64_t target = n * (t + 1) / nth;
-        uint32_t h = (t + 1 == nth) ? k : g;
-        while (h < k && voff[h] < target) h++;
-        if (h > g) {
-          ts.emplace_back(scatter, g, h);
-          g = h;
-        }
+      for (uint32_t t = 1; t < PARTS; t++) {
+        const uint64_t target = n * t / PARTS;
+        while (g < k && voff[g] < target) g++;
+        cut[t] = g;
       }
-      for (auto &t : ts) t.join();
+      S.pool->run(PARTS, [&](uint64_t t) {
+        if (cut[t + 1] > cut[t]) scatter(cut[t], cut[t + 1]);
+      });
     }
   } else {
     (void)hipStreamSynchronize(S.stream);
@@ -323,6 +510,8 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
     last_done = std::max(last_done, t1);
     batches++;
     batch_graphs += k;
+    if (replayed) recorded_batches++;
+    if (fell_back) record_fallbacks++;
   }
   in_flight -= k;
   cv_done.notify_all();
@@ -357,7 +546,11 @@ extern "C" int phem_server_create(const phem_server_opts *opts,
   S->o = o;
   S->slot_bytes = o.mem_bytes / o.slots / 256 * 256;
   S->slots.resize(o.slots);
+  // copy threads per slot: the box's share of host threads over the slots
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned per_slot = std::max(1u, std::min(8u, std::min(hw, 16u) / o.slots)) - 1;
   for (auto &sl : S->slots) {
+    sl.pool.reset(new CopyPool(per_slot));
     hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&sl.dev, S->slot_bytes);
     sl.hout_bytes = a256(4ull * o.max_graphs) + 2 * a256(8ull * o.max_graphs) +
@@ -388,6 +581,9 @@ extern "C" int phem_server_destroy(phem_server *srv) {
     if (sl.th.joinable()) sl.th.join();
   for (auto &sl : srv->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    for (auto &r : sl.rec) phem_batch_graph_destroy(r.g);
+    sl.rec.clear();
+    sl.pool.reset();
     if (sl.dev) (void)hipFree(sl.dev);
     if (sl.hin) (void)hipHostFree(sl.hin);
     if (sl.hout) (void)hipHostFree(sl.hout);
@@ -501,6 +697,9 @@ extern "C" int phem_server_get_stats(phem_server *srv, phem_server_stats *st) {
   s.mean_batch_graphs = srv->batches ? (double)srv->batch_graphs / srv->batches : 0;
   s.slot_bytes = srv->slot_bytes;
   s.slot_edges = srv->o.max_batch_edges;
+  s.recorded_batches = srv->recorded_batches;
+  s.record_fallbacks = srv->record_fallbacks;
+  s.recordings = srv->recordings;
   *st = s;
   return PHEM_OK;
 }
@@ -509,7 +708,8 @@ extern "C" int phem_server_reset_stats(phem_server *srv) {
   if (!srv) return set_err(PHEM_EINVAL, "NULL server");
   std::lock_guard<std::mutex> lk(srv->mu);
   srv->submitted = srv->completed = srv->refused = srv->batches = srv->edges = 0;
-  srv->batch_graphs = 0;
+  srv->batch_graphs = srv->recorded_batches = srv->record_fallbacks = 0;
+  srv->recordings = 0;
   srv->first_arrival = srv->last_done = 0;
   srv->lat.clear();
   return PHEM_OK;

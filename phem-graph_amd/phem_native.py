@@ -68,7 +68,8 @@ class MstStats(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("mem_bytes", ctypes.c_uint64), ("max_batch_edges", ctypes.c_uint64),
                 ("slots", ctypes.c_uint32), ("max_graphs", ctypes.c_uint32),
-                ("window_us", ctypes.c_uint32), ("wmode", ctypes.c_int32)]
+                ("window_us", ctypes.c_uint32), ("wmode", ctypes.c_int32),
+                ("record_rounds", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class Request(ctypes.Structure):
@@ -97,7 +98,9 @@ class ServerStats(ctypes.Structure):
                [(f, ctypes.c_double) for f in
                 ("first_arrival", "last_done", "lat_p50", "lat_p99", "lat_max", "lat_mean",
                  "mean_batch_graphs")] + \
-               [("slot_bytes", ctypes.c_uint64), ("slot_edges", ctypes.c_uint64)]
+               [("slot_bytes", ctypes.c_uint64), ("slot_edges", ctypes.c_uint64),
+                ("recorded_batches", ctypes.c_uint64), ("record_fallbacks", ctypes.c_uint64),
+                ("recordings", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -694,8 +697,9 @@ class Server:
     are kept referenced until the request is waited for."""
 
     def __init__(self, mem_bytes=0, slots=2, max_graphs=4096, max_batch_edges=0,
-                 window_us=0, wmode=W_F32):
-        o = ServerOpts(mem_bytes, max_batch_edges, slots, max_graphs, window_us, wmode)
+                 window_us=0, wmode=W_F32, record_rounds=0):
+        o = ServerOpts(mem_bytes, max_batch_edges, slots, max_graphs, window_us, wmode,
+                       record_rounds, 0)
         h = ctypes.c_void_p()
         _check(lib().phem_server_create(ctypes.byref(o), ctypes.byref(h)), "phem_server_create")
         self.h = h

@@ -263,3 +263,31 @@ def test_wait_claims_its_ticket(P):
         srv.wait(t)
     assert e.value.code == P.ENOTFOUND
     srv.close()
+
+
+@pytest.mark.parametrize("rounds", [12, 2])
+def test_recorded_batches_equal_oracle(P, rounds):
+    """With record_rounds set, batches replay the recorded solve of their
+    capacity bucket (host and device inputs mixed); with too few rounds they
+    fall back to the eager solve.  Either way every result equals the
+    oracle's."""
+    graphs = [host_graph(g, seed=7) for g in range(150)]
+    srv = P.Server(mem_bytes=512 << 20, slots=2, max_graphs=64, record_rounds=rounds)
+    out = [submit(srv, n, u, v, w, on_device=(g % 2 == 0))
+           for g, (n, u, v, w) in enumerate(graphs)]
+    srv.drain()
+    for g, (n, u, v, w) in enumerate(graphs):
+        t, ids, lab = out[g]
+        r = srv.wait(t)
+        assert r["status"] == 0, r["error"]
+        k_ids, k_lab = O.kruskal(n, u, v, w, 1)
+        assert np.array_equal(ids[: r["forest_count"]], k_ids), g
+        assert np.array_equal(lab[:n], k_lab), g
+        assert r["weight_fx"] == O.weight_sum(k_ids, w, 1)[0], g
+    st = srv.stats()
+    print(f"rounds={rounds}: {st['batches']} batches, {st['recorded_batches']} replayed, "
+          f"{st['record_fallbacks']} fell back, {st['recordings']} recordings", flush=True)
+    assert st["recorded_batches"] > 0 and st["recordings"] > 0
+    if rounds == 2:
+        assert st["record_fallbacks"] > 0
+    srv.close()
