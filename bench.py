@@ -137,13 +137,17 @@ def cpu_baseline(u_np, v_np, w_np, n, threads, runs):
     return len(u_np) / (mst_s + cc_s), mst_s, cc_s, ("reference" if ref_cc else "port")
 
 
-def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_graphs=256):
+def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_graphs=1024,
+                record_rounds=12):
     """The serving mix (graphs_n independent graphs of the mix, device
     resident): one-at-a-time plain solves, then the serving loop saturated
-    (all requests at once), then Poisson arrivals at `load` x the saturated
-    request rate, then the first host_graphs graphs again from host memory
-    (all at once; the rate includes the host-to-device copies).  Every request returns forest ids, labels and the exact
-    weight to host memory; latency = done - arrival."""
+    (all requests at once) and with Poisson arrivals at `load` x the saturated
+    request rate, each with batches solved eagerly and replaying recorded
+    solves (record_rounds rounds per phase), then the first host_graphs
+    graphs from host memory (all at once, the staging rate; then Poisson
+    arrivals, latencies including the host-to-device and result copies).
+    Every request returns forest ids, labels and the exact weight to host
+    memory; latency = done - arrival."""
     import numpy as np
     import phem_workloads as W
 
@@ -187,45 +191,70 @@ def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_gra
     print(f"# plain: {t_plain:.2f} s", file=sys.stderr, flush=True)
     del solver
 
-    srv = P.Server(mem_bytes=0, slots=slots, max_graphs=4096,
-                   max_batch_edges=max_batch_edges, wmode=P.W_F32)
-
-    def run_served(arrivals):
+    def run_served(srv, gl, outs, arrivals):
         srv.reset_stats()
         tick = []
         base = P.Server.clock()
-        for i, (n, u, v, w) in enumerate(graphs):
+        for i, (n, u, v, w) in enumerate(gl):
             if arrivals is not None:
                 at = base + arrivals[i]
                 while P.Server.clock() < at:
                     pass
             else:
                 at = -1.0
-            tick.append(srv.submit(n, u, v, w, out_ids[i], out_lab[i], at))
+            tick.append(srv.submit(n, u, v, w, outs[0][i], outs[1][i], at))
         res = [srv.wait(t) for t in tick]
         st = srv.stats()
         wall = st["last_done"] - st["first_arrival"]
         return res, st, wall
 
-    run_served(None)  # warm-up: slot memory and pinned buffers
-    res, st, wall = run_served(None)
-    bad = sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res))
-    cap_eps = edges / wall
-    rate = load * graphs_n / wall  # requests per second
+    def leg(st, wall, e, extra=None):
+        d = {"edges_per_s": e / wall, "wall_s": round(wall, 4), "batches": st["batches"],
+             "mean_batch_graphs": round(st["mean_batch_graphs"], 1),
+             "lat_p50_ms": round(st["lat_p50"] * 1e3, 3),
+             "lat_p99_ms": round(st["lat_p99"] * 1e3, 3),
+             "recorded_batches": st["recorded_batches"],
+             "record_fallbacks": st["record_fallbacks"], "recordings": st["recordings"]}
+        d.update(extra or {})
+        return d
+
+    bad = 0
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, graphs_n))
-    res_p, st_p, wall_p = run_served(arrivals)
-    bad += sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res_p))
-    # host-resident inputs (the PCIe-inclusive rate) on a prefix of the mix
+    modes = {}
+    for label, rr in (("eager", 0), ("recorded", record_rounds)):
+        srv = P.Server(mem_bytes=0, slots=slots, max_graphs=4096,
+                       max_batch_edges=max_batch_edges, wmode=P.W_F32, record_rounds=rr)
+        run_served(srv, graphs, (out_ids, out_lab), None)  # warm-up: memory, recordings
+        res, st, wall = run_served(srv, graphs, (out_ids, out_lab), None)
+        bad += sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res))
+        rate = load * graphs_n / wall  # requests per second
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, graphs_n))
+        res_p, st_p, wall_p = run_served(srv, graphs, (out_ids, out_lab), arrivals)
+        bad += sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res_p))
+        srv.close()
+        modes[label] = {"saturated": leg(st, wall, edges),
+                        "poisson": leg(st_p, wall_p, edges,
+                                       {"load": load, "rate_req_per_s": round(rate, 1)})}
+        print(f"# served {label}: saturated {edges / wall / 1e9:.2f} G edges/s, poisson "
+              f"p50 {st_p['lat_p50'] * 1e3:.2f} ms p99 {st_p['lat_p99'] * 1e3:.2f} ms",
+              file=sys.stderr, flush=True)
+    best = max(modes, key=lambda k: modes[k]["saturated"]["edges_per_s"])
+    # host-resident inputs on a prefix of the mix: every latency includes the
+    # staging into pinned memory, the host-to-device copies and the result
+    # copies back; all at once (the staging rate), then Poisson arrivals
     hk = min(host_graphs, graphs_n)
     hg = [(n, u.cpu().numpy().view(np.uint32), v.cpu().numpy().view(np.uint32),
            w.cpu().numpy().view(np.uint32)) for n, u, v, w in graphs[:hk]]
     h_edges = sum(len(g[1]) for g in hg)
-    srv.reset_stats()
-    tick = [srv.submit(n, u, v, w, out_ids[i], out_lab[i]) for i, (n, u, v, w) in enumerate(hg)]
-    res_h = [srv.wait(t) for t in tick]
-    st_h = srv.stats()
+    srv = P.Server(mem_bytes=0, slots=slots, max_graphs=4096, max_batch_edges=max_batch_edges,
+                   wmode=P.W_F32, record_rounds=record_rounds if best == "recorded" else 0)
+    run_served(srv, hg, (out_ids, out_lab), None)  # warm-up
+    res_h, st_h, wall_h = run_served(srv, hg, (out_ids, out_lab), None)
     bad += sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res_h))
+    rate_h = load * hk / wall_h
+    arr_h = np.cumsum(rng.exponential(1.0 / rate_h, hk))
+    res_hp, st_hp, wall_hp = run_served(srv, hg, (out_ids, out_lab), arr_h)
+    bad += sum(r["status"] != 0 or r["weight_fx"] != plain_w[i] for i, r in enumerate(res_hp))
     srv.close()
     lp = sorted(lat_plain)
     pct = lambda l, p: l[max(0, int(np.ceil(p * len(l))) - 1)]
@@ -235,25 +264,21 @@ def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_gra
         "plain": {"edges_per_s": edges / t_plain, "wall_s": round(t_plain, 4),
                   "lat_p50_ms": round(pct(lp, .5) * 1e3, 3),
                   "lat_p99_ms": round(pct(lp, .99) * 1e3, 3)},
-        "served_saturated": {"edges_per_s": cap_eps, "wall_s": round(wall, 4),
-                             "batches": st["batches"],
-                             "mean_batch_graphs": round(st["mean_batch_graphs"], 1),
-                             "lat_p50_ms": round(st["lat_p50"] * 1e3, 3),
-                             "lat_p99_ms": round(st["lat_p99"] * 1e3, 3)},
-        "served_poisson": {"load": load, "rate_req_per_s": round(rate, 1),
-                           "edges_per_s": edges / wall_p, "wall_s": round(wall_p, 4),
-                           "batches": st_p["batches"],
-                           "mean_batch_graphs": round(st_p["mean_batch_graphs"], 1),
-                           "lat_p50_ms": round(st_p["lat_p50"] * 1e3, 3),
-                           "lat_p99_ms": round(st_p["lat_p99"] * 1e3, 3)},
-        "served_host_inputs": {"graphs": hk, "edges": h_edges,
-                               "edges_per_s": h_edges / (st_h["last_done"] - st_h["first_arrival"]),
-                               "batches": st_h["batches"],
-                               "lat_p50_ms": round(st_h["lat_p50"] * 1e3, 3),
-                               "lat_p99_ms": round(st_h["lat_p99"] * 1e3, 3)},
-        "speedup_saturated_vs_plain": round(cap_eps * t_plain / edges, 2),
-        "speedup_poisson_vs_plain": round(edges / wall_p * t_plain / edges, 2),
-        "slots": slots, "max_batch_edges": max_batch_edges,
+        "served_saturated": modes[best]["saturated"],
+        "served_poisson": modes[best]["poisson"],
+        "served_mode": best,
+        "modes": modes,
+        "served_host_inputs": leg(st_h, wall_h, h_edges,
+                                  {"graphs": hk, "edges": h_edges,
+                                   "staged_GB_per_s": round(12 * h_edges / wall_h / 1e9, 2)}),
+        "served_host_poisson": leg(st_hp, wall_hp, h_edges,
+                                   {"graphs": hk, "load": load,
+                                    "rate_req_per_s": round(rate_h, 1),
+                                    "latency": "done - arrival, including staging, "
+                                               "host-to-device and result copies"}),
+        "speedup_saturated_vs_plain": round(modes[best]["saturated"]["edges_per_s"] * t_plain / edges, 2),
+        "speedup_poisson_vs_plain": round(modes[best]["poisson"]["edges_per_s"] * t_plain / edges, 2),
+        "slots": slots, "max_batch_edges": max_batch_edges, "record_rounds": record_rounds,
         "weights_mismatch_vs_plain": bad,
     }
 
@@ -288,6 +313,82 @@ def replay_bench(P, torch, scale, m_log2, seed, reps=30):
     return {"workload": f"rmat_n2^{scale}_m2^{m_log2}", "eager_ms": round(t_e * 1e3, 4),
             "replay_ms": round(t_g * 1e3, 4), "rounds": st["rounds"],
             "replay_equals_eager": same}
+
+
+def configs_bench(P, torch, reps=5):
+    """The other single-graph configurations, timed like the headline step
+    (device-resident inputs, HIP events, median of `reps` solves after a
+    warm-up): the 4096 x 4096 tie-heavy grid (20 % of edges deleted, integer
+    weights 1..16), the randomly permuted path on 2^22 vertices plus 2^20
+    chords (deep rounds), both generated exactly as the parity tests
+    generate them, and the headline RMAT graph with its vertex ids randomly
+    permuted (the same edges under a layout without id locality).  Each
+    reports MST and CC time, edges/s, and for the MST the algorithmic bytes
+    of its own round log (32 S_m - 20 m + 16 S_c + 12 n, exact when the
+    canonical schedule runs) over the time."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import synth as S
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ts = []
+        for _ in range(reps):
+            ev[0].record(stream)
+            out = fn()
+            ev[1].record(stream)
+            ev[1].synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        return sorted(ts)[len(ts) // 2], out
+
+    def one(name, n, du, dv, dw):
+        m = du.numel()
+        solver = P.MstSolver(n, m, device=dev)
+        mst_ms, (_, _, st) = timed(lambda: solver.solve(du, dv, dw, P.W_F32))
+        de = P.pack_edges(torch.minimum(du, dv), torch.maximum(du, dv))
+        par = torch.empty(n, dtype=torch.int32, device=dev)
+        lab = torch.empty(n, dtype=torch.int32, device=dev)
+        cc_ms, _ = timed(lambda: P._check(P.lib().phem_cc_solve(
+            n, P._p(de), m, P._p(par), None, P._p(lab), P._stream()), "phem_cc_solve"))
+        can = solver.solve(du, dv, dw, P.W_F32, light_factor=0.0)[2]
+        bm = b_mst(m, n, can["m_log"], can["c_log"])
+        del solver, de, par, lab
+        torch.cuda.empty_cache()
+        print(f"# {name}: mst {mst_ms:.3f} ms cc {cc_ms:.3f} ms rounds {st['rounds']}",
+              file=sys.stderr, flush=True)
+        return {"n": n, "m": m, "mst_ms": round(mst_ms, 4), "mst_edges_per_s": m / (mst_ms * 1e-3),
+                "cc_ms": round(cc_ms, 4), "cc_edges_per_s": m / (cc_ms * 1e-3),
+                "rounds": st["rounds"], "schedule": "split" if st["tau"] != P.NO_EDGE else "canonical",
+                "forest": st["mst_count"],
+                "B_canonical_bytes": bm,
+                "mst_roofline_frac_canonical_bytes": round(bm / (mst_ms * 1e-3) / 8e12, 4),
+                "cc_roofline_frac": round(b_cc(m, n) / (cc_ms * 1e-3) / 8e12, 4)}
+
+    out = {}
+    u, v, w = S.grid(4096, 3)
+    t = [torch.from_numpy(x.view(np.int32)).to(dev) for x in (u, v, w)]
+    out["grid_4096_tie_heavy"] = one("grid 4096^2", 4096 * 4096, *t)
+    del t
+    u, v, w = S.path_chords(1 << 22, 1 << 20, 4)
+    t = [torch.from_numpy(x.view(np.int32)).to(dev) for x in (u, v, w)]
+    out["path_2^22+2^20_chords"] = one("path 2^22 + chords", 1 << 22, *t)
+    del t
+    # RMAT 2^24 / 2^28 (the headline graph) with vertex ids permuted
+    du, dv, dw = P.gen_rmat(24, 1 << 28, 2, positive=True, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(9)
+    perm = torch.randperm(1 << 24, device=dev, generator=g, dtype=torch.int64).to(torch.int32)
+    du = perm[du.long()]
+    dv = perm[dv.long()]
+    del perm
+    out["rmat_n2^24_m2^28_permuted"] = one("rmat permuted", 1 << 24, du, dv, dw)
+    del du, dv, dw
+    torch.cuda.empty_cache()
+    return out
 
 
 def stream_bench(P, torch, scale, m0_log2, batches, batch_log2, seed):
@@ -353,6 +454,8 @@ def main():
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming run")
     ap.add_argument("--no-replay", action="store_true",
                     help="skip the recorded-solve (HIP graph) run")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the grid, path and permuted-RMAT runs")
     ap.add_argument("--serve-graphs", type=int, default=4096)
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-batch-log2", type=int, default=27,
@@ -551,6 +654,8 @@ def main():
         out["streaming"] = stream_bench(P, torch, 22, 24, 256, 20, 4)
     if not args.no_replay:
         out["recorded_solve"] = replay_bench(P, torch, 16, 20, 1)
+    if not args.no_configs:
+        out["configs"] = configs_bench(P, torch)
 
     print(json.dumps(out), flush=True)
 
