@@ -92,6 +92,11 @@ typedef struct {
   uint32_t cc_ept_filter;   /* records per thread of the filtered unions:
                                4, 8 (default), 16 */
   uint32_t reserved2;
+  double cpu_share;         /* phem_msf_host_chunks with PHEM_CPU_EDGES_AUTO:
+                               the host reduces cpu_share x (edges before
+                               the last chunk) edges of the last chunk
+                               (>= 0; default 0: on MI355X every host share
+                               measured slowed the solve; NaN: unchanged) */
 } phem_tuning;
 int phem_get_tuning(phem_tuning *t);
 int phem_set_tuning(const phem_tuning *t);
@@ -504,6 +509,42 @@ int phem_server_poll(phem_server *srv, uint64_t ticket, phem_result *res,
 int phem_server_drain(phem_server *srv);
 int phem_server_get_stats(phem_server *srv, phem_server_stats *st);
 int phem_server_reset_stats(phem_server *srv);
+
+/* ---------------------------------------------------------------------------
+ * Host-resident edge lists streamed through the device in chunks.
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+  double seconds;      /* the whole call */
+  double cpu_seconds;  /* the host's share (0 without one) */
+  uint64_t chunks;     /* chunks folded in on the device */
+  uint64_t cpu_edges;  /* edges the host reduced */
+  uint64_t cpu_kept;   /* edges of their forest */
+  uint64_t reserved[3];
+} phem_chunks_stats;
+
+#define PHEM_CPU_EDGES_AUTO 0xFFFFFFFFFFFFFFFFull
+
+/* Minimum spanning forest of the host edges (u[i], v[i], w[i]), i < m, in
+ * chunks of chunk_edges folded one by one into a forest kept in HBM,
+ * F = MSF(F u chunk) (the reference's streamline path, MST/src/kernels.cu:
+ * 849-1154, whose chunks are the contiguous slices of the edge array).  The
+ * next chunk is copied to the device while the current one is folded in.
+ * cpu_edges: the first cpu_edges edges of the last chunk are reduced on the
+ * host (phem_cpu_msf) while the device folds in the other chunks, and the
+ * last chunk carries their forest instead (the heterogeneous streamline,
+ * MST/src/kernels.cu:1155-1826, which gives the host at most 10^8 edges);
+ * PHEM_CPU_EDGES_AUTO: cpu_share (phem_tuning) x the edges before the last
+ * chunk; 0, or a single chunk: none.  ids_out (n host slots) receives the
+ * forest's edge ids ascending, *count their number: the forest of the whole
+ * list under the (weight, id) order whatever the chunking.  Runs on the
+ * current device on streams of its own and returns when done.  stats may be
+ * NULL. */
+int phem_msf_host_chunks(uint32_t n, uint64_t m, const uint32_t *u,
+                         const uint32_t *v, const uint32_t *w, int wmode,
+                         uint64_t chunk_edges, uint64_t cpu_edges,
+                         uint32_t *ids_out, uint64_t *count,
+                         phem_chunks_stats *stats);
 
 /* ---------------------------------------------------------------------------
  * Host backend: the same results computed on the CPU.

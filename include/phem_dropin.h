@@ -74,10 +74,15 @@ void mmapPopulateEdgeArray(const std::string &filename, edge *edges,
  *  - use_streamline: chunks of chunk_size edges (or num_chunks_ideal
  *    chunks) folded into a forest kept on the device, F = MSF(F u chunk)
  *    (MST/src/kernels.cu:849-1154);
- *  - use_cpu_gpu_streamline: the same, with the last chunk first reduced to
- *    its own forest on the CPU (phem_cpu_msf) while the GPU folds in the
- *    others (MST/src/kernels.cu:1155-1826);
- *  - use_malloc_managed: runs the default path (device memory);
+ *  - use_cpu_gpu_streamline: the same, with part of the last chunk first
+ *    reduced to its own forest on the CPU (phem_cpu_msf) while the GPU folds
+ *    in the others (MST/src/kernels.cu:1155-1826): the part the cpu_share
+ *    knob gives (phem_tuning; none by default, the measured best here);
+ *  - use_malloc_managed (debug or not): the unified-memory mode
+ *    (MST/src/new_boruvka.cu:183-323) as a host-resident graph read by the
+ *    device in chunks sized to its free memory (phem_msf_host_chunks);
+ *    prints "Time to execute T microseconds", "total weight W" and "number
+ *    of filled edges K";
  *  - use_cpu_only (not in debug mode, as in the reference): the forest is
  *    computed on the host by the function registered with phem_set_cpu_mst
  *    (the caller's ParlayLib cpu_compute_pbbs, MST/src/utils.cu:887, behind

@@ -28,6 +28,13 @@ std::once_flag g_tune_once;
 std::atomic<uint32_t> g_mst_block{256}, g_mst_ept{8}, g_mst_ept_compact{16},
     g_mst_compact_log2{22}, g_cc_block{256}, g_cc_ept{16}, g_cc_ept_filter{8};
 std::atomic<double> g_mst_split_ratio{1.5};
+// host share of the chunked solve: none by default.  Measured on MI355X
+// with 16 host threads (RMAT 2^24 / 2^28 in chunks of 2^26; RMAT 2^22 / 2^24
+// plus 2^28 uniform edges in chunks of 2^24): the device path runs at
+// 2.9-3.7 G edges/s including the host-to-device copies, the host reduces
+// 0.02-0.1 G edges/s, and every host share tried (2^20 edges up to the
+// whole last chunk) made the solve slower.
+std::atomic<double> g_cpu_share{0.0};
 
 bool one_of(uint32_t x, std::initializer_list<uint32_t> allowed) {
   for (uint32_t a : allowed)
@@ -91,6 +98,7 @@ KernelShape kernel_shape() {
   k.cc_block = g_cc_block.load(std::memory_order_relaxed);
   k.cc_ept = g_cc_ept.load(std::memory_order_relaxed);
   k.cc_ept_filter = g_cc_ept_filter.load(std::memory_order_relaxed);
+  k.cpu_share = g_cpu_share.load(std::memory_order_relaxed);
   return k;
 }
 
@@ -288,6 +296,7 @@ extern "C" int phem_get_tuning(phem_tuning *t) {
   t->cc_ept = k.cc_ept;
   t->cc_ept_filter = k.cc_ept_filter;
   t->reserved2 = 0;
+  t->cpu_share = k.cpu_share;
   return PHEM_OK;
 }
 
@@ -323,7 +332,10 @@ extern "C" int phem_set_tuning(const phem_tuning *t) {
   if (t->cc_ept_filter && !one_of(t->cc_ept_filter, {4, 8, 16}))
     return phem::set_err(PHEM_EINVAL, "cc_ept_filter %u: 4, 8 or 16",
                          t->cc_ept_filter);
+  if (!std::isnan(t->cpu_share) && !(t->cpu_share >= 0.0 && t->cpu_share <= 1e6))
+    return phem::set_err(PHEM_EINVAL, "cpu_share must be in [0, 1e6]");
   phem::tuning_init();
+  if (!std::isnan(t->cpu_share)) phem::g_cpu_share.store(t->cpu_share);
   if (t->mst_block) phem::g_mst_block.store(t->mst_block);
   if (t->mst_ept) phem::g_mst_ept.store(t->mst_ept);
   if (t->mst_ept_compact) phem::g_mst_ept_compact.store(t->mst_ept_compact);

@@ -249,82 +249,50 @@ phem_cpu_mst_fn g_cpu_mst = nullptr;
 
 // The streamline paths: chunks of cs edges folded into a forest kept on the
 // device, F = MSF(F u chunk) (MST/src/kernels.cu:849-1154); with `hetero`
-// the last chunk is first reduced to its own forest on the CPU while the GPU
-// folds in the others (MST/src/kernels.cu:1155-1826).  Returns the forest
-// and the elapsed microseconds of the solve (reading excluded).
+// the last chunk is first reduced on the CPU while the GPU folds in the
+// others (MST/src/kernels.cu:1155-1826), by the measured rule of
+// PHEM_CPU_EDGES_AUTO instead of the reference's fixed 10^8 edges.  Returns
+// the forest in ascending edge id and the elapsed microseconds of the solve
+// (reading excluded).
 std::vector<edge> streamline_forest(const phem_graph_file &g, uint64_t cs,
                                     bool hetero, long long *usec) {
-  const uint32_t n = g.n;
-  const uint64_t m = g.m;
   const auto t0 = std::chrono::steady_clock::now();
-  phem_stream_t *st = nullptr;
-  die_phem(phem_stream_create(n, cs, PHEM_W_U32, &st), "phem_stream_create");
-  uint32_t *d = nullptr;
-  die_hip(hipMalloc(&d, 12 * cs), "hipMalloc (chunk)");
-  auto apply = [&](const uint32_t *hu, const uint32_t *hv, const uint32_t *hw,
-                   uint64_t k) {
-    die_hip(hipMemcpy(d, hu, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
-    die_hip(hipMemcpy(d + cs, hv, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
-    die_hip(hipMemcpy(d + 2 * cs, hw, 4 * k, hipMemcpyHostToDevice), "hipMemcpy");
-    die_phem(phem_stream_apply(st, d, d + cs, d + 2 * cs, k, nullptr),
-             "phem_stream_apply");
-  };
-  const uint64_t nchunks = m ? (m + cs - 1) / cs : 0;
-  const uint64_t last = nchunks ? (nchunks - 1) * cs : 0;  // first edge of the last chunk
-  // heterogeneous: the CPU share is the first 10^8 edges of the last chunk,
-  // as the reference caps it; the rest of that chunk stays unfiltered
-  std::vector<uint32_t> cpu_ids;
-  uint64_t cpu_cnt = 0, cpu_span = 0;
-  std::thread cpu;
-  if (hetero && nchunks > 1) {
-    cpu_span = std::min<uint64_t>(m - last, 100000000ull);
-    cpu_ids.resize(n ? n : 1);
-    cpu = std::thread([&] {
-      die_phem(phem_cpu_msf(n, cpu_span, g.u + last, g.v + last, g.w + last,
-                            PHEM_W_U32, cpu_ids.data(), &cpu_cnt, 0),
-               "phem_cpu_msf");
-    });
-  }
-  for (uint64_t c = 0; c < nchunks; c++) {
-    const uint64_t off = c * cs, k = m - off < cs ? m - off : cs;
-    if (c + 1 < nchunks || !cpu.joinable()) {
-      apply(g.u + off, g.v + off, g.w + off, k);
-      continue;
-    }
-    cpu.join();
-    std::vector<uint32_t> fu, fv, fw;  // filtered chunk, input order kept
-    for (uint64_t i = 0; i < cpu_cnt; i++) {
-      const uint64_t e = last + cpu_ids[i];
-      fu.push_back(g.u[e]);
-      fv.push_back(g.v[e]);
-      fw.push_back(g.w[e]);
-    }
-    for (uint64_t e = last + cpu_span; e < m; e++) {
-      fu.push_back(g.u[e]);
-      fv.push_back(g.v[e]);
-      fw.push_back(g.w[e]);
-    }
-    std::printf("%llu cut down to %llu\n", (unsigned long long)cpu_span,
-                (unsigned long long)cpu_cnt);
-    apply(fu.data(), fv.data(), fw.data(), fu.size());
-  }
+  std::vector<uint32_t> ids(g.n ? g.n : 1);
   uint64_t fc = 0;
-  const uint32_t *fu, *fv, *fw;
-  const uint64_t *fg;
-  die_phem(phem_stream_forest(st, &fc, &fu, &fv, &fw, &fg), "phem_stream_forest");
-  std::vector<uint32_t> hu(fc), hv(fc), hw(fc);
-  if (fc) {
-    die_hip(hipMemcpy(hu.data(), fu, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
-    die_hip(hipMemcpy(hv.data(), fv, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
-    die_hip(hipMemcpy(hw.data(), fw, 4 * fc, hipMemcpyDeviceToHost), "hipMemcpy");
-  }
+  phem_chunks_stats st{};
+  die_phem(phem_msf_host_chunks(g.n, g.m, g.u, g.v, g.w, PHEM_W_U32, cs,
+                                hetero ? PHEM_CPU_EDGES_AUTO : 0, ids.data(),
+                                &fc, &st),
+           "phem_msf_host_chunks");
+  if (hetero && st.cpu_edges)
+    std::printf("%llu cut down to %llu\n", (unsigned long long)st.cpu_edges,
+                (unsigned long long)st.cpu_kept);
   std::vector<edge> forest(fc);
-  for (uint64_t i = 0; i < fc; i++) forest[i] = edge{hu[i], hv[i], hw[i]};
-  die_hip(hipFree(d), "hipFree");
-  die_phem(phem_stream_destroy(st), "phem_stream_destroy");
+  for (uint64_t i = 0; i < fc; i++)
+    forest[i] = edge{g.u[ids[i]], g.v[ids[i]], g.w[ids[i]]};
   *usec = std::chrono::duration_cast<std::chrono::microseconds>(
               std::chrono::steady_clock::now() - t0).count();
   return forest;
+}
+
+// Chunk size of the managed mode: the largest of m, m/2, m/4, ... whose
+// phem_msf_host_chunks state (two forest|chunk edge buffers, the solve
+// workspace, per-vertex arrays, two staged chunks) takes at most 3/4 of the
+// device's free memory.
+uint64_t managed_chunk(uint32_t n, uint64_t m) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return m ? m : 1;
+  }
+  uint64_t c = m ? m : 1;
+  for (; c > (1u << 16); c = (c + 1) / 2) {
+    uint64_t ws = 0;
+    if (phem_mst_workspace_bytes(n, (uint64_t)n + c, &ws) != PHEM_OK) continue;
+    const uint64_t need = 40 * ((uint64_t)n + c) + ws + 20ull * n + 32 * c;
+    if (need <= free_b / 4 * 3) break;
+  }
+  return c;
 }
 
 }  // namespace
@@ -370,14 +338,35 @@ void new_boruvka(std::string filename, std::string result_filename, bool debug,
     phem_graph_file_free(&g);
     return;
   }
-  if (use_malloc_managed)
-    std::printf("malloc managed: runs in device memory\n");
-  if (debug && save_weighted_graph && !use_malloc_managed) {
+  if (use_malloc_managed) {
+    // The reference's unified-memory path (MST/src/new_boruvka.cu:183-323,
+    // reached at :378-383 and :453-458, debug or not) keeps the graph in
+    // memory the host filled and the device pages in on first touch.  Here
+    // the graph stays in the host memory the reader filled and the device
+    // reads it in chunks sized to its free memory (one chunk when the graph
+    // fits, as it does up to billions of edges in 288 GB): explicit copies
+    // instead of page faults, which need XNACK.
+    int dev = 0, attr = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&attr, hipDeviceAttributeManagedMemory, dev);
+    const uint64_t cs = managed_chunk(n, m);
+    std::printf("malloc managed (device managed-memory attribute %d): host-resident "
+                "graph in chunks of %llu edges\n", attr, (unsigned long long)cs);
+    long long us = 0;
+    std::vector<edge> forest = streamline_forest(g, cs, false, &us);
+    std::printf("Time to execute %lld microseconds\n", us);
+    const uint64_t w = report_forest(forest.data(), forest.size(), result_filename);
+    std::printf("total weight %llu\n", (unsigned long long)w);
+    std::printf("number of filled edges %llu\n", (unsigned long long)forest.size());
+    phem_graph_file_free(&g);
+    return;
+  }
+  if (debug && save_weighted_graph) {
     die_phem(phem_write_mtx_weighted("weighted.mtx", &g), "phem_write_mtx_weighted");
     phem_graph_file_free(&g);
     return;
   }
-  if (!debug && use_cpu_only && !use_malloc_managed) {
+  if (!debug && use_cpu_only) {
     // the CPU path (MST/src/new_boruvka.cu:468-492): the caller's solver
     // registered with phem_set_cpu_mst (the ParlayLib cpu_compute_pbbs
     // behind a plain-array adapter), else the library's host solver

@@ -30,6 +30,7 @@ struct KernelShape {
   uint32_t mst_block, mst_ept, mst_ept_compact, mst_compact_log2;
   double mst_split_ratio;
   uint32_t cc_block, cc_ept, cc_ept_filter;
+  double cpu_share;
 };
 KernelShape kernel_shape();
 

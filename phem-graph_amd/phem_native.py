@@ -119,14 +119,22 @@ class Tuning(ctypes.Structure):
                 ("mst_ept_compact", ctypes.c_uint32), ("mst_compact_log2", ctypes.c_uint32),
                 ("mst_split_ratio", ctypes.c_double),
                 ("cc_block", ctypes.c_uint32), ("cc_ept", ctypes.c_uint32),
-                ("cc_ept_filter", ctypes.c_uint32), ("reserved2", ctypes.c_uint32)]
+                ("cc_ept_filter", ctypes.c_uint32), ("reserved2", ctypes.c_uint32),
+                ("cpu_share", ctypes.c_double)]
+
+
+class ChunksStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("cpu_seconds", ctypes.c_double),
+                ("chunks", ctypes.c_uint64), ("cpu_edges", ctypes.c_uint64),
+                ("cpu_kept", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 3)]
 
 
 # the knobs of phem_tuning, in order (the reserved words excluded)
 TUNING_KNOBS = ("blocks_per_cu", "mst_light_factor", "cc_sample_factor", "mst_block",
                 "mst_ept", "mst_ept_compact", "mst_compact_log2", "mst_split_ratio",
-                "cc_block", "cc_ept", "cc_ept_filter")
-_FLOAT_KNOBS = ("mst_light_factor", "cc_sample_factor", "mst_split_ratio")
+                "cc_block", "cc_ept", "cc_ept_filter", "cpu_share")
+_FLOAT_KNOBS = ("mst_light_factor", "cc_sample_factor", "mst_split_ratio", "cpu_share")
+CPU_EDGES_AUTO = 0xFFFFFFFFFFFFFFFF
 
 
 _lib = None
@@ -207,6 +215,8 @@ def lib():
         "phem_get_tuning": (i32, [c.POINTER(Tuning)]),
         "phem_set_tuning": (i32, [c.POINTER(Tuning)]),
         "phem_cpu_msf": (i32, [u32, u64, vp, vp, vp, i32, vp, c.POINTER(c.c_uint64), i32]),
+        "phem_msf_host_chunks": (i32, [u32, u64, vp, vp, vp, i32, u64, u64, vp,
+                                       c.POINTER(c.c_uint64), vp]),
         "phem_cpu_cc": (i32, [u32, u64, vp, vp, vp, i32]),
         "phem_read_mtx": (i32, [c.c_char_p, i32, c.POINTER(GraphFile)]),
         "phem_read_edge_list": (i32, [c.c_char_p, c.POINTER(GraphFile)]),
@@ -827,6 +837,24 @@ def cpu_msf(n, u, v, w, wmode=W_F32, threads=0):
     _check(lib().phem_cpu_msf(n, len(u), u.ctypes.data, v.ctypes.data, w.ctypes.data, wmode,
                               ids.ctypes.data, ctypes.byref(cnt), threads), "phem_cpu_msf")
     return ids[:cnt.value].copy()
+
+
+def msf_host_chunks(n, u, v, w, wmode=W_F32, chunk_edges=1 << 24, cpu_edges=0):
+    """Minimum spanning forest of host edges streamed through the device in
+    chunks (phem_msf_host_chunks): (ascending edge ids, stats dict).
+    cpu_edges: edges of the last chunk the host reduces meanwhile
+    (CPU_EDGES_AUTO: the cpu_share knob's rule)."""
+    import numpy as np
+    u, v, w = (np.ascontiguousarray(x, dtype=np.uint32) for x in (u, v, w))
+    ids = np.zeros(max(n, 1), np.uint32)
+    cnt = ctypes.c_uint64(0)
+    st = ChunksStats()
+    _check(lib().phem_msf_host_chunks(n, len(u), u.ctypes.data, v.ctypes.data, w.ctypes.data,
+                                      wmode, chunk_edges, cpu_edges, ids.ctypes.data,
+                                      ctypes.byref(cnt), ctypes.byref(st)),
+           "phem_msf_host_chunks")
+    d = {f: getattr(st, f) for f, _ in ChunksStats._fields_ if f != "reserved"}
+    return ids[:cnt.value].copy(), d
 
 
 def get_tuning():

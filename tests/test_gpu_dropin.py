@@ -146,14 +146,15 @@ def _mtx(tmp_path, n, a, b, w=None):
 @pytest.mark.parametrize("mode,chunk,nchunks,debug", [
     (0, 0, 0, 0), (1, 7000, 0, 0), (1, 0, 5, 0), (2, 0, 3, 0), (2, 0, 1, 0),
     (2, 9000, 0, 0), (3, 0, 0, 0), (4, 0, 0, 0), (0, 0, 0, 1), (1, 0, 4, 1),
-    (2, 0, 3, 1)])
+    (2, 0, 3, 1), (3, 0, 0, 1)])
 @pytest.mark.parametrize("gen", [0, 1])
 def test_new_boruvka_matrix_market(H, tmp_path, mode, chunk, nchunks, debug, gen):
     """The reference's MST driver on a MatrixMarket file: the forest written
     to the result file (u+1 v+1 w, ascending input order) equals the oracle's
     Kruskal forest of the graph the reader loaded, with integer weights (many
-    ties) or the generated-weight rule, whole or in chunks, on the GPU, on the
-    CPU (mode 4) and in debug mode (the streamline paths repeated 10 times)."""
+    ties) or the generated-weight rule, whole or in chunks, on the GPU, from
+    host memory (mode 3, managed), on the CPU (mode 4) and in debug mode (the
+    streamline paths repeated 10 times)."""
     import phem_native as P
     L = _new_boruvka_lib(H)
     rng = np.random.default_rng(3 + gen)
@@ -179,6 +180,34 @@ def test_new_boruvka_matrix_market(H, tmp_path, mode, chunk, nchunks, debug, gen
     aos = np.zeros(3 * m, np.uint32)
     L.harness_mmap_populate(path.encode(), aos, m, n, gen)
     assert np.array_equal(aos.reshape(-1, 3), np.stack([g["u"], g["v"], g["w"]], axis=1))
+
+
+@pytest.mark.parametrize("share", [0.05, 2.0])
+def test_new_boruvka_heterogeneous_host_share(H, tmp_path, share):
+    """The heterogeneous mode with a host share set (the default gives the
+    host none): the host reduces part (0.05) or all (2.0) of the last chunk,
+    and the result file still equals the oracle's forest."""
+    import phem_native as P
+    L = _new_boruvka_lib(H)
+    rng = np.random.default_rng(8)
+    n, m = 2000, 60000
+    a = rng.integers(1, n + 1, m)
+    b = rng.integers(1, n + 1, m)
+    w = rng.integers(0, 20, m)
+    path = _mtx(tmp_path, n, a, b, w)
+    g = P.read_mtx(path)
+    ids, _ = O.kruskal(g["n"], g["u"], g["v"], g["w"], 0)
+    res = str(tmp_path / "forest.txt")
+    old = P.get_tuning()["cpu_share"]
+    try:
+        P.set_tuning(cpu_share=share)
+        print(f"new_boruvka heterogeneous cpu_share={share}", flush=True)
+        assert L.harness_new_boruvka(path.encode(), res.encode(), 0, 3, 0, 2, 0) == 0
+    finally:
+        P.set_tuning(cpu_share=old)
+    got = np.loadtxt(res, dtype=np.uint64, ndmin=2)
+    want = np.stack([g["u"][ids] + 1, g["v"][ids] + 1, g["w"][ids]], axis=1).astype(np.uint64)
+    assert np.array_equal(got, want)
 
 
 def test_device_boruvka_after_workspace_release(H):

@@ -20,15 +20,17 @@ def test_set_get_tuning_round_trip():
                                   "cc_ept", "cc_ept_filter")} == {
             "mst_block": 256, "mst_ept": 8, "mst_ept_compact": 16, "mst_compact_log2": 22,
             "mst_split_ratio": 1.5, "cc_block": 256, "cc_ept": 16, "cc_ept_filter": 8}
-        P.set_tuning(mst_light_factor=2.0, mst_block=512, cc_ept=32)  # the others unchanged
+        P.set_tuning(mst_light_factor=2.0, mst_block=512, cc_ept=32,
+                     cpu_share=0.25)  # the others unchanged
         t = P.get_tuning()
         assert t["blocks_per_cu"] == 8 and t["mst_light_factor"] == 2.0
         assert (t["mst_block"], t["mst_ept"], t["cc_ept"]) == (512, 8, 32)
+        assert t["cpu_share"] == 0.25
         for bad in ({"blocks_per_cu": 65}, {"mst_block": 384}, {"mst_ept": 32},
                     {"mst_ept_compact": 4}, {"mst_compact_log2": 41},
                     {"mst_ept": 16},  # 512 x 16 edges per tile: beyond 4096
                     {"mst_split_ratio": 1.0}, {"cc_block": 1024}, {"cc_ept": 4},
-                    {"cc_ept_filter": 32}, {"no_such_knob": 1}):
+                    {"cc_ept_filter": 32}, {"cpu_share": -0.5}, {"no_such_knob": 1}):
             with pytest.raises(P.PhemError):
                 P.set_tuning(**bad)
         assert P.get_tuning() == t  # refused settings change nothing
