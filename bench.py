@@ -9,6 +9,7 @@ weights k * 2^-24 in (0,1]).
 Prints one JSON line.  `value` = m / (time per step) in edges/s.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -410,13 +411,22 @@ def stream_bench(P, torch, scale, m0_log2, batches, batch_log2, seed):
     t0 = time.perf_counter()
     st.apply(*base)
     t_base = time.perf_counter() - t0
-    lat = []
-    t0 = time.perf_counter()
-    for b in range(batches):
-        a = time.perf_counter()
-        st.apply(*bat[b])
-        lat.append(time.perf_counter() - a)
-    t_all = time.perf_counter() - t0
+    lat, rounds = [], []
+    # the garbage of earlier legs is collected now, not inside a batch
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        for b in range(batches):
+            a = time.perf_counter()
+            st.apply(*bat[b])
+            lat.append(time.perf_counter() - a)
+            rounds.append(st.stats()["rounds"])
+        t_all = time.perf_counter() - t0
+    finally:
+        gc.enable()
+    slow = [{"batch": b, "ms": round(lat[b] * 1e3, 3), "rounds": rounds[b]}
+            for b in sorted(range(batches), key=lambda b: -lat[b])[:3]]
     s = st.stats()
     fu, fv, fw, fg = st.forest()
     gid = fg.cpu().numpy()
@@ -435,6 +445,7 @@ def stream_bench(P, torch, scale, m0_log2, batches, batch_log2, seed):
             "batch_edges_per_s": batches * mb / t_all,
             "batch_lat_p50_ms": round(pct(.5) * 1e3, 3),
             "batch_lat_p99_ms": round(pct(.99) * 1e3, 3),
+            "slowest_batches": slow,
             "forest_edges": len(gid), "edges_inserted": s["edges_inserted"],
             "forest_equals_recompute": same}
 
