@@ -622,7 +622,29 @@ def main():
         "roofline": roof,
     }
 
+    # the CPU baseline runs after every GPU leg: the host thread pools it
+    # starts (OpenMP, ParlayLib) would otherwise share the host cores with
+    # the serving and streaming legs' host threads
+    cpu_in = None
     if not args.no_cpu:
+        k = min(m, 1 << args.cpu_sample_log2)
+        cpu_in = [x[:k].cpu().numpy().view(np.uint32) for x in (du, dv, dw)]
+    if not args.no_serve:
+        del du, dv, dw, de, solver, cc_par, cc_lab, cc_forest
+        torch.cuda.empty_cache()
+        out["serving"] = serve_bench(P, torch, args.serve_graphs, 5, args.serve_slots,
+                                     1 << args.serve_batch_log2, args.serve_load)
+        torch.cuda.empty_cache()
+    if not args.no_stream:
+        if args.no_serve:
+            del du, dv, dw, de, solver, cc_par, cc_lab, cc_forest
+            torch.cuda.empty_cache()
+        out["streaming"] = stream_bench(P, torch, 22, 24, 256, 20, 4)
+    if not args.no_replay:
+        out["recorded_solve"] = replay_bench(P, torch, 16, 20, 1)
+    if not args.no_configs:
+        out["configs"] = configs_bench(P, torch)
+    if cpu_in is not None:
         host = host_cpus()
         # every CPU the process may use, within its cgroup quota; 16 at most,
         # the host share of one GPU on the measurement boxes
@@ -630,10 +652,8 @@ def main():
         if host["cgroup_cpus"]:
             threads = min(threads, max(1, int(host["cgroup_cpus"])))
         threads = min(threads, 16)
-        k = min(m, 1 << args.cpu_sample_log2)
-        u_np = du[:k].cpu().numpy().view(np.uint32)
-        v_np = dv[:k].cpu().numpy().view(np.uint32)
-        w_np = dw[:k].cpu().numpy().view(np.uint32)
+        u_np, v_np, w_np = cpu_in
+        k = len(u_np)
         print(f"# cpu baseline on {k} edges, {threads} threads, host {host}",
               file=sys.stderr, flush=True)
         eps, t_mst, t_cc, cc_kind = cpu_baseline(u_np, v_np, w_np, n, threads, args.cpu_runs)
@@ -652,21 +672,6 @@ def main():
                       "cc": {"kind": cc_kind, "s": round(t_cc, 4),
                              "edges_per_s": k / t_cc}},
             "host": host}
-    if not args.no_serve:
-        del du, dv, dw, de, solver, cc_par, cc_lab, cc_forest
-        torch.cuda.empty_cache()
-        out["serving"] = serve_bench(P, torch, args.serve_graphs, 5, args.serve_slots,
-                                     1 << args.serve_batch_log2, args.serve_load)
-        torch.cuda.empty_cache()
-    if not args.no_stream:
-        if args.no_serve:
-            del du, dv, dw, de, solver, cc_par, cc_lab, cc_forest
-            torch.cuda.empty_cache()
-        out["streaming"] = stream_bench(P, torch, 22, 24, 256, 20, 4)
-    if not args.no_replay:
-        out["recorded_solve"] = replay_bench(P, torch, 16, 20, 1)
-    if not args.no_configs:
-        out["configs"] = configs_bench(P, torch)
 
     print(json.dumps(out), flush=True)
 
