@@ -113,12 +113,15 @@ bool mst_uses_split(uint32_t n, uint64_t m, const phem_mst_opts *opts);
 // device word that holds the forest size (the ids are in d_mst_ids).
 using MstTail = std::function<int(const unsigned long long *d_count)>;
 
-// phem_mst_solve with a tail (structure-of-arrays input).
+// phem_mst_solve with a tail (structure-of-arrays input).  first_rounds:
+// Boruvka rounds to launch before the first counter read (the rounds the
+// caller expects; the results do not depend on it; 0: the default batch).
 int mst_solve_tail(uint32_t n, uint64_t m, const uint32_t *d_u,
                    const uint32_t *d_v, const uint32_t *d_w, int wmode,
                    void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
                    uint32_t *d_labels, const phem_mst_opts *opts,
-                   phem_mst_stats *stats, hipStream_t s, const MstTail &tail);
+                   phem_mst_stats *stats, hipStream_t s, const MstTail &tail,
+                   int first_rounds = 0);
 
 // Pinned host buffers shared by every thread of the process: a free list
 // that short-lived threads (server workers) draw from and return to, so no

@@ -1299,6 +1299,9 @@ static void launch_input_pass(uint64_t m, hipStream_t s, A... args) {
 // grp: the graphs of a batch (giant component per graph), or NULL.
 // tail: work enqueued before the final counter read, or NULL.
 // fr: fixed-round schedule with no counter read at all (stats unwritten).
+// first_rounds: rounds launched before the solve's first counter read (a
+// caller that expects that many rounds, as a stream does from its previous
+// batch; at most what one read can report, and 0 for the default).
 template <class Src>
 static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                           void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
@@ -1307,7 +1310,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                           const GraphGroups *grp = nullptr,
                           const MstTail *tail = nullptr,
                           const FixedRounds *fr = nullptr,
-                          ull *host_sums = nullptr) {
+                          ull *host_sums = nullptr, int first_rounds = 0) {
   const bool groups = grp && grp->k > 1;
   if (m > PHEM_MAX_EDGES || n > PHEM_MAX_VERTICES)
     return set_err(PHEM_EINVAL, "n=%u m=%llu beyond limits", n,
@@ -1425,7 +1428,14 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       // come from upper bounds (alive edges and component lists only shrink
       // within a phase), and the kernels loop over the tiles the device
       // counters hold.
-      const int want = fixed ? fixed : SPEC;
+      // one read reports 2 batch + 2 values and, on a phase's first read,
+      // the extra ones: at most MAX_SUMS
+      const int read_cap = (MAX_SUMS - 2 - (first ? (int)extra.size() : 0)) / 2;
+      int want = fixed ? fixed : SPEC;
+      if (!fixed && first_rounds > 0) {
+        want = first_rounds < read_cap ? first_rounds : read_cap;
+        first_rounds = 0;  // the hint covers the solve's first read only
+      }
       const int batch = PHEM_MAX_ROUNDS - r < want ? PHEM_MAX_ROUNDS - r : want;
       if (batch <= 0 || (fixed && batch < fixed))
         return set_err(PHEM_ENOCONV, "no convergence after %d rounds", r);
@@ -1866,12 +1876,14 @@ int mst_solve_tail(uint32_t n, uint64_t m, const uint32_t *d_u,
                    const uint32_t *d_v, const uint32_t *d_w, int wmode,
                    void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
                    uint32_t *d_labels, const phem_mst_opts *opts,
-                   phem_mst_stats *stats, hipStream_t s, const MstTail &tail) {
+                   phem_mst_stats *stats, hipStream_t s, const MstTail &tail,
+                   int first_rounds) {
   if (m && (!d_u || !d_v || !d_w))
     return set_err(PHEM_EINVAL, "edge arrays are NULL");
   const bool vec = aligned16(d_u) && aligned16(d_v) && aligned16(d_w);
   return mst_solve_impl(n, m, SrcSoA{d_u, d_v, d_w, vec}, wmode, d_ws, ws_bytes,
-                        d_mst_ids, d_labels, opts, stats, s, nullptr, &tail);
+                        d_mst_ids, d_labels, opts, stats, s, nullptr, &tail,
+                        nullptr, nullptr, first_rounds);
 }
 
 }  // namespace phem

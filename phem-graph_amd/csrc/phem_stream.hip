@@ -210,10 +210,15 @@ extern "C" int phem_stream_apply(phem_stream_t *s, const uint32_t *d_u,
     return phem_cc_union(s->n, (const uint64_t *)s->packed, mb, s->parent,
                          (uint64_t *)s->cc_forest, stream);
   };
+  // consecutive batches of a stream take about as many rounds: the previous
+  // canonical solve's rounds after round 0 are launched before the first
+  // counter read, which then usually finds the solve converged
+  const int hint = s->last.rounds > 1 && s->last.tau == PHEM_NO_EDGE
+                       ? (int)s->last.rounds - 1 : 0;
   phem_mst_stats last{};
   rc = mst_solve_tail(s->n, total, s->u[c], s->v[c], s->w[c], s->wmode, s->ws,
                       s->ws_bytes, s->ids, s->labels, nullptr, &last, st,
-                      tail);
+                      tail, hint);
   if (rc) {
     // once the union is enqueued the two states disagree: refuse later use
     if (unioned) s->broken = true;
