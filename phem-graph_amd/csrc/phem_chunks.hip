@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "phem.h"
+#include "phem_copypool.h"
 #include "phem_internal.h"
 
 namespace phem {
@@ -47,9 +48,11 @@ struct Stager {
   bool used[RING] = {};
   int next = 0;
   unsigned threads = 1;
+  std::unique_ptr<CopyPool> pool;  // threads - 1 workers besides the caller
 
   int init(unsigned copy_threads) {
     threads = copy_threads ? copy_threads : 1;
+    pool.reset(new CopyPool(threads - 1));
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
       return fail("copy stream");
     for (int i = 0; i < RING; i++) {
@@ -87,15 +90,11 @@ struct Stager {
         return set_err(PHEM_EHIP, "staging copy failed");
       uint32_t *p = pin[r];
       const uint32_t *s = src + o;
-      const unsigned T = len >= (1u << 18) ? threads : 1;
-      std::vector<std::thread> ts;
-      auto part = [&](unsigned t) {
+      const uint64_t T = len >= (1u << 18) ? threads : 1;
+      pool->run(T, [&](uint64_t t) {
         const uint64_t b = len * t / T, e = len * (t + 1) / T;
         std::memcpy(p + b, s + b, 4 * (e - b));
-      };
-      for (unsigned t = 1; t < T; t++) ts.emplace_back(part, t);
-      part(0);
-      for (auto &t : ts) t.join();
+      });
       if (hipMemcpyAsync(dst + o, p, 4 * len, hipMemcpyHostToDevice, cs) != hipSuccess ||
           hipEventRecord(ev[r], cs) != hipSuccess)
         return fail("staging copy");
