@@ -21,6 +21,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -189,6 +190,8 @@ extern "C" int phem_msf_host_chunks(uint32_t n, uint64_t m, const uint32_t *u,
   uint64_t cpu_cnt = 0;
   int cpu_rc = PHEM_OK;
   double cpu_s = 0;
+  // error messages are per thread: a worker's is carried back to the caller
+  std::string cpu_err, next_err;
   std::thread cpu;
   if (cpu_span) {
     cpu_ids.resize(n);
@@ -196,6 +199,7 @@ extern "C" int phem_msf_host_chunks(uint32_t n, uint64_t m, const uint32_t *u,
       const double c0 = now_s();
       cpu_rc = phem_cpu_msf(n, cpu_span, u + last, v + last, w + last, wmode,
                             cpu_ids.data(), &cpu_cnt, (int)std::max(1u, hw - stg.threads));
+      if (cpu_rc) cpu_err = phem_last_error();
       cpu_s = now_s() - c0;
     });
   }
@@ -212,7 +216,7 @@ extern "C" int phem_msf_host_chunks(uint32_t n, uint64_t m, const uint32_t *u,
     const uint64_t k = std::min(chunk_edges, m - off);
     if (c + 1 == nchunks && cpu.joinable()) {
       cpu.join();
-      if (cpu_rc) return cpu_rc;
+      if (cpu_rc) return set_err(cpu_rc, "host share: %s", cpu_err.c_str());
       const uint64_t rest = m - last - cpu_span;
       last_len = cpu_cnt + rest;
       fl.resize(cpu_cnt ? cpu_cnt : 1);
@@ -241,7 +245,11 @@ extern "C" int phem_msf_host_chunks(uint32_t n, uint64_t m, const uint32_t *u,
   for (uint64_t c = 0; c < nchunks && rc == PHEM_OK; c++) {
     int next_rc = PHEM_OK;
     std::thread next;
-    if (c + 1 < nchunks) next = std::thread([&, c] { next_rc = stage(c + 1); });
+    if (c + 1 < nchunks)
+      next = std::thread([&, c] {
+        next_rc = stage(c + 1);
+        if (next_rc) next_err = phem_last_error();
+      });
     if (hipStreamWaitEvent(ks, ready[c & 1], 0) != hipSuccess) {
       rc = Stager::fail("chunk wait");
     } else {
@@ -250,7 +258,9 @@ extern "C" int phem_msf_host_chunks(uint32_t n, uint64_t m, const uint32_t *u,
                              chunk_len(c), ks);
     }
     if (next.joinable()) next.join();
-    if (rc == PHEM_OK) rc = next_rc;
+    if (rc == PHEM_OK && next_rc)
+      rc = set_err(next_rc, "staging chunk %llu: %s", (unsigned long long)(c + 1),
+                   next_err.c_str());
   }
   if (cpu.joinable()) cpu.join();
   if (rc) return rc;
