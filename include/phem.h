@@ -97,6 +97,12 @@ typedef struct {
                                the last chunk) edges of the last chunk
                                (>= 0; default 0: on MI355X every host share
                                measured slowed the solve; NaN: unchanged) */
+  double mst_light_factor_skewed; /* light factor of solves that leave it to
+                               the library when their threshold sample
+                               finds degree skew (hubs repeat as sampled
+                               endpoints; default 0.625; 0: the one
+                               default factor for every graph; NaN:
+                               unchanged) */
 } phem_tuning;
 int phem_get_tuning(phem_tuning *t);
 int phem_set_tuning(const phem_tuning *t);
@@ -159,7 +165,9 @@ typedef struct phem_mst_stats {
  *   (MST/src/kernels.cu:507-689), whose per-round log is the reference's.
  *   NaN, like NULL options: the mst_light_factor knob (phem_set_tuning;
  *   1.0 unless set or given by the environment variable
- *   PHEM_MST_LIGHT_FACTOR).
+ *   PHEM_MST_LIGHT_FACTOR), or the mst_light_factor_skewed knob (0.625)
+ *   when the key sample's endpoints repeat far more often than in a
+ *   uniform graph (hubs).  Whether the split runs follows from the first.
  * flags: PHEM_MST_MAXIMUM or 0. */
 typedef struct phem_mst_opts {
   double light_factor;

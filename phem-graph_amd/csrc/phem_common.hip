@@ -35,6 +35,12 @@ std::atomic<double> g_mst_split_ratio{1.5};
 // 0.02-0.1 G edges/s, and every host share tried (2^20 edges up to the
 // whole last chunk) made the solve slower.
 std::atomic<double> g_cpu_share{0.0};
+// light factor of degree-skewed graphs (measured on MI355X, MST ms at light
+// factor 0.5 / 0.625 / 1.0: RMAT 2^24 / 2^28 6.17 / 6.15 / 6.44, the same
+// with permuted ids 6.59 / 6.64 / 7.11, RMAT 2^20 with 2^22 edges 0.84 /
+// 0.87 / 0.91; uniform graphs need 1.0: 2^24 / 2^28 takes 77.8 / 47.5 /
+// 12.2 ms)
+std::atomic<double> g_lf_skewed{0.625};
 
 bool one_of(uint32_t x, std::initializer_list<uint32_t> allowed) {
   for (uint32_t a : allowed)
@@ -99,6 +105,7 @@ KernelShape kernel_shape() {
   k.cc_ept = g_cc_ept.load(std::memory_order_relaxed);
   k.cc_ept_filter = g_cc_ept_filter.load(std::memory_order_relaxed);
   k.cpu_share = g_cpu_share.load(std::memory_order_relaxed);
+  k.mst_light_factor_skewed = g_lf_skewed.load(std::memory_order_relaxed);
   return k;
 }
 
@@ -297,6 +304,7 @@ extern "C" int phem_get_tuning(phem_tuning *t) {
   t->cc_ept_filter = k.cc_ept_filter;
   t->reserved2 = 0;
   t->cpu_share = k.cpu_share;
+  t->mst_light_factor_skewed = k.mst_light_factor_skewed;
   return PHEM_OK;
 }
 
@@ -334,8 +342,13 @@ extern "C" int phem_set_tuning(const phem_tuning *t) {
                          t->cc_ept_filter);
   if (!std::isnan(t->cpu_share) && !(t->cpu_share >= 0.0 && t->cpu_share <= 1e6))
     return phem::set_err(PHEM_EINVAL, "cpu_share must be in [0, 1e6]");
+  if (!std::isnan(t->mst_light_factor_skewed) &&
+      !(t->mst_light_factor_skewed >= 0.0 && t->mst_light_factor_skewed <= 1e6))
+    return phem::set_err(PHEM_EINVAL, "mst_light_factor_skewed must be in [0, 1e6]");
   phem::tuning_init();
   if (!std::isnan(t->cpu_share)) phem::g_cpu_share.store(t->cpu_share);
+  if (!std::isnan(t->mst_light_factor_skewed))
+    phem::g_lf_skewed.store(t->mst_light_factor_skewed);
   if (t->mst_block) phem::g_mst_block.store(t->mst_block);
   if (t->mst_ept) phem::g_mst_ept.store(t->mst_ept);
   if (t->mst_ept_compact) phem::g_mst_ept_compact.store(t->mst_ept_compact);

@@ -31,6 +31,7 @@ struct KernelShape {
   double mst_split_ratio;
   uint32_t cc_block, cc_ept, cc_ept_filter;
   double cpu_share;
+  double mst_light_factor_skewed;
 };
 KernelShape kernel_shape();
 

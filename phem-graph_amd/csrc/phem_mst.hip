@@ -943,53 +943,129 @@ __global__ __launch_bounds__(BLK) void k_giant_emit_groups(
 // tau = key of rank ~light_edges in a strided sample of TSAMP input keys
 // (KEY_NONE: every edge is light).  The light phase only needs about
 // light_edges edges; the results do not depend on tau.
+//
+// Degree skew picks light_edges (when light_skewed differs from it): the
+// sampled edges' endpoints go into an LDS hash set, and an endpoint already
+// there counts as a repeat.  Among k endpoints drawn uniformly from n
+// vertices about k^2 / 2n repeat; a graph with hubs (RMAT, power laws)
+// repeats many times more, and its light subgraph still has a giant
+// component with fewer light edges (light_skewed), while a uniform graph's
+// light subgraph needs an average degree above 1 (light_edges > n / 2) for
+// one.  Skewed: at least 16 repeats and 6 times the uniform expectation.
 constexpr int TSAMP = 1024;
+constexpr int HSLOTS = 4096;  // hash slots for the 2 * TSAMP sampled endpoints
+
+// Inserts vertex x into the LDS set hs of SLOTS (a power of two) slots;
+// counts a repeat in *rep.
+template <int SLOTS>
+__device__ __forceinline__ void sample_set_insert(uint32_t *hs, uint32_t x,
+                                                  uint32_t *rep) {
+  static_assert((SLOTS & (SLOTS - 1)) == 0 && SLOTS > 1, "SLOTS: a power of two");
+  // multiplicative hash: the product's top bits
+  const uint32_t h = (x * 2654435761u) >> (32 - __builtin_ctz((unsigned)SLOTS));
+  for (uint32_t p = 0; p < (uint32_t)SLOTS; p++) {
+    const uint32_t slot = (h + p) & (SLOTS - 1);
+    const uint32_t old = atomicCAS(&hs[slot], U32_NONE, x);
+    if (old == U32_NONE) return;
+    if (old == x) {
+      atomicAdd(rep, 1u);
+      return;
+    }
+  }
+}
+
 template <class Src>
 __global__ __launch_bounds__(SBLK) void k_threshold(Src src, uint64_t m,
                                                     uint32_t n, int ord,
                                                     uint64_t light_edges,
+                                                    uint64_t light_skewed,
                                                     ull *tau_out) {
+  static_assert(HSLOTS >= 2 * TSAMP, "the endpoint set needs free slots");
   __shared__ ull s[TSAMP];
+  __shared__ uint32_t hs[HSLOTS];
+  __shared__ uint32_t s_rep, s_k;
+  const bool detect = light_skewed != light_edges;
+  if (detect) {
+    for (int j = threadIdx.x; j < HSLOTS; j += SBLK) hs[j] = U32_NONE;
+    if (threadIdx.x == 0) s_rep = s_k = 0;
+    __syncthreads();
+  }
   for (int j = threadIdx.x; j < TSAMP; j += SBLK) {
     const uint64_t i = (uint64_t)j * m / TSAMP;
     const uint32_t a = src.U(i), b = src.V(i);
-    s[j] = (a < n && b < n && a != b)
-               ? make_key(src.W(i), i, ord)
-               : KEY_NONE;
+    const bool valid = a < n && b < n && a != b;
+    s[j] = valid ? make_key(src.W(i), i, ord) : KEY_NONE;
+    if (detect && valid) {
+      sample_set_insert<HSLOTS>(hs, a, &s_rep);
+      sample_set_insert<HSLOTS>(hs, b, &s_rep);
+      atomicAdd(&s_k, 2u);
+    }
   }
   __syncthreads();
   bitonic_sort_lds<TSAMP>(s);
   if (threadIdx.x == 0) {
-    ull r = (ull)((double)light_edges * TSAMP / (double)m);
-    *tau_out = (light_edges >= m || r >= TSAMP) ? KEY_NONE : s[r];
+    uint64_t le = light_edges;
+    if (detect) {
+      const double k = (double)s_k;
+      const double uniform = k * k / (2.0 * (double)n);
+      if (s_rep >= 16 && (double)s_rep >= 6.0 * uniform) le = light_skewed;
+    }
+    ull r = (ull)((double)le * TSAMP / (double)m);
+    *tau_out = (le >= m || r >= TSAMP) ? KEY_NONE : s[r];
   }
 }
 
 // Per graph g of a batch: taus[g] = the key of rank ~light_factor * n_g
 // among up to TSAMP_G evenly spaced keys of the graph's edges (KEY_NONE:
 // every edge of the graph is light).  One block per graph.
+// Each graph's factor follows its own degree skew as in k_threshold (the
+// sampled endpoints' repeats against k^2 / 2 n_g; skewed: at least 4
+// repeats and 6 times that), when lf_skewed differs from light_factor.
 constexpr int TSAMP_G = 256;
+constexpr int HSLOTS_G = 1024;  // hash slots for the 2 * TSAMP_G endpoints
 template <class Src>
 __global__ __launch_bounds__(TSAMP_G) void k_threshold_groups(
     Src src, uint32_t k, const ull *__restrict__ voff,
     const ull *__restrict__ eoff, uint32_t n, int ord, double light_factor,
-    ull *__restrict__ taus) {
+    double lf_skewed, ull *__restrict__ taus) {
+  static_assert(HSLOTS_G >= 2 * TSAMP_G, "the endpoint set needs free slots");
   __shared__ ull s[TSAMP_G];
+  __shared__ uint32_t hs[HSLOTS_G];
+  __shared__ uint32_t s_rep, s_k;
+  const bool detect = lf_skewed != light_factor;
   for (uint32_t g = blockIdx.x; g < k; g += gridDim.x) {
     const ull e0 = eoff[g], mg = eoff[g + 1] - e0;
     const ull cnt = mg < (ull)TSAMP_G ? mg : (ull)TSAMP_G;
     const uint32_t j = threadIdx.x;  // one sample per thread
+    if (detect) {
+      for (int x = j; x < HSLOTS_G; x += TSAMP_G) hs[x] = U32_NONE;
+      if (j == 0) s_rep = s_k = 0;
+      __syncthreads();
+    }
     ull key = KEY_NONE;
     if (j < cnt) {
       const uint64_t i = e0 + (ull)j * mg / cnt;
       const uint32_t a = src.U(i), b = src.V(i);
-      if (a < n && b < n && a != b) key = make_key(src.W(i), i, ord);
+      if (a < n && b < n && a != b) {
+        key = make_key(src.W(i), i, ord);
+        if (detect) {
+          sample_set_insert<HSLOTS_G>(hs, a, &s_rep);
+          sample_set_insert<HSLOTS_G>(hs, b, &s_rep);
+          atomicAdd(&s_k, 2u);
+        }
+      }
     }
     s[j] = key;
     __syncthreads();
     bitonic_sort_lds<TSAMP_G, TSAMP_G>(s);
     if (j == 0) {
-      const double want = light_factor * (double)(voff[g + 1] - voff[g]);
+      const double ng = (double)(voff[g + 1] - voff[g]);
+      double lf = light_factor;
+      if (detect && ng > 0) {
+        const double kk = (double)s_k;
+        if (s_rep >= 4 && (double)s_rep >= 6.0 * kk * kk / (2.0 * ng)) lf = lf_skewed;
+      }
+      const double want = lf * ng;
       ull t = KEY_NONE;
       if (want < (double)mg) {
         const ull r = (ull)(want * (double)cnt / (double)mg);
@@ -1335,6 +1411,13 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   const double light_factor = mst_light_factor(opts);
   const uint64_t light_edges = mst_light_edges(n, m, light_factor);
   const bool split = mst_split(n, m, light_factor);
+  // a solve that leaves light_factor to the library takes the skewed-graph
+  // factor when the threshold sample finds hubs (k_threshold)
+  const double lf_skew = kernel_shape().mst_light_factor_skewed;
+  const uint64_t light_skewed =
+      (!opts || std::isnan(opts->light_factor)) && lf_skew > 0 && light_factor > 0
+          ? mst_light_edges(n, m, lf_skew)
+          : light_edges;
 
   char *ws = (char *)d_ws;
   uint32_t *par = (uint32_t *)(ws + L.par);
@@ -1604,10 +1687,12 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       if (groups)
         hipLaunchKernelGGL(k_threshold_groups<Src>, dim3(grid_for(grp->k, 1)),
                            dim3(TSAMP_G), 0, s, src, grp->k, grp->voff,
-                           grp->eoff, n, ord, light_factor, grp->taus);
+                           grp->eoff, n, ord, light_factor,
+                           light_skewed != light_edges ? lf_skew : light_factor,
+                           grp->taus);
       else
         hipLaunchKernelGGL(k_threshold<Src>, dim3(1), dim3(SBLK), 0, s, src, m,
-                           n, ord, light_edges, ctr + C_TAU);
+                           n, ord, light_edges, light_skewed, ctr + C_TAU);
     }
     PHEM_LAUNCHED("k_threshold");
     {

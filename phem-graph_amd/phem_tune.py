@@ -4,7 +4,9 @@
   kernels (grid size of every edge and vertex pass);
 - mst_light_factor: size of the light phase of the minimum spanning forest
   solve (edges below the filter threshold ~ factor * n), i.e. where the
-  filter-Kruskal split falls;
+  filter-Kruskal split falls; a tuned factor applies to every graph of its
+  size class, so the tuned configuration turns the library's choice of a
+  separate factor for degree-skewed graphs off (mst_light_factor_skewed 0);
 - cc_sample_factor: size of the sampling phase of the connected-components
   solve (first factor * n records);
 - kernel shapes, each a choice among kernels compiled for it: mst_block and
@@ -150,7 +152,9 @@ def tune(n, m, graph="rmat", seed=1, reps=3, grid=None, device="cuda", verbose=F
         return sorted(ts)[len(ts) // 2]
 
     start = P.get_tuning()
-    best = dict(start)
+    # the tuned light factor applies to every graph (no separate factor for
+    # degree-skewed ones, which would hide the light factor's effect here)
+    best = {**start, "mst_light_factor_skewed": 0.0}
     log = []
 
     def cost(knob):

@@ -72,6 +72,29 @@ def test_mst_split_matches_oracle(P, name, lf):
         assert st["weight_fx"] == ws
 
 
+def test_light_factor_follows_degree_skew(P):
+    """A solve that leaves the light factor to the library takes the skewed
+    factor (0.625) for an RMAT graph, whose sampled endpoints repeat (hubs),
+    and the default (1.0) for a uniform graph; mst_light_factor_skewed = 0
+    turns the choice off.  The light edges the solve reports follow the
+    factor (the threshold comes from a 1024-key sample, hence the margins);
+    the forest is the oracle's in every case."""
+    rm = cases.make("rmat_small")
+    un = cases.make("uniform_c1")
+    old = P.get_tuning()["mst_light_factor_skewed"]
+    try:
+        for skew, (n, u, v, w, wmode), lo, hi in (
+                (0.625, rm, 0.3, 0.8), (0.625, un, 0.85, 1.2), (0.0, rm, 0.85, 1.2)):
+            P.set_tuning(mst_light_factor_skewed=skew)
+            ids, lab, st = run_mst(P, n, u, v, w, wmode)
+            print(f"skew knob {skew}: n={n} light edges {st['light_edges']}", flush=True)
+            assert lo * n <= st["light_edges"] <= hi * n
+            k_ids, k_lab = O.kruskal(n, u, v, w, wmode)
+            assert np.array_equal(ids, k_ids) and np.array_equal(lab, k_lab)
+    finally:
+        P.set_tuning(mst_light_factor_skewed=old)
+
+
 def test_mst_repeat_is_deterministic(P):
     n, u, v, w, wmode = cases.make("rmat_small")
     solver = P.MstSolver(n, len(u))

@@ -26,11 +26,16 @@ def test_set_get_tuning_round_trip():
         assert t["blocks_per_cu"] == 8 and t["mst_light_factor"] == 2.0
         assert (t["mst_block"], t["mst_ept"], t["cc_ept"]) == (512, 8, 32)
         assert t["cpu_share"] == 0.25
+        assert old["mst_light_factor_skewed"] == 0.625  # built-in
+        P.set_tuning(mst_light_factor_skewed=0.0)
+        assert P.get_tuning()["mst_light_factor_skewed"] == 0.0
+        t = P.get_tuning()
         for bad in ({"blocks_per_cu": 65}, {"mst_block": 384}, {"mst_ept": 32},
                     {"mst_ept_compact": 4}, {"mst_compact_log2": 41},
                     {"mst_ept": 16},  # 512 x 16 edges per tile: beyond 4096
                     {"mst_split_ratio": 1.0}, {"cc_block": 1024}, {"cc_ept": 4},
-                    {"cc_ept_filter": 32}, {"cpu_share": -0.5}, {"no_such_knob": 1}):
+                    {"cc_ept_filter": 32}, {"cpu_share": -0.5},
+                    {"mst_light_factor_skewed": -1.0}, {"no_such_knob": 1}):
             with pytest.raises(P.PhemError):
                 P.set_tuning(**bad)
         assert P.get_tuning() == t  # refused settings change nothing
