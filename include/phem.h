@@ -457,9 +457,13 @@ typedef struct {
   int32_t wmode;            /* PHEM_W_U32 or PHEM_W_F32 for every request */
   uint32_t record_rounds;   /* > 0: batches replay a batched solve recorded
                                (phem_batch_graph_*) for the batch's capacity
-                               bucket with this many rounds per phase; a
-                               batch that needs more falls back to the eager
-                               solve.  0: every batch is solved eagerly */
+                               bucket: a bucket's first batch runs eagerly,
+                               later ones replay a recording with the rounds
+                               per phase it needed plus two, at most this
+                               many; a batch that needs more falls back to
+                               the eager solve (and the bucket is recorded
+                               again with more rounds).  0: every batch is
+                               solved eagerly */
   uint32_t reserved;
 } phem_server_opts;
 

@@ -15,7 +15,11 @@
 // With record_rounds set, a batch replays a batched solve recorded for its
 // capacity bucket (graph count rounded up to a power of two, vertex and edge
 // counts to a quarter octave) instead of launching the solve eagerly; each
-// slot keeps its most recently used recordings.  Host inputs are copied into
+// slot keeps its most recently used recordings.  A bucket's first batch runs
+// eagerly and notes the rounds it needed; the recording is made when the
+// bucket comes again, with those rounds plus two (at most record_rounds), so
+// shapes seen once cost no recording and replays run no idle rounds beyond
+// the margin.  Host inputs are copied into
 // pinned staging by the slot's copy threads in chunks, each chunk's
 // host-to-device copy running while the next chunk is filled.
 #include <algorithm>
@@ -84,6 +88,15 @@ struct Recorded {
 
 constexpr size_t MAX_RECORDINGS = 16;  // per slot
 
+// Rounds per phase a bucket's eager batch needed (the larger phase).
+struct Seen {
+  Bucket b;
+  int rounds = 0;
+  uint64_t used = 0;
+};
+
+constexpr size_t MAX_SEEN = 256;  // buckets remembered per slot
+
 struct Slot {
   hipStream_t stream = nullptr;
   void *dev = nullptr;
@@ -115,6 +128,10 @@ struct phem_server {
   uint64_t slot_bytes = 0;
   std::vector<Slot> slots;
   std::mutex mu;
+  // buckets seen by an eager batch, with the rounds per phase it needed
+  // (any slot; under mu)
+  std::vector<Seen> seen;
+  uint64_t seen_tick = 0;
   std::condition_variable cv_work, cv_done;
   std::deque<Req *> queue;
   std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
@@ -160,10 +177,36 @@ struct phem_server {
   }
   void worker(int si);
   void run_batch(Slot &S, std::vector<Req *> &batch);
+  // Rounds per phase bucket b's eager batches needed (0: none seen yet), and
+  // noting them (the least recently used bucket is forgotten beyond
+  // MAX_SEEN).
+  int seen_rounds(const Bucket &b) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &e : seen)
+      if (e.b == b) {
+        e.used = ++seen_tick;
+        return e.rounds;
+      }
+    return 0;
+  }
+  void note_rounds(const Bucket &b, int rounds) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &e : seen)
+      if (e.b == b) {
+        e.rounds = rounds > e.rounds ? rounds : e.rounds;
+        e.used = ++seen_tick;
+        return;
+      }
+    if (seen.size() >= MAX_SEEN)
+      seen.erase(std::min_element(seen.begin(), seen.end(), [](const Seen &x, const Seen &y) {
+        return x.used < y.used;
+      }));
+    seen.push_back(Seen{b, rounds, ++seen_tick});
+  }
   int stage_inputs(Slot &S, const std::vector<Req *> &batch,
                    const std::vector<uint64_t> &hoff, uint64_t mh,
                    uint32_t *d_su, uint32_t *d_sv, uint32_t *d_sw);
-  phem_batch_graph_t *recording(Slot &S, const Bucket &b, char *ws,
+  phem_batch_graph_t *recording(Slot &S, const Bucket &b, int rounds, char *ws,
                                 uint64_t ws_bytes, uint32_t *ids, uint32_t *cnt,
                                 uint32_t *lab, uint64_t *wfx, uint64_t *bad);
 };
@@ -255,9 +298,17 @@ int phem_server::stage_inputs(Slot &S, const std::vector<Req *> &batch,
   return PHEM_OK;
 }
 
-// The slot's recording for bucket b (made on first use; the least recently
-// used one is dropped beyond MAX_RECORDINGS), or NULL if recording failed.
-phem_batch_graph_t *phem_server::recording(Slot &S, const Bucket &b, char *ws,
+// Rounds per phase of a solve's larger phase, from its statistics.
+int phase_rounds(const phem_mst_stats &st) {
+  const int light = (int)st.light_rounds, heavy = (int)st.rounds - light;
+  return light > heavy ? light : heavy;
+}
+
+// The slot's recording for bucket b (made on first use with `rounds` per
+// phase; the least recently used one is dropped beyond MAX_RECORDINGS), or
+// NULL if recording failed.
+phem_batch_graph_t *phem_server::recording(Slot &S, const Bucket &b, int rounds,
+                                           char *ws,
                                            uint64_t ws_bytes, uint32_t *ids,
                                            uint32_t *cnt, uint32_t *lab,
                                            uint64_t *wfx, uint64_t *bad) {
@@ -278,8 +329,7 @@ phem_batch_graph_t *phem_server::recording(Slot &S, const Bucket &b, char *ws,
   }
   phem_batch_graph_t *g = nullptr;
   if (phem_batch_graph_create(b.k_cap, b.n_cap, b.m_cap, o.wmode, ws, ws_bytes,
-                              ids, cnt, lab, wfx, bad, nullptr,
-                              (int)o.record_rounds, &g) != PHEM_OK)
+                              ids, cnt, lab, wfx, bad, nullptr, rounds, &g) != PHEM_OK)
     return nullptr;
   S.rec.push_back(Recorded{b, g, S.tick});
   {
@@ -334,17 +384,30 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
     }
   }
   bool replayed = false, fell_back = false;
-  if (rc == PHEM_OK && rec) {
+  // a bucket seen before replays its recording; its first batch runs eagerly
+  const int need = rec ? seen_rounds(b) : 0;
+  if (rc == PHEM_OK && need > 0) {
     phem_mst_stats st{};
-    phem_batch_graph_t *g = recording(S, b, d_ws, ws, d_ids, d_cnt, d_lab, d_wfx, d_bad);
+    const int rounds = need + 2 < (int)o.record_rounds ? need + 2 : (int)o.record_rounds;
+    phem_batch_graph_t *g =
+        recording(S, b, rounds, d_ws, ws, d_ids, d_cnt, d_lab, d_wfx, d_bad);
     if (g) {
       rc = phem_batch_graph_launch(g, k, voff.data(), eoff.data(), pu.data(),
                                    pv.data(), pw.data(), S.stream);
       if (rc == PHEM_OK) rc = phem_batch_graph_stats(g, &st, S.stream);
       replayed = true;
-      if (rc == PHEM_ENOCONV) {  // more rounds than recorded: solve eagerly
+      if (rc == PHEM_ENOCONV) {
+        // more rounds than recorded: solve eagerly, and record the bucket
+        // again next time with the rounds this batch needs
         fell_back = true;
         rc = PHEM_OK;
+        (void)hipStreamSynchronize(S.stream);
+        for (auto it = S.rec.begin(); it != S.rec.end(); ++it)
+          if (it->b == b) {
+            phem_batch_graph_destroy(it->g);
+            S.rec.erase(it);
+            break;
+          }
       }
     }
   }
@@ -367,6 +430,7 @@ void phem_server::run_batch(Slot &S, std::vector<Req *> &batch) {
     A.d_inexact = d_bad;
     A.stats = &st;
     rc = batch_solve(A, S.stream);
+    if (rc == PHEM_OK && rec) note_rounds(b, phase_rounds(st));
   }
   // results: one copy per field into the pinned buffer, then to each request
   char *h = S.hout;

@@ -277,23 +277,28 @@ def test_wait_claims_its_ticket(P):
 
 @pytest.mark.parametrize("rounds", [12, 2])
 def test_recorded_batches_equal_oracle(P, rounds):
-    """With record_rounds set, batches replay the recorded solve of their
-    capacity bucket (host and device inputs mixed); with too few rounds they
-    fall back to the eager solve.  Either way every result equals the
-    oracle's."""
-    graphs = [host_graph(g, seed=7) for g in range(150)]
-    srv = P.Server(mem_bytes=512 << 20, slots=2, max_graphs=64, record_rounds=rounds)
-    out = [submit(srv, n, u, v, w, on_device=(g % 2 == 0))
-           for g, (n, u, v, w) in enumerate(graphs)]
-    srv.drain()
-    for g, (n, u, v, w) in enumerate(graphs):
-        t, ids, lab = out[g]
-        r = srv.wait(t)
-        assert r["status"] == 0, r["error"]
-        k_ids, k_lab = O.kruskal(n, u, v, w, 1)
-        assert np.array_equal(ids[: r["forest_count"]], k_ids), g
-        assert np.array_equal(lab[:n], k_lab), g
-        assert r["weight_fx"] == O.weight_sum(k_ids, w, 1)[0], g
+    """With record_rounds set, a capacity bucket's first batch runs eagerly
+    and later batches of the bucket replay its recorded solve (host and
+    device inputs mixed; the same 40 graphs submitted in four waves, each
+    gathered into one batch by the batching window, so that buckets repeat);
+    with too few rounds a replay falls back to the eager solve.  Either way
+    every result equals the oracle's."""
+    graphs = [host_graph(g, seed=7) for g in range(40)]
+    srv = P.Server(mem_bytes=512 << 20, slots=2, max_graphs=64, record_rounds=rounds,
+                   window_us=50000)
+    want = [O.kruskal(n, u, v, w, 1) for (n, u, v, w) in graphs]
+    for wave in range(4):
+        out = [submit(srv, n, u, v, w, on_device=(g % 2 == wave % 2))
+               for g, (n, u, v, w) in enumerate(graphs)]
+        srv.drain()
+        for g, (n, u, v, w) in enumerate(graphs):
+            t, ids, lab = out[g]
+            r = srv.wait(t)
+            assert r["status"] == 0, r["error"]
+            k_ids, k_lab = want[g]
+            assert np.array_equal(ids[: r["forest_count"]], k_ids), (wave, g)
+            assert np.array_equal(lab[:n], k_lab), (wave, g)
+            assert r["weight_fx"] == O.weight_sum(k_ids, w, 1)[0], (wave, g)
     st = srv.stats()
     print(f"rounds={rounds}: {st['batches']} batches, {st['recorded_batches']} replayed, "
           f"{st['record_fallbacks']} fell back, {st['recordings']} recordings", flush=True)
