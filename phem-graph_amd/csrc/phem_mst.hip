@@ -46,6 +46,7 @@
 // Every appended list (alive edges, component lists) is segmented: tile v of
 // a producer appends to segment v mod KSEG, whose counter has its own cache
 // line, so no global atomic is shared by all tiles.
+#include <atomic>
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -1373,11 +1374,27 @@ static void launch_input_pass(uint64_t m, hipStream_t s, A... args) {
 }
 
 // grp: the graphs of a batch (giant component per graph), or NULL.
+// Rounds the last solve of each size class (log2 n, log2 m) needed in its
+// first phase (canonical or light) and its heavy phase.  A solve launches
+// that many rounds of a phase before the phase's first counter read (at
+// most what one read reports), so a run of similar solves reads the
+// counters once per phase; the results do not depend on it.
+static std::atomic<uint8_t> g_phase_rounds[2][34][42];
+
+static int size_log2(uint64_t x, int cap) {
+  int l = 0;
+  while (x > 1 && l < cap) {
+    x >>= 1;
+    l++;
+  }
+  return l;
+}
+
 // tail: work enqueued before the final counter read, or NULL.
 // fr: fixed-round schedule with no counter read at all (stats unwritten).
-// first_rounds: rounds launched before the solve's first counter read (a
-// caller that expects that many rounds, as a stream does from its previous
-// batch; at most what one read can report, and 0 for the default).
+// first_rounds: rounds of the first phase launched before its first counter
+// read (a caller that expects that many, as a stream does from its previous
+// batch; 0: the size class's last count, or the default batch).
 template <class Src>
 static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
                           void *d_ws, uint64_t ws_bytes, uint32_t *d_mst_ids,
@@ -1418,6 +1435,11 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       (!opts || std::isnan(opts->light_factor)) && lf_skew > 0 && light_factor > 0
           ? mst_light_edges(n, m, lf_skew)
           : light_edges;
+
+  const int lgn = size_log2(n, 33), lgm = size_log2(m, 41);
+  int hint[2] = {first_rounds > 0 ? first_rounds
+                                  : (int)g_phase_rounds[0][lgn][lgm].load(std::memory_order_relaxed),
+                 (int)g_phase_rounds[1][lgn][lgm].load(std::memory_order_relaxed)};
 
   char *ws = (char *)d_ws;
   uint32_t *par = (uint32_t *)(ws + L.par);
@@ -1501,9 +1523,11 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
   // so the phase's first compaction relabels in two steps.
   auto run_rounds = [&](uint64_t list_ub, bool first_all,
                         std::initializer_list<std::pair<int, int>> extra,
-                        ull *extra_out, int fixed, bool giant_hooked = false) -> int {
+                        ull *extra_out, int fixed, int phase,
+                        bool giant_hooked = false) -> int {
     bool all = first_all;
     bool first = true;
+    int phase_real = 0;  // rounds of this phase that had alive edges
     while (m_r > 0) {
       // Up to SPEC rounds per counter read (exactly `fixed` rounds and no
       // read for a recorded solve).  A round launched after the phase has
@@ -1515,10 +1539,8 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       // the extra ones: at most MAX_SUMS
       const int read_cap = (MAX_SUMS - 2 - (first ? (int)extra.size() : 0)) / 2;
       int want = fixed ? fixed : SPEC;
-      if (!fixed && first_rounds > 0) {
-        want = first_rounds < read_cap ? first_rounds : read_cap;
-        first_rounds = 0;  // the hint covers the solve's first read only
-      }
+      if (!fixed && first && hint[phase] > 0)
+        want = hint[phase] < read_cap ? hint[phase] : read_cap;
       const int batch = PHEM_MAX_ROUNDS - r < want ? PHEM_MAX_ROUNDS - r : want;
       if (batch <= 0 || (fixed && batch < fixed))
         return set_err(PHEM_ENOCONV, "no convergence after %d rounds", r);
@@ -1615,6 +1637,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
         }
         real++;
       }
+      phase_real += real;
       m_r = got[batch];        // E(r + batch); 0 when the phase converged
       list_ub = got[2 * batch + 1];
       cur ^= (real & 1);
@@ -1622,6 +1645,9 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       all = false;
       if (real < batch) m_r = 0;
     }
+    if (!fixed && phase_real > 0)
+      g_phase_rounds[phase][lgn][lgm].store((uint8_t)(phase_real < 255 ? phase_real : 255),
+                                            std::memory_order_relaxed);
     return PHEM_OK;
   };
 
@@ -1665,7 +1691,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     m_r = m;
     cur = 0;
     int rc = run_rounds(n, false, {{cPart(0), KSEG}, {cA(0), KSEG}}, hv,
-                        fr ? fr->light : 0);
+                        fr ? fr->light : 0, 0);
     if (rc) return rc;
     const uint64_t valid = fr ? 1 : hv[0];
     // a round counts when it starts with at least one alive edge
@@ -1707,7 +1733,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     m_r = m;
     cur = 0;
     int rc = run_rounds(0, true, {{C_TAU, 1}, {cE(0), KSEG}}, hv,
-                        fr ? fr->light : 0);
+                        fr ? fr->light : 0, 0);
     if (rc) return rc;
     if (fr) {
       if (groups)
@@ -1803,7 +1829,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
     m_r = m + n;
     const int r_heavy = r;
     rc = run_rounds(0, true, {{cE(r_heavy), KSEG}, {cPart(3), KSEG}}, hv,
-                    fr ? fr->heavy : 0, true);
+                    fr ? fr->heavy : 0, 1, true);
     if (rc) return rc;
     if (stats && !fr) {
       stats->heavy_edges = hv[0];
