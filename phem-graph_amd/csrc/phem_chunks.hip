@@ -41,7 +41,8 @@ double now_s() {
       .count();
 }
 
-// Host-to-device staging through RING pinned pieces on one copy stream.
+// Host-to-device staging through RING pinned pieces (from the process's
+// pinned pool, returned when the call ends) on one copy stream.
 struct Stager {
   hipStream_t cs = nullptr;
   uint32_t *pin[RING] = {};
@@ -57,10 +58,8 @@ struct Stager {
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
       return fail("copy stream");
     for (int i = 0; i < RING; i++) {
-      if (hipHostMalloc((void **)&pin[i], 4 * PIECE) != hipSuccess) {
-        pin[i] = nullptr;
-        return fail("pinned staging");
-      }
+      pin[i] = (uint32_t *)pinned_acquire(4 * PIECE);
+      if (!pin[i]) return set_err(PHEM_ENOMEM, "pinned staging");
       if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) {
         ev[i] = nullptr;
         return fail("staging event");
@@ -72,7 +71,7 @@ struct Stager {
     if (cs) (void)hipStreamSynchronize(cs);
     for (int i = 0; i < RING; i++) {
       if (ev[i]) (void)hipEventDestroy(ev[i]);
-      if (pin[i]) (void)hipHostFree(pin[i]);
+      pinned_release(pin[i]);  // the copy stream was synchronised above
     }
     if (cs) (void)hipStreamDestroy(cs);
   }
