@@ -588,6 +588,8 @@ def main():
             "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": None if achieved is None else round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic,
+            "traffic_from": "profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE "
+                            "runs of this workload, calibrated with a 1 GiB copy",
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(per_launch_ms, 4)}
 
@@ -646,12 +648,10 @@ def main():
         out["configs"] = configs_bench(P, torch)
     if cpu_in is not None:
         host = host_cpus()
-        # every CPU the process may use, within its cgroup quota; 16 at most,
-        # the host share of one GPU on the measurement boxes
+        # every CPU the process may use, within its cgroup quota
         threads = host["affinity"] or 1
         if host["cgroup_cpus"]:
             threads = min(threads, max(1, int(host["cgroup_cpus"])))
-        threads = min(threads, 16)
         u_np, v_np, w_np = cpu_in
         k = len(u_np)
         print(f"# cpu baseline on {k} edges, {threads} threads, host {host}",
