@@ -326,7 +326,10 @@ def configs_bench(P, torch, reps=5):
     permuted (the same edges under a layout without id locality).  Each
     reports MST and CC time, edges/s, and for the MST the algorithmic bytes
     of its own round log (32 S_m - 20 m + 16 S_c + 12 n, exact when the
-    canonical schedule runs) over the time."""
+    canonical schedule runs) over the time.  Last, the uniform graph on 2^16
+    vertices with 2^20 edges, on the device and on the library's host backend
+    (phem_cpu_msf, phem_cpu_cc, every CPU the process may use, median of 5
+    after a warm-up)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import synth as S
@@ -389,6 +392,27 @@ def configs_bench(P, torch, reps=5):
     out["rmat_n2^24_m2^28_permuted"] = one("rmat permuted", 1 << 24, du, dv, dw)
     del du, dv, dw
     torch.cuda.empty_cache()
+    # C1: uniform n = 2^16, m = 2^20 on the GPU, and on the library's host
+    # backend (phem_cpu_msf, phem_cpu_cc) with every CPU the process may use
+    du, dv, dw = P.gen_uniform(1 << 16, 1 << 20, 1, device=dev)
+    c1 = one("uniform 2^16 / 2^20", 1 << 16, du, dv, dw)
+    hu, hv, hw = (x.cpu().numpy().view(np.uint32) for x in (du, dv, dw))
+    threads = len(os.sched_getaffinity(0))
+
+    def host_time(fn, reps=5):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2]
+    c1["host_backend"] = {
+        "threads": threads,
+        "msf_ms": round(host_time(lambda: P.cpu_msf(1 << 16, hu, hv, hw, P.W_F32, threads)) * 1e3, 3),
+        "cc_ms": round(host_time(lambda: P.cpu_cc(1 << 16, hu, hv, threads)) * 1e3, 3)}
+    out["c1_uniform_2^16_2^20"] = c1
+    del du, dv, dw
     return out
 
 
@@ -631,6 +655,9 @@ def main():
     if not args.no_cpu:
         k = min(m, 1 << args.cpu_sample_log2)
         cpu_in = [x[:k].cpu().numpy().view(np.uint32) for x in (du, dv, dw)]
+        # the uniform n = 2^16, m = 2^20 graph, timed whole on the same CPU path
+        cpu_c1 = [x.cpu().numpy().view(np.uint32)
+                  for x in P.gen_uniform(1 << 16, 1 << 20, 1, device=dev)]
     if not args.no_serve:
         del du, dv, dw, de, solver, cc_par, cc_lab, cc_forest
         torch.cuda.empty_cache()
@@ -672,6 +699,10 @@ def main():
                       "cc": {"kind": cc_kind, "s": round(t_cc, 4),
                              "edges_per_s": k / t_cc}},
             "host": host}
+        c1_eps, c1_mst, c1_cc, c1_kind = cpu_baseline(*cpu_c1, 1 << 16, threads, args.cpu_runs)
+        out["cpu_baseline"]["parts"]["c1_uniform_2^16_2^20"] = {
+            "edges_per_s": c1_eps, "mst_s": round(c1_mst, 5), "cc_s": round(c1_cc, 5),
+            "cc_kind": c1_kind}
 
     print(json.dumps(out), flush=True)
 
