@@ -85,13 +85,16 @@ typedef struct {
   double mst_split_ratio;   /* the light/heavy split runs when
                                m > ratio * light_factor * n (default 1.5;
                                > 1) */
-  uint32_t cc_block;        /* threads per block of the CC unions: 256
-                               (default), 512 */
+  uint32_t cc_block;        /* threads per block of the unfiltered CC union:
+                               256 (default), 512 */
   uint32_t cc_ept;          /* records per thread of the unfiltered union:
                                8, 16 (default), 32 */
   uint32_t cc_ept_filter;   /* records per thread of the filtered unions:
                                4, 8 (default), 16 */
-  uint32_t reserved2;
+  uint32_t cc_block_filter; /* threads per block of the filtered unions, whose
+                               LDS summary of the giant bitmap grows with the
+                               block (16 KB per 256 threads): 256, 512, 1024
+                               (default; cc_ept_filter 4 or 8) */
   double cpu_share;         /* phem_msf_host_chunks with PHEM_CPU_EDGES_AUTO:
                                the host reduces cpu_share x (edges before
                                the last chunk) edges of the last chunk

@@ -19,12 +19,22 @@ namespace phem {
 
 constexpr int CBLK = 256;  // threads per block of the small passes
 // The unions run with B threads per block and E records per thread per tile
-// (the cc_block, cc_ept and cc_ept_filter knobs).  Built-in shapes: 256
-// threads; 16 records per thread in the unfiltered union (every record a
-// union: more records in flight per thread measured faster, 0.72 ms against
-// 0.80 ms for the first n records at n = 2^24), 8 in the filtered passes
-// (fewer registers and more resident waves: 2.09 ms against 2.46 ms).
-constexpr uint32_t GSUM_CC = 4096;  // giant-summary words in LDS (16 KB)
+// (the cc_block / cc_ept knobs for the unfiltered union, cc_block_filter /
+// cc_ept_filter for the filtered ones).  Built-in shapes: 256 threads and 16
+// records per thread in the unfiltered union (every record a union: more
+// records in flight per thread measured faster, 0.72 ms against 0.80 ms for
+// the first n records at n = 2^24); 1024 threads and 8 records per thread in
+// the filtered passes (8 rather than 16: fewer registers and more resident
+// waves, 2.09 ms against 2.46 ms at 256 threads; 1024 threads: a 64 KB
+// summary at 8 waves per SIMD, 1.79 ms).
+
+// giant-summary words in LDS of the filtered unions: 16 KB per 256 threads
+// (8 resident blocks of 256 threads, 4 of 512 or 2 of 1024 use 128 KB of
+// the 160 KB)
+template <int B>
+constexpr uint32_t gsum_cc_words() {
+  return B >= 1024 ? 16384u : B >= 512 ? 8192u : 4096u;
+}
 
 __global__ __launch_bounds__(CBLK) void k_cc_init(uint32_t n, uint32_t *par,
                                                   ull *forest) {
@@ -52,9 +62,9 @@ __global__ __launch_bounds__(B) void k_cc_union(const ull *__restrict__ edges,
                                                 uint32_t *par, ull *forest,
                                                 const uint32_t *__restrict__ giant,
                                                 const uint32_t *gsum) {
-  __shared__ uint32_t s_sum[FILTER ? GSUM_CC : 1];
+  __shared__ uint32_t s_sum[FILTER ? gsum_cc_words<B>() : 1];
   uint32_t covered = 0;
-  if (FILTER) covered = gsum_load<B, GSUM_CC>(s_sum, gsum, n);
+  if (FILTER) covered = gsum_load<B, gsum_cc_words<B>()>(s_sum, gsum, n);
   constexpr int CTILE = B * CEPT;
   for (uint64_t tile = blockIdx.x; tile * CTILE < m; tile += gridDim.x) {
     const uint64_t base = tile * CTILE + threadIdx.x;
@@ -69,16 +79,19 @@ __global__ __launch_bounds__(B) void k_cc_union(const ull *__restrict__ edges,
     // parent is inside one tree already.  A parent read here may be older than
     // the current one, but every value it ever held is an ancestor, so equal
     // parents still prove one tree.
-    uint32_t pa[CEPT], pb[CEPT];
     bool ok[CEPT];
 #pragma unroll
     for (int j = 0; j < CEPT; j++) {
       const uint32_t a = (uint32_t)(e[j] >> 32), b = (uint32_t)e[j];
       ok[j] = a < b && b < n;
+      // b's test only when a is in the giant component (measured faster
+      // than issuing both bitmap reads together: 2.09 against 2.22 ms of
+      // filtered unions at RMAT n = 2^24, m = 2^28)
       if (FILTER && ok[j])
         ok[j] = !(giant_test(a, s_sum, covered, giant) &&
                   giant_test(b, s_sum, covered, giant));
     }
+    uint32_t pa[CEPT], pb[CEPT];
 #pragma unroll
     for (int j = 0; j < CEPT; j++) {
       const uint32_t a = (uint32_t)(e[j] >> 32), b = (uint32_t)e[j];
@@ -123,7 +136,8 @@ static void launch_cc_union(const ull *edges, uint64_t m, uint32_t n,
                             uint32_t *par, ull *forest, const uint32_t *giant,
                             const uint32_t *gsum, hipStream_t s) {
   const KernelShape k = kernel_shape();
-  const uint32_t B = k.cc_block, E = FILTER ? k.cc_ept_filter : k.cc_ept;
+  const uint32_t B = FILTER ? k.cc_block_filter : k.cc_block;
+  const uint32_t E = FILTER ? k.cc_ept_filter : k.cc_ept;
   const unsigned grid = grid_for_threads(m, (uint64_t)B * E, B);
 #define PHEM_CC_UNION(BB, EE)                                                  \
   if (B == BB && E == EE) {                                                    \
@@ -134,12 +148,13 @@ static void launch_cc_union(const ull *edges, uint64_t m, uint32_t n,
   if constexpr (FILTER) {
     PHEM_CC_UNION(256, 4) PHEM_CC_UNION(256, 8) PHEM_CC_UNION(256, 16)
     PHEM_CC_UNION(512, 4) PHEM_CC_UNION(512, 8) PHEM_CC_UNION(512, 16)
+    PHEM_CC_UNION(1024, 4) PHEM_CC_UNION(1024, 8)
   } else {
     PHEM_CC_UNION(256, 8) PHEM_CC_UNION(256, 16) PHEM_CC_UNION(256, 32)
     PHEM_CC_UNION(512, 8) PHEM_CC_UNION(512, 16) PHEM_CC_UNION(512, 32)
   }
 #undef PHEM_CC_UNION
-  // phem_set_tuning admits no other shape; the built-in one as a backstop
+  // phem_set_tuning admits no other shape; 256 threads as a backstop
   hipLaunchKernelGGL((k_cc_union<FILTER, 256, FILTER ? 8 : 16>),
                      dim3(grid_for(m, 256ull * (FILTER ? 8 : 16))), dim3(256),
                      0, s, edges, m, n, par, forest, giant, gsum);

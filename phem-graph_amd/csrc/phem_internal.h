@@ -29,7 +29,7 @@ double cc_default_sample_factor();
 struct KernelShape {
   uint32_t mst_block, mst_ept, mst_ept_compact, mst_compact_log2;
   double mst_split_ratio;
-  uint32_t cc_block, cc_ept, cc_ept_filter;
+  uint32_t cc_block, cc_ept, cc_ept_filter, cc_block_filter;
   double cpu_share;
   double mst_light_factor_skewed;
 };

@@ -124,11 +124,11 @@ __global__ __launch_bounds__(SPLIT_BLK) void k_giant_bits(uint32_t n,
 // Summary of the giant bitmap kept in LDS by the kernels that test it: bit
 // w of the summary = bitmap word w is all ones (its 32 vertices are all in
 // the giant component).  Summary word k covers vertices [1024 k, 1024 k +
-// 1024); the array holds up to GSUM_MAX words and a kernel keeps the first W
-// of them in LDS (W * 4 bytes).  A set bit answers the test without an L2
+// 1024); the array holds up to GSUM_MAX words (every vertex id below 2^24)
+// and a kernel keeps the first W of them in LDS (W * 4 bytes).  A set bit answers the test without an L2
 // request; a clear bit, or a vertex beyond the kernel's W words, falls back
 // to the bitmap.
-constexpr uint32_t GSUM_MAX = 8192;
+constexpr uint32_t GSUM_MAX = 16384;
 
 __host__ __device__ constexpr uint32_t gsum_words(uint32_t n, uint32_t W) {
   return ((uint64_t)n + 1023) / 1024 < W ? (uint32_t)(((uint64_t)n + 1023) / 1024)

@@ -119,7 +119,7 @@ class Tuning(ctypes.Structure):
                 ("mst_ept_compact", ctypes.c_uint32), ("mst_compact_log2", ctypes.c_uint32),
                 ("mst_split_ratio", ctypes.c_double),
                 ("cc_block", ctypes.c_uint32), ("cc_ept", ctypes.c_uint32),
-                ("cc_ept_filter", ctypes.c_uint32), ("reserved2", ctypes.c_uint32),
+                ("cc_ept_filter", ctypes.c_uint32), ("cc_block_filter", ctypes.c_uint32),
                 ("cpu_share", ctypes.c_double), ("mst_light_factor_skewed", ctypes.c_double)]
 
 
@@ -132,7 +132,8 @@ class ChunksStats(ctypes.Structure):
 # the knobs of phem_tuning, in order (the reserved words excluded)
 TUNING_KNOBS = ("blocks_per_cu", "mst_light_factor", "cc_sample_factor", "mst_block",
                 "mst_ept", "mst_ept_compact", "mst_compact_log2", "mst_split_ratio",
-                "cc_block", "cc_ept", "cc_ept_filter", "cpu_share", "mst_light_factor_skewed")
+                "cc_block", "cc_ept", "cc_ept_filter", "cpu_share", "mst_light_factor_skewed",
+                "cc_block_filter")
 _FLOAT_KNOBS = ("mst_light_factor", "cc_sample_factor", "mst_split_ratio", "cpu_share",
                 "mst_light_factor_skewed")
 CPU_EDGES_AUTO = 0xFFFFFFFFFFFFFFFF

@@ -13,9 +13,10 @@
   mst_ept (threads per block and edges per thread of the MST input passes),
   mst_ept_compact and mst_compact_log2 (edges per thread of the compaction
   rounds and the edge count from which they use it), mst_split_ratio (the
-  light/heavy split runs when m > ratio * light_factor * n), cc_block,
-  cc_ept and cc_ept_filter (the CC unions' threads per block and records per
-  thread, unfiltered and filtered).
+  light/heavy split runs when m > ratio * light_factor * n), cc_block and
+  cc_ept (threads per block and records per thread of the unfiltered CC
+  union), cc_block_filter and cc_ept_filter (the same of the filtered unions,
+  whose LDS summary of the giant bitmap grows with the block).
 
 tune() builds a synthetic graph of the caller's shape on the device (RMAT or
 uniform, n vertices, m edges), then improves one knob at a time (coordinate
@@ -48,16 +49,18 @@ GRID = {
     "cc_block": [256, 512],
     "cc_ept": [8, 16, 32],
     "cc_ept_filter": [4, 8, 16],
+    "cc_block_filter": [256, 512, 1024],
     "cc_sample_factor": [0.5, 1.0, 1.5, 2.0],
 }
 # knobs timed on the MST solve alone and on the CC solve alone (the rest on
 # both), in the order coordinate descent visits them
 MST_KNOBS = ("mst_block", "mst_ept", "mst_ept_compact", "mst_compact_log2",
              "mst_light_factor", "mst_split_ratio")
-CC_KNOBS = ("cc_block", "cc_ept", "cc_ept_filter", "cc_sample_factor")
+CC_KNOBS = ("cc_block", "cc_ept", "cc_block_filter", "cc_ept_filter", "cc_sample_factor")
 ORDER = ("blocks_per_cu",) + MST_KNOBS + CC_KNOBS
 # swept together
-JOINT = (("mst_block", "mst_ept"), ("cc_block", "cc_ept", "cc_ept_filter"))
+JOINT = (("mst_block", "mst_ept"), ("cc_block", "cc_ept"),
+         ("cc_block_filter", "cc_ept_filter"))
 
 
 def _device_name():
@@ -175,6 +178,8 @@ def tune(n, m, graph="rmat", seed=1, reps=3, grid=None, device="cuda", verbose=F
             trial = {**best, **c}
             if trial["mst_block"] * trial["mst_ept"] > 4096:
                 continue  # a tile beyond 4096 edges is not compiled
+            if trial["cc_block_filter"] == 1024 and trial["cc_ept_filter"] > 8:
+                continue  # not compiled
             P.set_tuning(**trial)
             ms = cost(knobs[0])
             scores.append((ms, c))

@@ -110,6 +110,31 @@ def test_cc_merge_cpu_and_gpu_forests(P):
     assert int((fo != NO_EDGE).sum()) == n - len(np.unique(want))
 
 
+@pytest.mark.parametrize("fblock", [256, 512, 1024])
+def test_cc_filtered_shapes_rmat(P, fblock):
+    """Every block size of the filtered unions (LDS summaries of 16, 32 and
+    64 KB; at n = 2^23 the summary covers all vertices from 512 threads up)
+    and both record counts per thread give the oracle's labels and a spanning
+    forest, on a graph large enough for a giant component to form."""
+    scale, m = 23, 1 << 26
+    n = 1 << scale
+    du, dv, _ = P.gen_rmat(scale, m, 5, positive=True)
+    e = P.pack_edges(du, dv)
+    u, v = du.cpu().numpy().view(np.uint32), dv.cpu().numpy().view(np.uint32)
+    want = O.par_cc(n, u, v)
+    old = P.get_tuning()
+    try:
+        for eptf in (4, 8):
+            P.set_tuning(cc_block_filter=fblock, cc_ept_filter=eptf)
+            print(f"cc filtered block={fblock} ept={eptf}", flush=True)
+            lab, _, f = P.cc_solve(n, e)
+            torch.cuda.synchronize()
+            assert np.array_equal(lab.cpu().numpy().view(np.uint32), want), eptf
+            assert_spanning_forest(n, f.cpu().numpy().view(np.uint64), want)
+    finally:
+        P.set_tuning(**old)
+
+
 @pytest.mark.slow
 def test_cc_rmat_full_size(P):
     scale, m = 24, 1 << 28
@@ -128,3 +153,15 @@ def test_cc_rmat_full_size(P):
     want = O.par_cc(n, u, v)
     assert np.array_equal(lab, want)
     assert_spanning_forest(n, f.cpu().numpy().view(np.uint64), want)
+    # the largest filtered-union shape (64 KB LDS summary over all 2^24 ids)
+    old = P.get_tuning()
+    try:
+        P.set_tuning(cc_block_filter=1024, cc_ept_filter=8)
+        e = P.pack_edges(torch.from_numpy(u.view(np.int32)).cuda(),
+                         torch.from_numpy(v.view(np.int32)).cuda())
+        lab2, _, f2 = P.cc_solve(n, e)
+        torch.cuda.synchronize()
+        assert np.array_equal(lab2.cpu().numpy().view(np.uint32), want)
+        assert_spanning_forest(n, f2.cpu().numpy().view(np.uint64), want)
+    finally:
+        P.set_tuning(**old)

@@ -116,11 +116,15 @@ def test_batch_with_shapes(P, block, ept):
         P.set_tuning(mst_block=256, mst_ept=8)
 
 
-@pytest.mark.parametrize("block", [256, 512])
-def test_cc_union_shapes(P, block):
+@pytest.mark.parametrize("block,fblock", [(256, 256), (512, 512), (256, 1024)])
+def test_cc_union_shapes(P, block, fblock):
     for ept, eptf in itertools.product([8, 16, 32], [4, 8, 16]):
-        P.set_tuning(cc_block=block, cc_ept=ept, cc_ept_filter=eptf)
-        print(f"cc shape block={block} ept={ept} filter={eptf}", flush=True)
+        if fblock == 1024 and eptf > 8:
+            continue  # not compiled
+        P.set_tuning(cc_block=block, cc_ept=ept, cc_block_filter=fblock,
+                     cc_ept_filter=eptf)
+        print(f"cc shape block={block} ept={ept} filter block={fblock} ept={eptf}",
+              flush=True)
         for name in CASES:
             n, u, v, w, wmode = cases.make(name)
             rec = ((np.minimum(u, v).astype(np.uint64) << np.uint64(32))
@@ -130,4 +134,4 @@ def test_cc_union_shapes(P, block):
             assert np.array_equal(host(lab), want), name
             f = forest.cpu().numpy().view(np.uint64)
             assert int((f != np.uint64(P.NO_EDGE)).sum()) == n - len(np.unique(want)), name
-    P.set_tuning(cc_block=256, cc_ept=16, cc_ept_filter=8)
+    P.set_tuning(cc_block=256, cc_ept=16, cc_block_filter=256, cc_ept_filter=8)

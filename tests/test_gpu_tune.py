@@ -28,8 +28,9 @@ def test_tune_small_graph(P, tmp_path):
         r = T.tune(1 << 16, 1 << 20, "rmat", seed=3, reps=2, grid=grid)
         for k, vals in grid.items():
             assert r["config"][k] in vals
-        # 2 + 4 (block x ept jointly) + 2 + 2 (cc shapes jointly) + 2 settings
-        assert len(r["log"]) == 12
+        # 2 + 4 (block x ept jointly) + 2 + 2 (cc_block x cc_ept) + 1 (the
+        # filtered unions' shape) + 2 settings
+        assert len(r["log"]) == 13
         path = str(tmp_path / "tune.json")
         T.save(r, path)
         assert T.load(1 << 16, 1 << 20, path) == r["config"]
