@@ -422,7 +422,9 @@ def stream_bench(P, torch, scale, m0_log2, batches, batch_log2, seed):
     insertions (all generated on the device beforehand).  Per-batch latency is
     the host time of phem_stream_apply (which returns with the forest and the
     union-find labels updated).  At the end the resident forest is compared
-    with a from-scratch solve over every edge inserted."""
+    with a from-scratch solve over every edge inserted.  The same batches then
+    go through the union-find alone (spanning forest and labels, no minimum
+    forest), timed per batch the same way."""
     import numpy as np
     import phem_workloads as W
 
@@ -454,24 +456,66 @@ def stream_bench(P, torch, scale, m0_log2, batches, batch_log2, seed):
     s = st.stats()
     fu, fv, fw, fg = st.forest()
     gid = fg.cpu().numpy()
+    stream_labels = st.labels()
+    st.close()
+
+    # Union-find alone over the same batches: the reference's batched
+    # spanning-forest loop (CC/ExternalSpanningTree.cu:133-218) with the parent
+    # array and the union-find forest resident; a batch's latency runs until
+    # its unions have finished on the device.
+    rec_base = P.pack_edges(base[0], base[1])
+    recs = [P.pack_edges(x[0], x[1]) for x in bat]
+    cpar = torch.empty(n, dtype=torch.int32, device=dev)
+    cfor = torch.empty(n, dtype=torch.int64, device=dev)
+    P.cc_init(n, cpar, cfor)
+    P.cc_union(n, rec_base, cpar, cfor)
+    torch.cuda.synchronize()
+    lat_cc = []
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        for b in range(batches):
+            a = time.perf_counter()
+            P.cc_union(n, recs[b], cpar, cfor)
+            torch.cuda.synchronize()
+            lat_cc.append(time.perf_counter() - a)
+        t_cc = time.perf_counter() - t0
+    finally:
+        gc.enable()
+    clab = torch.empty(n, dtype=torch.int32, device=dev)
+    P.cc_labels(n, cpar, clab)
+    cc_forest = int((cfor != -1).sum())
+    del recs, rec_base
+
     # every edge inserted, in insertion order -> one solve from scratch
     cat = [torch.cat([base[i]] + [x[i] for x in bat]) for i in range(3)]
     del bat
     ids, lab, rs = P.mst_solve(n, cat[0], cat[1], cat[2], P.W_F32)
     same = bool(np.array_equal(ids.cpu().numpy().view(np.uint32).astype(np.int64), gid)
                 and rs["weight_fx"] == s["weight_fx"]
-                and torch.equal(lab, st.labels()))
-    st.close()
+                and torch.equal(lab, stream_labels))
+    # the union-find stream: the same labels, and a forest of n - c edges
+    comps = int(torch.unique(lab).numel())
+    cc_same = bool(torch.equal(clab, lab)) and cc_forest == n - comps
     lat.sort()
-    pct = lambda p: lat[max(0, int(np.ceil(p * len(lat))) - 1)]
+    lat_cc.sort()
+    pct = lambda x, p: x[max(0, int(np.ceil(p * len(x))) - 1)]
     return {"workload": f"stream_rmat_n2^{scale}_m2^{m0_log2}+{batches}x2^{batch_log2}",
             "base_ms": round(t_base * 1e3, 3),
             "batch_edges_per_s": batches * mb / t_all,
-            "batch_lat_p50_ms": round(pct(.5) * 1e3, 3),
-            "batch_lat_p99_ms": round(pct(.99) * 1e3, 3),
+            "batch_lat_p50_ms": round(pct(lat, .5) * 1e3, 3),
+            "batch_lat_p99_ms": round(pct(lat, .99) * 1e3, 3),
             "slowest_batches": slow,
             "forest_edges": len(gid), "edges_inserted": s["edges_inserted"],
-            "forest_equals_recompute": same}
+            "forest_equals_recompute": same,
+            "union_find_only": {
+                "what": "phem_cc_union per batch on the resident parent array "
+                        "(spanning forest and labels; no minimum forest)",
+                "batch_edges_per_s": batches * mb / t_cc,
+                "batch_lat_p50_ms": round(pct(lat_cc, .5) * 1e3, 4),
+                "batch_lat_p99_ms": round(pct(lat_cc, .99) * 1e3, 4),
+                "labels_equal_recompute": cc_same}}
 
 
 def main():
