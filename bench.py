@@ -239,7 +239,11 @@ def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_gra
         print(f"# served {label}: saturated {edges / wall / 1e9:.2f} G edges/s, poisson "
               f"p50 {st_p['lat_p50'] * 1e3:.2f} ms p99 {st_p['lat_p99'] * 1e3:.2f} ms",
               file=sys.stderr, flush=True)
-    best = max(modes, key=lambda k: modes[k]["saturated"]["edges_per_s"])
+    # the headline mode: the lower Poisson p99 among the modes within 3 % of
+    # the best saturated rate (both modes are reported under "modes")
+    top = max(modes[k]["saturated"]["edges_per_s"] for k in modes)
+    near = [k for k in modes if modes[k]["saturated"]["edges_per_s"] >= 0.97 * top]
+    best = min(near, key=lambda k: modes[k]["poisson"]["lat_p99_ms"])
     # host-resident inputs on a prefix of the mix: every latency includes the
     # staging into pinned memory, the host-to-device copies and the result
     # copies back; all at once (the staging rate), then Poisson arrivals
