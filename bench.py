@@ -244,6 +244,19 @@ def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_gra
     top = max(modes[k]["saturated"]["edges_per_s"] for k in modes)
     near = [k for k in modes if modes[k]["saturated"]["edges_per_s"] >= 0.97 * top]
     best = min(near, key=lambda k: modes[k]["poisson"]["lat_p99_ms"])
+    # resident streams in the mix: 4 streams (an RMAT 2^20 base of 2^22 edges,
+    # then batches of 2^16 uniform edges) whose 64 batches arrive as a
+    # Poisson stream over the span of the graph requests' Poisson arrivals
+    # (the same rate as the Poisson leg); every stream's forest weight after
+    # each batch is checked against a stand-alone stream fed the same batches
+    streams_leg = serve_streams_leg(P, torch, graphs, (out_ids, out_lab), run_served,
+                                    slots, max_batch_edges,
+                                    record_rounds if best == "recorded" else 0,
+                                    load * graphs_n / (edges / modes[best]["saturated"]["edges_per_s"]),
+                                    rng)
+    print(f"# served with streams: graphs p99 {streams_leg['graph_lat_p99_ms']:.2f} ms, "
+          f"stream batches p50 {streams_leg['stream_lat_p50_ms']:.2f} ms "
+          f"p99 {streams_leg['stream_lat_p99_ms']:.2f} ms", file=sys.stderr, flush=True)
     # host-resident inputs on a prefix of the mix: every latency includes the
     # staging into pinned memory, the host-to-device copies and the result
     # copies back; all at once (the staging rate), then Poisson arrivals
@@ -281,11 +294,91 @@ def serve_bench(P, torch, graphs_n, seed, slots, max_batch_edges, load, host_gra
                                     "rate_req_per_s": round(rate_h, 1),
                                     "latency": "done - arrival, including staging, "
                                                "host-to-device and result copies"}),
+        "served_with_streams": streams_leg,
         "speedup_saturated_vs_plain": round(modes[best]["saturated"]["edges_per_s"] * t_plain / edges, 2),
         "speedup_poisson_vs_plain": round(modes[best]["poisson"]["edges_per_s"] * t_plain / edges, 2),
         "slots": slots, "max_batch_edges": max_batch_edges, "record_rounds": record_rounds,
         "weights_mismatch_vs_plain": bad,
     }
+
+
+def serve_streams_leg(P, torch, graphs, outs, run_served, slots, max_batch_edges,
+                      record_rounds, rate, rng, n_streams=4, batches=16, batch_log2=16):
+    """The graph mix with Poisson arrivals at `rate` requests/s, served beside
+    n_streams resident streams (phem_server_stream_open; an RMAT 2^20 base of
+    2^22 edges each, applied before the timed run) whose n_streams x batches
+    batches of 2^batch_log2 uniform edges arrive as a Poisson stream over the
+    same span, stream s taking every n_streams-th batch.  Reports the latency
+    (done - arrival) of the graph requests and of the stream batches apart,
+    and the aggregate edges/s of both; each stream's forest size and exact
+    weight after every batch are compared with a stand-alone phem_stream fed
+    the same batches."""
+    import numpy as np
+    import phem_workloads as W
+
+    dev = torch.device("cuda:0")
+    ns, nb, sb = n_streams, batches, 1 << batch_log2
+    sn, sbase = 1 << 20, 1 << 22
+    srv = P.Server(mem_bytes=0, slots=slots, max_graphs=4096,
+                   max_batch_edges=max_batch_edges, wmode=P.W_F32,
+                   record_rounds=record_rounds)
+    sids = [srv.stream_open(sn, sbase) for _ in range(ns)]
+    bases = [P.gen_rmat(20, sbase, 100 + s, positive=True, device=dev) for s in range(ns)]
+    sbat = [[P.gen_uniform(sn, sb, W.stream_seed(200 + s, b), device=dev) for b in range(nb)]
+            for s in range(ns)]
+    torch.cuda.synchronize()
+    for s in range(ns):
+        r = srv.wait(srv.submit_batch(sids[s], *bases[s]))
+        assert r["status"] == 0, r["error"]
+    run_served(srv, graphs, outs, None)  # warm-up
+    srv.reset_stats()
+    g_arr = np.cumsum(rng.exponential(1.0 / rate, len(graphs)))
+    s_arr = np.cumsum(rng.exponential(g_arr[-1] / (ns * nb), ns * nb))
+    events = sorted([(t, 0, i) for i, t in enumerate(g_arr)] +
+                    [(t, 1, j) for j, t in enumerate(s_arr)])
+    tick_g, tick_s = [None] * len(graphs), [None] * (ns * nb)
+    base = P.Server.clock()
+    for t, kind, i in events:
+        at = base + t
+        while P.Server.clock() < at:
+            pass
+        if kind == 0:
+            n, u, v, w = graphs[i]
+            tick_g[i] = srv.submit(n, u, v, w, outs[0][i], outs[1][i], at)
+        else:
+            tick_s[i] = srv.submit_batch(sids[i % ns], *sbat[i % ns][i // ns], arrival=at)
+    res_g = [srv.wait(t) for t in tick_g]
+    res_s = [srv.wait(t) for t in tick_s]
+    st = srv.stats()
+    for sid in sids:
+        srv.stream_close(sid)
+    srv.close()
+    bad = sum(r["status"] != 0 for r in res_g + res_s)
+    # stand-alone streams fed the same batches in the same order
+    same = bad == 0
+    for s in range(ns):
+        ref = P.Stream(sn, sbase, wmode=P.W_F32, device=dev)
+        ref.apply(*bases[s])
+        for b in range(nb):
+            ref.apply(*sbat[s][b])
+            rs = ref.stats()
+            r = res_s[b * ns + s]
+            same = same and r["forest_count"] == rs["mst_count"] and r["weight_fx"] == rs["weight_fx"]
+        ref.close()
+    lat = lambda rs: sorted(r["done"] - r["arrival"] for r in rs)
+    pct = lambda l, p: l[max(0, int(np.ceil(p * len(l))) - 1)]
+    lg, ls = lat(res_g), lat(res_s)
+    e_g = sum(g[1].numel() for g in graphs)
+    wall = st["last_done"] - st["first_arrival"]
+    return {"streams": ns, "stream_vertices": sn, "stream_base_edges": sbase,
+            "stream_batches": ns * nb, "stream_batch_edges": sb,
+            "graphs": len(graphs), "rate_req_per_s": round(rate, 1),
+            "edges_per_s": (e_g + ns * nb * sb) / wall,
+            "graph_lat_p50_ms": round(pct(lg, .5) * 1e3, 3),
+            "graph_lat_p99_ms": round(pct(lg, .99) * 1e3, 3),
+            "stream_lat_p50_ms": round(pct(ls, .5) * 1e3, 3),
+            "stream_lat_p99_ms": round(pct(ls, .99) * 1e3, 3),
+            "failed": bad, "streams_equal_standalone": bool(same)}
 
 
 def replay_bench(P, torch, scale, m_log2, seed, reps=30):

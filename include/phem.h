@@ -311,6 +311,10 @@ int phem_cc_merge_forests(uint32_t n, const uint64_t *d_forest,
 
 typedef struct phem_stream phem_stream_t;
 
+/* Device bytes phem_stream_create allocates for n vertices and batches of up
+ * to max_batch edges (no counterpart in the reference; the server admits
+ * resident streams by it). */
+int phem_stream_bytes(uint32_t n, uint64_t max_batch, uint64_t *bytes);
 /* Device memory for n vertices and batches of up to max_batch edges
  * (n + max_batch <= PHEM_MAX_EDGES), on the current device. */
 int phem_stream_create(uint32_t n, uint64_t max_batch, int wmode,
@@ -446,6 +450,11 @@ int phem_batch_graph_destroy(phem_batch_graph_t *g);
  * joins the next batch a worker forms, so work from several batches overlaps
  * on the device.  Results go to the request's own host buffers.
  * A request larger than one slot can hold is refused with PHEM_EBUSY.
+ * The server also keeps resident streams (phem_server_stream_open): their
+ * forest, parent array and union-find state stay in HBM between the
+ * streaming edge batches that clients submit to them (PHEM_REQ_STREAM
+ * requests), which queue with the graph requests and run on the same
+ * workers, each batch on its own.
  * Times are seconds on phem_server_clock() (a monotonic clock).
  * ------------------------------------------------------------------------- */
 typedef struct phem_server phem_server;
@@ -467,10 +476,14 @@ typedef struct {
                                the eager solve (and the bucket is recorded
                                again with more rounds).  0: every batch is
                                solved eagerly */
-  uint32_t reserved;
+  uint32_t stream_mem_mib;  /* device memory of the resident streams, MiB
+                               (0: as much as mem_bytes) */
 } phem_server_opts;
 
 #define PHEM_REQ_DEVICE 1u /* u, v, w are device pointers */
+#define PHEM_REQ_STREAM 2u /* a batch of m edges (the stream's vertex ids) for
+                              resident stream stream_id; n is ignored and
+                              forest_ids must be NULL */
 
 typedef struct {
   uint32_t n;
@@ -481,11 +494,13 @@ typedef struct {
                                 (NULL: not returned) */
   uint32_t *labels;          /* host, n slots: min-id labels (or NULL) */
   double arrival;            /* < 0: the time of submission */
+  uint64_t stream_id;        /* PHEM_REQ_STREAM: the resident stream */
 } phem_request;
 
 typedef struct {
   int32_t status;        /* PHEM_OK or the error of its batch */
-  uint32_t forest_count; /* edges in forest_ids */
+  uint32_t forest_count; /* edges in forest_ids (a stream batch: edges of the
+                            stream's forest after the batch) */
   uint64_t weight_fx;    /* exact forest weight (x 2^24 for float weights) */
   int32_t weight_exact;
   uint32_t batch_graphs; /* graphs in the batch the request ran in */
@@ -505,6 +520,11 @@ typedef struct {
   uint64_t record_fallbacks;  /* of those, solved again eagerly (more rounds
                                  needed than recorded) */
   uint64_t recordings;        /* recorded solves created (capture + instantiate) */
+  /* stream batches (counted in completed, edges and the latencies above
+   * too), their own latency percentiles, and the resident streams */
+  uint64_t stream_batches, stream_edges;
+  double stream_lat_p50, stream_lat_p99;
+  uint64_t streams_open, stream_bytes; /* device bytes the open streams hold */
 } phem_server_stats;
 
 double phem_server_clock(void);
@@ -524,6 +544,20 @@ int phem_server_poll(phem_server *srv, uint64_t ticket, phem_result *res,
 int phem_server_drain(phem_server *srv);
 int phem_server_get_stats(phem_server *srv, phem_server_stats *st);
 int phem_server_reset_stats(phem_server *srv);
+/* A resident stream of n vertices taking batches of up to max_batch edges
+ * (a phem_stream owned by the server, on the server's device).  Admitted by
+ * its device footprint (phem_stream_bytes): beyond the stream memory still
+ * free (opts.stream_mem_mib) it is refused with PHEM_EBUSY.  A stream's
+ * batches are applied in submission order, whichever workers take them; a
+ * batch's result holds the forest's edge count and exact weight after it,
+ * and its labels (if requested) the stream's min-id labels after it.  No
+ * counterpart in the reference, whose streams live in one process run
+ * (MST/src/kernels.cu:942-1102, CC/ExternalSpanningTree.cu:170-204). */
+int phem_server_stream_open(phem_server *srv, uint32_t n, uint64_t max_batch,
+                            uint64_t *stream_id);
+/* Frees the stream; PHEM_EBUSY while batches of it are queued or running,
+ * PHEM_ENOTFOUND for an unknown id. */
+int phem_server_stream_close(phem_server *srv, uint64_t stream_id);
 
 /* ---------------------------------------------------------------------------
  * Host-resident edge lists streamed through the device in chunks.

@@ -19,7 +19,17 @@
 // eagerly and notes the rounds it needed; the recording is made when the
 // bucket comes again, with those rounds plus two (at most record_rounds), so
 // shapes seen once cost no recording and replays run no idle rounds beyond
-// the margin.  Host inputs are copied into
+// the margin.
+//
+// Resident streams (phem_server_stream_open) are phem_streams the server
+// owns, admitted by their device footprint against a memory budget of their
+// own.  A stream batch queues with the graph requests and runs alone on the
+// worker that takes it.  A stream's batches carry sequence numbers, and a
+// worker takes a stream batch only when its stream is idle and the batch
+// before it has been applied (it passes over the others, which stay queued),
+// so the batches of one stream are applied in submission order while batches
+// of other streams and graph batches run on the other workers.  Host inputs
+// are copied into
 // pinned staging by the slot's copy threads in chunks, each chunk's
 // host-to-device copy running while the next chunk is filled.
 #include <algorithm>
@@ -49,12 +59,27 @@ double now_s() {
   return std::chrono::duration<double>(Clock::now().time_since_epoch()).count();
 }
 
+struct Resident;
+
 struct Req {
   phem_request q;
   phem_result res;
   uint64_t ticket;
   bool done = false;
   bool claimed = false;  // a phem_server_wait is waiting for it
+  Resident *rs = nullptr;  // a stream batch: its stream and sequence number
+  uint64_t seq = 0;
+};
+
+// A resident stream: the phem_stream, its footprint, and the sequence
+// numbers of its batches (all under the server mutex).
+struct Resident {
+  phem_stream_t *s = nullptr;
+  uint32_t n = 0;
+  uint64_t max_batch = 0, bytes = 0;
+  uint64_t submitted = 0;  // batches submitted: the next sequence number
+  uint64_t applied = 0;    // batches finished (applied or failed)
+  bool running = false;    // a worker is applying one of its batches
 };
 
 // A batch shape a recording is made for.
@@ -137,12 +162,17 @@ struct phem_server {
   std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
   uint64_t next_ticket = 1, in_flight = 0;
   bool stop = false;
+  // resident streams (under mu)
+  std::unordered_map<uint64_t, std::unique_ptr<Resident>> streams;
+  uint64_t next_stream = 1, stream_mem = 0, stream_used = 0;
   // statistics since the last reset
   uint64_t submitted = 0, completed = 0, refused = 0, batches = 0, edges = 0,
            batch_graphs = 0, recorded_batches = 0, record_fallbacks = 0,
            recordings = 0;
   double first_arrival = 0, last_done = 0;
   std::vector<double> lat;
+  uint64_t stream_batches = 0, stream_edges = 0;
+  std::vector<double> stream_lat;
 
   // Device bytes of a batch of k graphs with n vertices, m edges of which
   // mh come from host memory, solved eagerly (exact sizes) or replaying the
@@ -175,8 +205,14 @@ struct phem_server {
     if (!fits(n, m, mh, k)) return false;
     return !o.record_rounds || k == 1 || recorded_fits(n, m, mh, k);
   }
+  // A batch of m edges for a stream of n vertices: the slot holds its labels
+  // and, from host memory, its staged edges (within the pinned staging).
+  bool stream_fits(uint32_t n, uint64_t mh) const {
+    return mh <= o.max_batch_edges && a256(4ull * n) + 12 * mh <= slot_bytes;
+  }
   void worker(int si);
   void run_batch(Slot &S, std::vector<Req *> &batch);
+  void run_stream(Slot &S, Req *r);
   // Rounds per phase bucket b's eager batches needed (0: none seen yet), and
   // noting them (the least recently used bucket is forgotten beyond
   // MAX_SEEN).
@@ -219,31 +255,130 @@ void phem_server::worker(int si) {
     batch.clear();
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-      if (queue.empty()) return;  // stop requested and nothing left
-      if (o.window_us && queue.size() < o.max_graphs && !stop) {
-        const auto until = Clock::now() + std::chrono::microseconds(o.window_us);
-        cv_work.wait_until(lk, until, [&] {
-          return stop || queue.size() >= o.max_graphs;
-        });
-        if (queue.empty()) continue;  // another worker took them
-      }
-      uint64_t n = 0, m = 0, mh = 0;
-      while (!queue.empty()) {
-        Req *r = queue.front();
-        const uint64_t rn = r->q.n, rm = r->q.m;
-        const uint64_t rmh = (r->q.flags & PHEM_REQ_DEVICE) ? 0 : rm;
-        if (!admits(n + rn, m + rm, mh + rmh, (uint32_t)batch.size() + 1)) break;
-        n += rn;
-        m += rm;
-        mh += rmh;
-        batch.push_back(r);
-        queue.pop_front();
+      for (;;) {
+        cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+        if (queue.empty()) return;  // stop requested and nothing left
+        if (o.window_us && queue.size() < o.max_graphs && !stop) {
+          const auto until = Clock::now() + std::chrono::microseconds(o.window_us);
+          cv_work.wait_until(lk, until, [&] {
+            return stop || queue.size() >= o.max_graphs;
+          });
+          if (queue.empty()) continue;  // another worker took them
+        }
+        // graph requests in arrival order while they fit; a stream batch
+        // runs on its own, and only once its stream is idle and its
+        // predecessor applied (others are passed over and stay queued)
+        uint64_t n = 0, m = 0, mh = 0;
+        for (auto it = queue.begin(); it != queue.end();) {
+          Req *r = *it;
+          if (r->rs) {
+            if (batch.empty() && !r->rs->running && r->rs->applied == r->seq) {
+              r->rs->running = true;
+              batch.push_back(r);
+              queue.erase(it);
+              break;
+            }
+            ++it;
+            continue;
+          }
+          const uint64_t rn = r->q.n, rm = r->q.m;
+          const uint64_t rmh = (r->q.flags & PHEM_REQ_DEVICE) ? 0 : rm;
+          if (!admits(n + rn, m + rm, mh + rmh, (uint32_t)batch.size() + 1)) break;
+          n += rn;
+          m += rm;
+          mh += rmh;
+          batch.push_back(r);
+          it = queue.erase(it);
+        }
+        if (!batch.empty()) break;
+        // only stream batches waiting for their streams: sleep until a
+        // submission or a finished stream batch
+        cv_work.wait(lk);
       }
       in_flight += batch.size();
     }
-    if (!batch.empty()) run_batch(S, batch);
+    if (batch.size() == 1 && batch[0]->rs)
+      run_stream(S, batch[0]);
+    else if (!batch.empty())
+      run_batch(S, batch);
   }
+}
+
+// One batch of a resident stream: staged to the slot's memory if it comes
+// from host memory, applied after the stream's previous batch, then the
+// forest's size and weight (and labels) returned.
+void phem_server::run_stream(Slot &S, Req *r) {
+  Resident *rs = r->rs;  // taken only when idle with its predecessor applied
+  const double t0 = now_s();
+  const phem_request &q = r->q;
+  const uint64_t m = q.m;
+  const bool host = !(q.flags & PHEM_REQ_DEVICE);
+  // the slot's memory: labels (n words), then the staged u, v, w (m words
+  // each) of a batch from host memory (stream_fits checked both at submit)
+  uint32_t *d_lab = (uint32_t *)S.dev;
+  uint32_t *d_st = (uint32_t *)((char *)S.dev + a256(4ull * rs->n));
+  int rc = PHEM_OK;
+  const uint32_t *du = q.u, *dv = q.v, *dw = q.w;
+  if (host && m) {
+    std::vector<Req *> one{r};
+    std::vector<uint64_t> hoff{0, m};
+    rc = stage_inputs(S, one, hoff, m, d_st, d_st + m, d_st + 2 * m);
+    du = d_st;
+    dv = d_st + m;
+    dw = d_st + 2 * m;
+  }
+  phem_mst_stats st{};
+  if (rc == PHEM_OK) rc = phem_stream_apply(rs->s, du, dv, dw, m, S.stream);
+  if (rc == PHEM_OK) rc = phem_stream_stats(rs->s, &st, nullptr, nullptr);
+  if (rc == PHEM_OK && q.labels) {
+    rc = phem_stream_labels(rs->s, d_lab, S.stream);
+    if (rc == PHEM_OK) {
+      // through the pinned result buffer when it holds them
+      const bool pinned = rs->n <= o.max_batch_edges;
+      uint32_t *h_lab = pinned ? (uint32_t *)(S.hout + S.hout_bytes - a256(4 * o.max_batch_edges))
+                               : q.labels;
+      hipError_t e = hipMemcpyAsync(h_lab, d_lab, 4ull * rs->n,
+                                    hipMemcpyDeviceToHost, S.stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(S.stream);
+      if (e != hipSuccess)
+        rc = set_err(PHEM_EHIP, "label copy: %s", hipGetErrorString(e));
+      else if (pinned)
+        std::memcpy(q.labels, h_lab, 4ull * rs->n);
+    }
+  }
+  if (rc != PHEM_OK) (void)hipStreamSynchronize(S.stream);
+  const double t1 = now_s();
+  char err[sizeof(((phem_result *)nullptr)->error)] = "";
+  if (rc != PHEM_OK) {
+    std::strncpy(err, phem_last_error(), sizeof(err) - 1);
+    err[sizeof(err) - 1] = 0;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  r->res.status = rc;
+  std::memcpy(r->res.error, err, sizeof(err));
+  r->res.forest_count = rc == PHEM_OK ? (uint32_t)st.mst_count : 0;
+  r->res.weight_fx = rc == PHEM_OK ? st.weight_fx : 0;
+  r->res.weight_exact = rc == PHEM_OK && st.weight_exact;
+  r->res.batch_graphs = 1;
+  r->res.start = t0;
+  r->res.done = t1;
+  r->done = true;
+  if (rc == PHEM_OK) {
+    if (completed == 0 || r->res.arrival < first_arrival)
+      first_arrival = r->res.arrival;
+    completed++;
+    edges += m;
+    lat.push_back(t1 - r->res.arrival);
+    stream_batches++;
+    stream_edges += m;
+    stream_lat.push_back(t1 - r->res.arrival);
+    last_done = std::max(last_done, t1);
+  }
+  rs->applied++;
+  rs->running = false;
+  in_flight -= 1;
+  cv_done.notify_all();
+  cv_work.notify_all();  // the stream's next batch may run now
 }
 
 // Host edges of the batch -> pinned staging -> device, in chunks of the
@@ -549,6 +684,7 @@ extern "C" int phem_server_create(const phem_server_opts *opts,
   S->dev = dev;
   S->o = o;
   S->slot_bytes = o.mem_bytes / o.slots / 256 * 256;
+  S->stream_mem = o.stream_mem_mib ? (uint64_t)o.stream_mem_mib << 20 : o.mem_bytes;
   S->slots.resize(o.slots);
   // copy threads per slot: the box's share of host threads over the slots
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -593,6 +729,8 @@ extern "C" int phem_server_destroy(phem_server *srv) {
     if (sl.hout) (void)hipHostFree(sl.hout);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
+  for (auto &e : srv->streams) phem_stream_destroy(e.second->s);
+  srv->streams.clear();
   delete srv;
   return PHEM_OK;
 }
@@ -610,7 +748,26 @@ extern "C" int phem_server_submit(phem_server *srv, const phem_request *req,
   r->res.arrival = t;
   std::lock_guard<std::mutex> lk(srv->mu);
   if (srv->stop) return set_err(PHEM_EINVAL, "server is shutting down");
-  if (!srv->fits(req->n, req->m, mh, 1)) {
+  if (req->flags & PHEM_REQ_STREAM) {
+    auto it = srv->streams.find(req->stream_id);
+    if (it == srv->streams.end())
+      return set_err(PHEM_ENOTFOUND, "unknown stream %llu",
+                     (unsigned long long)req->stream_id);
+    Resident *rs = it->second.get();
+    if (req->forest_ids)
+      return set_err(PHEM_EINVAL, "a stream batch returns no forest ids");
+    if (req->m > rs->max_batch)
+      return set_err(PHEM_EINVAL, "batch of %llu edges exceeds the stream's max_batch %llu",
+                     (unsigned long long)req->m, (unsigned long long)rs->max_batch);
+    if (!srv->stream_fits(rs->n, mh)) {
+      srv->refused++;
+      return set_err(PHEM_EBUSY, "stream batch of %llu host edges exceeds one slot",
+                     (unsigned long long)mh);
+    }
+    r->rs = rs;
+    r->seq = rs->submitted++;
+    r->q.n = rs->n;
+  } else if (!srv->fits(req->n, req->m, mh, 1)) {
     srv->refused++;
     return set_err(PHEM_EBUSY,
                    "request of %u vertices, %llu edges exceeds one slot "
@@ -704,6 +861,20 @@ extern "C" int phem_server_get_stats(phem_server *srv, phem_server_stats *st) {
   s.recorded_batches = srv->recorded_batches;
   s.record_fallbacks = srv->record_fallbacks;
   s.recordings = srv->recordings;
+  s.stream_batches = srv->stream_batches;
+  s.stream_edges = srv->stream_edges;
+  if (!srv->stream_lat.empty()) {
+    std::vector<double> l = srv->stream_lat;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double p) {
+      size_t i = (size_t)std::ceil(p * l.size());
+      return l[i ? i - 1 : 0];
+    };
+    s.stream_lat_p50 = pct(0.50);
+    s.stream_lat_p99 = pct(0.99);
+  }
+  s.streams_open = srv->streams.size();
+  s.stream_bytes = srv->stream_used;
   *st = s;
   return PHEM_OK;
 }
@@ -716,5 +887,75 @@ extern "C" int phem_server_reset_stats(phem_server *srv) {
   srv->recordings = 0;
   srv->first_arrival = srv->last_done = 0;
   srv->lat.clear();
+  srv->stream_batches = srv->stream_edges = 0;
+  srv->stream_lat.clear();
   return PHEM_OK;
+}
+
+extern "C" int phem_server_stream_open(phem_server *srv, uint32_t n,
+                                       uint64_t max_batch,
+                                       uint64_t *stream_id) {
+  if (!srv || !stream_id) return set_err(PHEM_EINVAL, "NULL argument");
+  *stream_id = 0;
+  uint64_t bytes = 0;
+  int rc = phem_stream_bytes(n, max_batch, &bytes);
+  if (rc) return rc;
+  {
+    // admission: reserve the footprint before allocating
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->stop) return set_err(PHEM_EINVAL, "server is shutting down");
+    if (srv->stream_used + bytes > srv->stream_mem) {
+      srv->refused++;
+      return set_err(PHEM_EBUSY,
+                     "stream of %u vertices needs %llu device bytes, %llu of %llu free",
+                     n, (unsigned long long)bytes,
+                     (unsigned long long)(srv->stream_mem - srv->stream_used),
+                     (unsigned long long)srv->stream_mem);
+    }
+    srv->stream_used += bytes;
+  }
+  // the stream's memory on the server's device (the caller's device is
+  // restored afterwards)
+  std::unique_ptr<Resident> rs(new (std::nothrow) Resident());
+  int dev = 0;
+  if (!rs) {
+    rc = set_err(PHEM_ENOMEM, "host allocation failed");
+  } else if (hipGetDevice(&dev) != hipSuccess ||
+             (dev != srv->dev && hipSetDevice(srv->dev) != hipSuccess)) {
+    (void)hipGetLastError();
+    rc = set_err(PHEM_EHIP, "cannot select the server's device %d", srv->dev);
+  } else {
+    rc = phem_stream_create(n, max_batch, srv->o.wmode, &rs->s);
+    if (dev != srv->dev) (void)hipSetDevice(dev);
+  }
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (rc) {
+    srv->stream_used -= bytes;
+    return rc;
+  }
+  rs->n = n;
+  rs->max_batch = max_batch;
+  rs->bytes = bytes;
+  *stream_id = srv->next_stream++;
+  srv->streams.emplace(*stream_id, std::move(rs));
+  return PHEM_OK;
+}
+
+extern "C" int phem_server_stream_close(phem_server *srv, uint64_t stream_id) {
+  if (!srv) return set_err(PHEM_EINVAL, "NULL server");
+  std::unique_ptr<Resident> rs;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->streams.find(stream_id);
+    if (it == srv->streams.end())
+      return set_err(PHEM_ENOTFOUND, "unknown stream %llu",
+                     (unsigned long long)stream_id);
+    if (it->second->applied != it->second->submitted)
+      return set_err(PHEM_EBUSY, "stream %llu has batches queued or running",
+                     (unsigned long long)stream_id);
+    rs = std::move(it->second);
+    srv->streams.erase(it);
+    srv->stream_used -= rs->bytes;
+  }
+  return phem_stream_destroy(rs->s);
 }

@@ -103,6 +103,26 @@ static void stream_free(phem_stream *s) {
 
 using namespace phem;
 
+extern "C" int phem_stream_bytes(uint32_t n, uint64_t max_batch,
+                                 uint64_t *bytes) {
+  if (!bytes) return set_err(PHEM_EINVAL, "bytes is NULL");
+  *bytes = 0;
+  if (n < 1 || n > PHEM_MAX_VERTICES || max_batch < 1)
+    return set_err(PHEM_EINVAL, "bad n=%u / max_batch=%llu", n,
+                   (unsigned long long)max_batch);
+  const uint64_t cap = (uint64_t)n + max_batch;
+  if (cap > PHEM_MAX_EDGES)
+    return set_err(PHEM_EINVAL, "n + max_batch exceeds the edge-id range");
+  uint64_t ws = 0;
+  int rc = phem_mst_workspace_bytes(n, cap, &ws);
+  if (rc) return rc;
+  // the allocations of phem_stream_create: two [forest | batch] buffers
+  // (u, v, w, global id), the solve's workspace, forest positions, labels,
+  // parent, union-find forest and the batch's packed records
+  *bytes = 2 * 20 * cap + ws + 12ull * n + 8ull * n + 8 * max_batch;
+  return PHEM_OK;
+}
+
 extern "C" int phem_stream_create(uint32_t n, uint64_t max_batch, int wmode,
                                   phem_stream_t **out) {
   if (!out) return set_err(PHEM_EINVAL, "out is NULL");

@@ -28,6 +28,7 @@ class PhemError(RuntimeError):
 
 EINVAL, ENOMEM, EHIP, ENOCONV, ENODEV, EBUSY, ENOTFOUND = -1, -2, -3, -4, -5, -6, -7
 REQ_DEVICE = 1
+REQ_STREAM = 2
 
 
 class MstOpts(ctypes.Structure):
@@ -69,14 +70,14 @@ class ServerOpts(ctypes.Structure):
     _fields_ = [("mem_bytes", ctypes.c_uint64), ("max_batch_edges", ctypes.c_uint64),
                 ("slots", ctypes.c_uint32), ("max_graphs", ctypes.c_uint32),
                 ("window_us", ctypes.c_uint32), ("wmode", ctypes.c_int32),
-                ("record_rounds", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("record_rounds", ctypes.c_uint32), ("stream_mem_mib", ctypes.c_uint32)]
 
 
 class Request(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("m", ctypes.c_uint64),
                 ("u", ctypes.c_void_p), ("v", ctypes.c_void_p), ("w", ctypes.c_void_p),
                 ("forest_ids", ctypes.c_void_p), ("labels", ctypes.c_void_p),
-                ("arrival", ctypes.c_double)]
+                ("arrival", ctypes.c_double), ("stream_id", ctypes.c_uint64)]
 
 
 class Result(ctypes.Structure):
@@ -100,7 +101,10 @@ class ServerStats(ctypes.Structure):
                  "mean_batch_graphs")] + \
                [("slot_bytes", ctypes.c_uint64), ("slot_edges", ctypes.c_uint64),
                 ("recorded_batches", ctypes.c_uint64), ("record_fallbacks", ctypes.c_uint64),
-                ("recordings", ctypes.c_uint64)]
+                ("recordings", ctypes.c_uint64), ("stream_batches", ctypes.c_uint64),
+                ("stream_edges", ctypes.c_uint64), ("stream_lat_p50", ctypes.c_double),
+                ("stream_lat_p99", ctypes.c_double), ("streams_open", ctypes.c_uint64),
+                ("stream_bytes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -214,6 +218,9 @@ def lib():
         "phem_server_drain": (i32, [vp]),
         "phem_server_get_stats": (i32, [vp, c.POINTER(ServerStats)]),
         "phem_server_reset_stats": (i32, [vp]),
+        "phem_server_stream_open": (i32, [vp, u32, u64, c.POINTER(c.c_uint64)]),
+        "phem_server_stream_close": (i32, [vp, u64]),
+        "phem_stream_bytes": (i32, [u32, u64, c.POINTER(c.c_uint64)]),
         "phem_get_tuning": (i32, [c.POINTER(Tuning)]),
         "phem_set_tuning": (i32, [c.POINTER(Tuning)]),
         "phem_cpu_msf": (i32, [u32, u64, vp, vp, vp, i32, vp, c.POINTER(c.c_uint64), i32]),
@@ -709,9 +716,9 @@ class Server:
     are kept referenced until the request is waited for."""
 
     def __init__(self, mem_bytes=0, slots=2, max_graphs=4096, max_batch_edges=0,
-                 window_us=0, wmode=W_F32, record_rounds=0):
+                 window_us=0, wmode=W_F32, record_rounds=0, stream_mem_mib=0):
         o = ServerOpts(mem_bytes, max_batch_edges, slots, max_graphs, window_us, wmode,
-                       record_rounds, 0)
+                       record_rounds, stream_mem_mib)
         h = ctypes.c_void_p()
         _check(lib().phem_server_create(ctypes.byref(o), ctypes.byref(h)), "phem_server_create")
         self.h = h
@@ -730,11 +737,41 @@ class Server:
             ptr = _host_ptr
             m = u.size
         r = Request(n, REQ_DEVICE if dev else 0, m, ptr(u), ptr(v), ptr(w),
-                    _host_ptr(forest_ids), _host_ptr(labels), arrival)
+                    _host_ptr(forest_ids), _host_ptr(labels), arrival, 0)
         t = ctypes.c_uint64(0)
         _check(lib().phem_server_submit(self.h, ctypes.byref(r), ctypes.byref(t)),
                "phem_server_submit")
         self.keep[t.value] = (u, v, w, forest_ids, labels)
+        return t.value
+
+    def stream_open(self, n, max_batch):
+        """A resident stream of n vertices (batches of up to max_batch edges);
+        returns its id."""
+        sid = ctypes.c_uint64(0)
+        _check(lib().phem_server_stream_open(self.h, n, max_batch, ctypes.byref(sid)),
+               "phem_server_stream_open")
+        return sid.value
+
+    def stream_close(self, sid):
+        _check(lib().phem_server_stream_close(self.h, sid), "phem_server_stream_close")
+
+    def submit_batch(self, sid, u, v, w, labels=None, arrival=-1.0):
+        """A batch of edges (int32 device tensors or numpy uint32 host arrays)
+        for resident stream sid; labels: optional numpy uint32 array of the
+        stream's n slots receiving the labels after the batch."""
+        dev = isinstance(u, torch.Tensor)
+        if dev:
+            ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t.numel() else None
+            m = u.numel()
+        else:
+            ptr = _host_ptr
+            m = u.size
+        r = Request(0, (REQ_DEVICE if dev else 0) | REQ_STREAM, m, ptr(u), ptr(v), ptr(w),
+                    None, _host_ptr(labels), arrival, sid)
+        t = ctypes.c_uint64(0)
+        _check(lib().phem_server_submit(self.h, ctypes.byref(r), ctypes.byref(t)),
+               "phem_server_submit")
+        self.keep[t.value] = (u, v, w, labels)
         return t.value
 
     def wait(self, ticket):

@@ -44,6 +44,27 @@ def test_workspace_query_without_device():
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libphem.so not built")
+def test_stream_footprint_query_without_device():
+    """phem_stream_bytes (what the server admits a resident stream by) adds up
+    the stream's allocations: two [forest | batch] buffers of 20 B per slot,
+    the solve's workspace for n + max_batch edges, 24 B per vertex and 8 B
+    per batch edge; bad sizes are refused before any device call."""
+    lib = ctypes.CDLL(LIB)
+    n, mb = 1 << 20, 1 << 16
+    b, ws = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    assert lib.phem_stream_bytes(ctypes.c_uint32(n), ctypes.c_uint64(mb), ctypes.byref(b)) == 0
+    assert lib.phem_mst_workspace_bytes(ctypes.c_uint32(n), ctypes.c_uint64(n + mb),
+                                        ctypes.byref(ws)) == 0
+    assert b.value == 40 * (n + mb) + ws.value + 20 * n + 8 * mb
+    for bad in ((0, 1), (1, 0), (1, 1 << 40)):
+        assert lib.phem_stream_bytes(ctypes.c_uint32(bad[0]), ctypes.c_uint64(bad[1]),
+                                     ctypes.byref(b)) == -1
+    assert lib.phem_server_stream_open(None, ctypes.c_uint32(1), ctypes.c_uint64(1),
+                                       ctypes.byref(b)) == -1
+    assert lib.phem_server_stream_close(None, ctypes.c_uint64(1)) == -1
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libphem.so not built")
 def test_noalloc_refuses_graphs_beyond_its_scratch():
     """The size check comes before any device call: a graph whose workspace
     exceeds the 64 MiB static scratch gets PHEM_ENOMEM with a message."""

@@ -306,3 +306,87 @@ def test_recorded_batches_equal_oracle(P, rounds):
     if rounds == 2:
         assert st["record_fallbacks"] > 0
     srv.close()
+
+
+def test_resident_streams_in_the_mix_equal_oracle(P):
+    """Two resident streams fed interleaved batches (host and device inputs)
+    while graph requests are served around them: every batch's result (the
+    forest's edge count and exact weight after it) and the labels returned
+    with it equal the oracle's Kruskal forest of every edge the stream has
+    received so far; the graph requests equal the oracle too.  Then the
+    stream memory budget, the batch-size limit and the close rules."""
+    srv = P.Server(slots=3, max_batch_edges=1 << 20, wmode=P.W_F32)
+    specs = [(1 << 12, 1 << 11, 11), (3000, 700, 12)]  # (n, batch edges, seed)
+    sids = [srv.stream_open(n, mb) for n, mb, _ in specs]
+    fed = [([], [], []) for _ in specs]
+    expect = []   # (kind, ticket, payload)
+    graphs = [host_graph(g, seed=8) for g in range(24)]
+    gi = 0
+    for b in range(8):
+        for s, (n, mb, seed) in enumerate(specs):
+            if b == 0:
+                u, v, w = S.rmat(12, 4 * n, seed, positive=True) if s == 0 else \
+                    S.uniform(n, mb, S.stream_seed(seed, 99), positive=True)
+                u, v, w = u[:mb], v[:mb], w[:mb]
+            else:
+                u, v, w = S.uniform(n, mb if b % 3 else mb // 3, S.stream_seed(seed, b))
+            for k, x in enumerate((u, v, w)):
+                fed[s][k].append(x)
+            lab = np.zeros(n, np.uint32)
+            on_device = (b + s) % 2 == 1
+            if on_device:
+                t = srv.submit_batch(sids[s], dev(u), dev(v), dev(w), lab)
+            else:
+                t = srv.submit_batch(sids[s], u, v, w, lab)
+            au, av, aw = (np.concatenate(x) for x in fed[s])
+            expect.append(("stream", t, (s, n, au, av, aw, lab)))
+        for _ in range(3):
+            n, u, v, w = graphs[gi]
+            t, ids, lab = submit(srv, n, u, v, w, on_device=gi % 2 == 0)
+            expect.append(("graph", t, (n, u, v, w, ids, lab)))
+            gi += 1
+    for kind, t, pay in expect:
+        r = srv.wait(t)
+        assert r["status"] == 0, r["error"]
+        if kind == "stream":
+            s, n, au, av, aw, lab = pay
+            k_ids, k_lab = O.kruskal(n, au, av, aw, 1)
+            ws, exact = O.weight_sum(k_ids, aw, 1)
+            assert r["forest_count"] == len(k_ids), (s, len(au))
+            assert exact and r["weight_fx"] == ws and r["weight_exact"], (s, len(au))
+            assert np.array_equal(lab, k_lab), (s, len(au))
+        else:
+            n, u, v, w, ids, lab = pay
+            k_ids, k_lab = O.kruskal(n, u, v, w, 1)
+            assert r["forest_count"] == len(k_ids)
+            assert np.array_equal(ids[:len(k_ids)], k_ids) and np.array_equal(lab[:n], k_lab)
+    st = srv.stats()
+    assert st["stream_batches"] == 16 and st["streams_open"] == 2
+    assert st["stream_edges"] == sum(sum(x.size for x in f[0]) for f in fed)
+    assert st["completed"] == 16 + 24 and st["stream_lat_p99"] >= st["stream_lat_p50"] > 0
+    # a batch beyond the stream's max_batch, forest ids asked of a stream
+    u, v, w = S.uniform(3000, 701, 5)
+    with pytest.raises(P.PhemError):
+        srv.submit_batch(sids[1], u, v, w)
+    with pytest.raises(P.PhemError) as e:
+        srv.stream_close(12345)
+    assert e.value.code == P.ENOTFOUND
+    for sid in sids:
+        srv.stream_close(sid)
+    assert srv.stats()["streams_open"] == 0 and srv.stats()["stream_bytes"] == 0
+    with pytest.raises(P.PhemError) as e:
+        srv.submit_batch(sids[0], u, v, w)
+    assert e.value.code == P.ENOTFOUND
+    srv.close()
+    # admission by footprint: a 4 MiB stream budget refuses a 2^20-vertex
+    # stream (over 100 MB) and admits a 16-vertex one (1.8 MB, most of it
+    # the solve's fixed counter area)
+    srv = P.Server(slots=1, max_batch_edges=1 << 16, wmode=P.W_F32, stream_mem_mib=4)
+    with pytest.raises(P.PhemError) as e:
+        srv.stream_open(1 << 20, 1 << 10)
+    assert e.value.code == P.EBUSY
+    assert srv.stats()["refused"] == 1 and srv.stats()["streams_open"] == 0
+    small = srv.stream_open(16, 8)
+    assert srv.stats()["streams_open"] == 1
+    srv.close()  # closes the stream with the server
+    del small
