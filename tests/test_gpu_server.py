@@ -390,3 +390,40 @@ def test_resident_streams_in_the_mix_equal_oracle(P):
     assert srv.stats()["streams_open"] == 1
     srv.close()  # closes the stream with the server
     del small
+
+
+def test_stream_batches_from_many_threads_apply_in_submission_order(P):
+    """Four client threads submit batches to one resident stream at once.
+    Tickets are issued under the server's lock, so ticket order is the
+    submission order; every batch's result equals a stand-alone stream fed
+    the batches in ticket order."""
+    n, mb = 5000, 1500
+    srv = P.Server(slots=3, max_batch_edges=1 << 18, wmode=P.W_F32)
+    sid = srv.stream_open(n, mb)
+    got = []
+    lock = threading.Lock()
+
+    def client(c):
+        for b in range(10):
+            u, v, w = S.uniform(n, mb - 100 * c, S.stream_seed(40 + c, b))
+            t = srv.submit_batch(sid, u, v, w) if b % 2 else \
+                srv.submit_batch(sid, dev(u), dev(v), dev(w))
+            with lock:
+                got.append((t, (u, v, w)))
+
+    th = [threading.Thread(target=client, args=(c,)) for c in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    got.sort(key=lambda g: g[0])
+    res = [srv.wait(t) for t, _ in got]
+    ref = P.Stream(n, mb, wmode=P.W_F32)
+    for (t, (u, v, w)), r in zip(got, res):
+        ref.apply(dev(u), dev(v), dev(w))
+        s = ref.stats()
+        assert r["status"] == 0, r["error"]
+        assert (r["forest_count"], r["weight_fx"]) == (s["mst_count"], s["weight_fx"]), t
+    ref.close()
+    srv.stream_close(sid)
+    srv.close()
