@@ -74,11 +74,6 @@ __global__ __launch_bounds__(B) void k_cc_union(const ull *__restrict__ edges,
       uint64_t i = base + (uint64_t)j * B;
       e[j] = i < m ? __builtin_nontemporal_load(&edges[i]) : KEY_NONE;
     }
-    // Fast path: the parents of both endpoints of all edges are loaded
-    // together (independent loads in flight); an edge whose endpoints share a
-    // parent is inside one tree already.  A parent read here may be older than
-    // the current one, but every value it ever held is an ancestor, so equal
-    // parents still prove one tree.
     bool ok[CEPT];
 #pragma unroll
     for (int j = 0; j < CEPT; j++) {
@@ -91,6 +86,11 @@ __global__ __launch_bounds__(B) void k_cc_union(const ull *__restrict__ edges,
         ok[j] = !(giant_test(a, s_sum, covered, giant) &&
                   giant_test(b, s_sum, covered, giant));
     }
+    // Fast path: the parents of both endpoints of all edges are loaded
+    // together (independent loads in flight); an edge whose endpoints share a
+    // parent is inside one tree already.  A parent read here may be older than
+    // the current one, but every value it ever held is an ancestor, so equal
+    // parents still prove one tree.
     uint32_t pa[CEPT], pb[CEPT];
 #pragma unroll
     for (int j = 0; j < CEPT; j++) {
