@@ -246,9 +246,9 @@ __device__ __forceinline__ void min_store_ends(ull *best, const uint32_t (&a)[N]
 // Reads are ordinary cached loads and may observe an older value of an entry
 // (another XCD's L2 is not kept coherent within a kernel).  Every value an
 // entry ever holds is an ancestor of the entry, so an older value still lies
-// on a path to the true root; only the compare-and-swap on a root, performed
-// at device scope, is authoritative, and a failed one returns the current
-// parent, which is strictly smaller.  Path-halving stores only write an
+// on a path to the true root; only device-scope accesses are authoritative:
+// the compare-and-swap on a root (a failed one returns the current parent,
+// which is strictly smaller) and the device-scope load that may precede it.  Path-halving stores only write an
 // ancestor into a non-root entry, which no compare-and-swap ever targets.
 
 __device__ __forceinline__ uint32_t par_load(const uint32_t *p) {
@@ -298,13 +298,32 @@ __device__ __forceinline__ uint32_t uf_root(const uint32_t *par, uint32_t x) {
 // hooked under the smaller by compare-and-swap (a failed swap means the root
 // was hooked meanwhile; the walk goes on from its new root).  Returns the
 // hooked root, or U32_NONE if the trees were already one.
+//
+// CHECK: read the entry at device scope first and swap only if it still
+// holds the root.  A root that was hooked long ago can still read as a root
+// from another XCD's L2, and every union that walks through it then fails a
+// swap on that one entry, which the memory side performs one at a time.
+// While a large component grows, its root changes many times (each merge
+// with a component holding a smaller id), so the unions of a fresh parent
+// array pile their failed swaps onto a few old roots: with random vertex ids
+// (RMAT n = 2^24, first 2^24 records) 0.3-0.5 M failed swaps, 80 % of them
+// on ten entries, and 2.1-6.1 ms instead of 0.84-0.91 ms with the check.
+// Where the large component's root is fixed from the start (it has the
+// smallest id) or the unions are few, the check costs about as much as it
+// saves, so only the unions over a fresh parent array use it.
+template <bool CHECK = false>
 __device__ __forceinline__ uint32_t uf_hook_roots(uint32_t *par, uint32_t a,
                                                   uint32_t b) {
   while (a != b) {
     uint32_t hi = a > b ? a : b;
     uint32_t lo = a ^ b ^ hi;
-    uint32_t old = atomicCAS(&par[hi], hi, lo);
-    if (old == hi) return hi;
+    uint32_t old = hi;
+    if (CHECK)
+      old = __hip_atomic_load(&par[hi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == hi) {
+      old = atomicCAS(&par[hi], hi, lo);
+      if (old == hi) return hi;
+    }
     // hi was hooked meanwhile; `old` is its current parent
     uint32_t r = uf_find(par, old);
     if (hi == a) a = r; else b = r;
@@ -313,9 +332,10 @@ __device__ __forceinline__ uint32_t uf_hook_roots(uint32_t *par, uint32_t a,
 }
 
 // Union of the trees of a and b (uf_hook_roots of their roots).
+template <bool CHECK = false>
 __device__ __forceinline__ uint32_t uf_union(uint32_t *par, uint32_t a,
                                              uint32_t b) {
-  return uf_hook_roots(par, uf_find(par, a), uf_find(par, b));
+  return uf_hook_roots<CHECK>(par, uf_find(par, a), uf_find(par, b));
 }
 
 // uf_union of a and b whose parents pa = par[a], pb = par[b] are already
