@@ -135,6 +135,38 @@ def test_cc_filtered_shapes_rmat(P, fblock):
         P.set_tuning(**old)
 
 
+@pytest.mark.parametrize("seed", [9, 10])
+def test_cc_rmat_permuted_ids(P, seed):
+    """RMAT n = 2^22, m = 2^26 with its vertex ids permuted: the giant
+    component's root (its smallest id) changes many times while the sample
+    phase builds it, the case the device-scope root check in the unions is
+    for.  Labels, spanning forest and the merge of two half forests equal the
+    oracle's."""
+    scale, m = 22, 1 << 26
+    n = 1 << scale
+    du, dv, _ = P.gen_rmat(scale, m, 3, positive=True)
+    pm = torch.from_numpy(np.random.default_rng(seed).permutation(n).astype(np.int32)).cuda()
+    du, dv = pm[du.long()], pm[dv.long()]
+    e = P.pack_edges(torch.minimum(du, dv), torch.maximum(du, dv))
+    u, v = du.cpu().numpy().view(np.uint32), dv.cpu().numpy().view(np.uint32)
+    want = O.par_cc(n, u, v)
+    lab, _, f = P.cc_solve(n, e)
+    torch.cuda.synchronize()
+    assert np.array_equal(lab.cpu().numpy().view(np.uint32), want)
+    assert_spanning_forest(n, f.cpu().numpy().view(np.uint64), want)
+    # the forests of the two halves of the records, merged
+    h = m // 2
+    _, _, f1 = P.cc_solve(n, e[:h].contiguous())
+    _, _, f2 = P.cc_solve(n, e[h:].contiguous())
+    par = torch.empty(n, dtype=torch.int32, device="cuda")
+    lab3 = torch.empty(n, dtype=torch.int32, device="cuda")
+    f3 = torch.empty(n, dtype=torch.int64, device="cuda")
+    P.cc_merge_forests(n, f1, f2, par, f3, lab3)
+    torch.cuda.synchronize()
+    assert np.array_equal(lab3.cpu().numpy().view(np.uint32), want)
+    assert_spanning_forest(n, f3.cpu().numpy().view(np.uint64), want)
+
+
 @pytest.mark.slow
 def test_cc_rmat_full_size(P):
     scale, m = 24, 1 << 28

@@ -188,6 +188,39 @@ def test_mst_rmat_full_size_vs_parallel_oracle(P):
     assert exact and st["weight_fx"] == ws
 
 
+@pytest.mark.slow
+def test_mst_cc_rmat_permuted_ids_full_size(P):
+    """The benchmark's permuted graph: RMAT n=2^24, m=2^28 with its vertex ids
+    permuted on the device (seed 9), so the hubs and the giant component are
+    spread over the id range.  The forest, exact weight and labels equal the
+    oracle's; the CC solve's labels (its unions then build the giant
+    component under roots that change many times) equal them too."""
+    scale, m = 24, 1 << 28
+    n = 1 << scale
+    print("generating rmat 2^24 / 2^28, ids permuted", flush=True)
+    du, dv, dw = P.gen_rmat(scale, m, 2, positive=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(9)
+    perm = torch.randperm(n, device="cuda", generator=g, dtype=torch.int64).to(torch.int32)
+    du, dv = perm[du.long()], perm[dv.long()]
+    del perm
+    ids, lab, st = P.mst_solve(n, du, dv, dw, P.W_F32)
+    e = P.pack_edges(torch.minimum(du, dv), torch.maximum(du, dv))
+    clab, _, _ = P.cc_solve(n, e, forest=False)
+    torch.cuda.synchronize()
+    d_ids, d_lab, c_lab = host(ids).copy(), host(lab).copy(), host(clab).copy()
+    u, v, w = host(du).copy(), host(dv).copy(), host(dw).copy()
+    del e, du, dv, dw, ids, lab, clab
+    print(f"device forest {len(d_ids)} edges, rounds {st['rounds']}", flush=True)
+    p_ids = O.par_mst(n, u, v, w, 1)
+    assert np.array_equal(d_ids, p_ids)
+    ws, exact = O.weight_sum(p_ids, w, 1)
+    assert exact and st["weight_fx"] == ws
+    want = O.par_cc(n, u, v)
+    assert np.array_equal(d_lab, want)
+    assert np.array_equal(c_lab, want)
+
+
 @pytest.mark.parametrize("name", ["rmat_small", "grid_small", "out_of_range", "u32_weights"])
 def test_mst_aos_layout_matches_soa(P, name):
     n, u, v, w, wmode = cases.make(name)
