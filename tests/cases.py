@@ -62,6 +62,16 @@ def _star():
     return n, u, v, f32(rng.random(n - 1, dtype=np.float32)), 1
 
 
+def _star_hub_last():
+    # the hub has the largest id: each leaf smaller than the hub's current
+    # root hooks that root, which changes many times as the leaves join
+    n = 1 << 16
+    u = np.arange(n - 1, dtype=np.uint32)
+    v = np.full(n - 1, n - 1, np.uint32)
+    rng = np.random.default_rng(5)
+    return n, u, v, f32(rng.random(n - 1, dtype=np.float32)), 1
+
+
 def _u32_weights():
     rng = np.random.default_rng(11)
     n, m = 3000, 40000
@@ -121,6 +131,7 @@ CASES = {
     "equal_weight_path": _equal_weight_path,
     "reverse_path": _reverse_path,
     "star": _star,
+    "star_hub_last": _star_hub_last,
     "u32_weights": _u32_weights,
     "signed_floats": _signed_floats,
     "many_components": _many_components,
