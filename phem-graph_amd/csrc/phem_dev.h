@@ -207,7 +207,16 @@ __device__ __forceinline__ void min_store(ull *best, uint32_t c, uint64_t key) {
 // pre-check reads only the high word of best[c]: best only decreases, so a
 // stale high word is an upper bound of the current one and key >> 32 <= it
 // whenever key < best[c].  best must have at least one entry.
-template <int N>
+//
+// TIES: a key whose high word (the weight) equals the one read is sent to
+// the atomic only after the whole entry is read again and found larger, a
+// read issued only in waves where some lane has such a tie.  Otherwise every
+// offer of an entry's own weight reaches it: with tied weights (a grid's
+// integer weights 1..16) a component's many offers of its lightest weight
+// queue on its one entry at the memory side.  The heavy pass over the
+// 4096^2 grid took 1.84 ms that way and takes 0.48 ms with the check; with
+// float weights the check is not measurable (RMAT n = 2^24, m = 2^28).
+template <int N, bool TIES = false>
 __device__ __forceinline__ void min_store_n(ull *best, const uint32_t (&c)[N],
                                             const ull (&key)[N],
                                             const bool (&on)[N]) {
@@ -215,12 +224,17 @@ __device__ __forceinline__ void min_store_n(ull *best, const uint32_t (&c)[N],
 #pragma unroll
   for (int j = 0; j < N; j++) hi[j] = (uint32_t)(best[on[j] ? c[j] : 0u] >> 32);
 #pragma unroll
-  for (int j = 0; j < N; j++)
-    if (on[j] && (uint32_t)(key[j] >> 32) <= hi[j]) atomicMin(&best[c[j]], key[j]);
+  for (int j = 0; j < N; j++) {
+    const uint32_t kh = (uint32_t)(key[j] >> 32);
+    if (on[j] && (TIES ? kh < hi[j] : kh <= hi[j])) atomicMin(&best[c[j]], key[j]);
+    if (TIES && __ballot(on[j] && kh == hi[j]))
+      if (on[j] && kh == hi[j] && key[j] < best[c[j]]) atomicMin(&best[c[j]], key[j]);
+  }
 }
 
-// min_store_n to both endpoints a[j] and b[j] of every edge j with on[j].
-template <int N>
+// min_store_n to both endpoints a[j] and b[j] of every edge j with on[j]
+// (TIES as there).
+template <int N, bool TIES = false>
 __device__ __forceinline__ void min_store_ends(ull *best, const uint32_t (&a)[N],
                                                const uint32_t (&b)[N],
                                                const ull (&key)[N],
@@ -233,10 +247,15 @@ __device__ __forceinline__ void min_store_ends(ull *best, const uint32_t (&a)[N]
   }
 #pragma unroll
   for (int j = 0; j < N; j++) {
-    if (!on[j]) continue;
     const uint32_t kh = (uint32_t)(key[j] >> 32);
-    if (kh <= ha[j]) atomicMin(&best[a[j]], key[j]);
-    if (kh <= hb[j]) atomicMin(&best[b[j]], key[j]);
+    if (on[j]) {
+      if (TIES ? kh < ha[j] : kh <= ha[j]) atomicMin(&best[a[j]], key[j]);
+      if (TIES ? kh < hb[j] : kh <= hb[j]) atomicMin(&best[b[j]], key[j]);
+    }
+    if (TIES && __ballot(on[j] && (kh == ha[j] || kh == hb[j]))) {
+      if (on[j] && kh == ha[j] && key[j] < best[a[j]]) atomicMin(&best[a[j]], key[j]);
+      if (on[j] && kh == hb[j] && key[j] < best[b[j]]) atomicMin(&best[b[j]], key[j]);
+    }
   }
 }
 
@@ -248,8 +267,9 @@ __device__ __forceinline__ void min_store_ends(ull *best, const uint32_t (&a)[N]
 // entry ever holds is an ancestor of the entry, so an older value still lies
 // on a path to the true root; only device-scope accesses are authoritative:
 // the compare-and-swap on a root (a failed one returns the current parent,
-// which is strictly smaller) and the device-scope load that may precede it.  Path-halving stores only write an
-// ancestor into a non-root entry, which no compare-and-swap ever targets.
+// which is strictly smaller) and the device-scope load that may precede it.
+// Path-halving stores only write an ancestor into a non-root entry, which no
+// compare-and-swap ever targets.
 
 __device__ __forceinline__ uint32_t par_load(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

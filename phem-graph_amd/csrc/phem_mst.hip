@@ -588,7 +588,7 @@ __global__ __launch_bounds__(B_) void k_pass(
         x[j] = ga[j] ? b[j] : a[j];
         val[j] = val[j] && !(ga[j] || gb[j]);
       }
-      min_store_n<NE>(gmin, x, key, one);
+      min_store_n<NE, true>(gmin, x, key, one);
     }
     if (MODE == P_EDGES) {
       // alive edges of late rounds name a few hot components; one edge's
@@ -623,7 +623,9 @@ __global__ __launch_bounds__(B_) void k_pass(
     if (MODE == P_LIGHT || MODE == P_HEAVY) {
       // input edges spread their offers over many components: the
       // pre-check reads of the tile are issued together
-      min_store_ends<NE>(best, a, b, key, keep);
+      // (tie-checked in the heavy pass, whose survivors join few large
+      // components; the light pass measured no gain from it)
+      min_store_ends<NE, MODE == P_HEAVY>(best, a, b, key, keep);
     } else {
       // relabelled edges concentrate on few components in late rounds: each
       // pre-check follows the thread's previous atomic, so a component's
