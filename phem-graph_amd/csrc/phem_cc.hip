@@ -45,28 +45,43 @@ __global__ __launch_bounds__(CBLK) void k_cc_init(uint32_t n, uint32_t *par,
   }
 }
 
-// CHECK: device-scope read of a root before its compare-and-swap (uf_hook_roots)
+// CHECK (and check at run time): device-scope read of a root before its
+// compare-and-swap (uf_hook_roots)
 template <bool CHECK>
 __device__ __forceinline__ void cc_edge(ull e, uint32_t n, uint32_t *par,
-                                        ull *forest) {
+                                        ull *forest, bool check = true) {
   const uint32_t a = (uint32_t)(e >> 32), b = (uint32_t)e;
   if (a < b && b < n) {
-    uint32_t h = uf_union<CHECK>(par, a, b);
+    uint32_t h = uf_union<CHECK>(par, a, b, check);
     if (forest && h != U32_NONE) forest[h] = e;
   }
 }
 
 // FILTER: edges whose endpoints are both in the giant component of the
 // sampling phase (bitmap `giant`) are skipped without touching par.
+// gcount: how many of the GSAMP sampled vertices the giant holds.
 template <bool FILTER, int B, int CEPT>
 __global__ __launch_bounds__(B) void k_cc_union(const ull *__restrict__ edges,
                                                 uint64_t m, uint32_t n,
                                                 uint32_t *par, ull *forest,
                                                 const uint32_t *__restrict__ giant,
-                                                const uint32_t *gsum) {
+                                                const uint32_t *gsum,
+                                                const uint32_t *gcount) {
   __shared__ uint32_t s_sum[FILTER ? gsum_cc_words<B>() : 1];
   uint32_t covered = 0;
   if (FILTER) covered = gsum_load<B, gsum_cc_words<B>()>(s_sum, gsum, n);
+  // Checked hooks (uf_hook_roots) where many records become unions: the
+  // unfiltered union, which builds trees from single vertices in a CC
+  // solve's sample phase (0.70 -> 0.66 ms at RMAT n = 2^24, m = 2^28;
+  // 2.1-6.1 -> 0.84-0.91 ms with the vertex ids permuted; streaming
+  // batches on a resident parent array unchanged), and filtered passes whose
+  // giant holds under 7/8 of the sampled vertices, so that by 1 - (7/8)^2
+  // about a quarter or more of the records (with even degrees) are not
+  // skipped: the 4096^2 grid's pass keeps all its 10 M records (1.36 ->
+  // 0.79 ms), a uniform graph's first pass a third.  Behind a giant holding
+  // nearly all of the sample (RMAT: 4-9 % of the records kept) the hooks are
+  // few and the check measured 0.09 ms slower.
+  const bool check = !FILTER || *gcount < GSAMP * 7 / 8;
   constexpr int CTILE = B * CEPT;
   for (uint64_t tile = blockIdx.x; tile * CTILE < m; tile += gridDim.x) {
     const uint64_t base = tile * CTILE + threadIdx.x;
@@ -102,13 +117,7 @@ __global__ __launch_bounds__(B) void k_cc_union(const ull *__restrict__ edges,
     }
 #pragma unroll
     for (int j = 0; j < CEPT; j++)
-      // checked hooks in the unfiltered union, which builds trees from single
-      // vertices in a CC solve's sample phase (0.70 -> 0.66 ms at RMAT n =
-      // 2^24, m = 2^28; 2.1-6.1 -> 0.84-0.91 ms with the vertex ids
-      // permuted; streaming batches on a resident parent array unchanged);
-      // the filtered passes find the large roots settled, and the check
-      // measured 0.09 ms slower there
-      if (pa[j] != pb[j]) cc_edge<!FILTER>(e[j], n, par, forest);
+      if (pa[j] != pb[j]) cc_edge<true>(e[j], n, par, forest, check);
   }
 }
 
@@ -142,7 +151,8 @@ namespace phem {
 template <bool FILTER>
 static void launch_cc_union(const ull *edges, uint64_t m, uint32_t n,
                             uint32_t *par, ull *forest, const uint32_t *giant,
-                            const uint32_t *gsum, hipStream_t s) {
+                            const uint32_t *gsum, const uint32_t *gcount,
+                            hipStream_t s) {
   const KernelShape k = kernel_shape();
   const uint32_t B = FILTER ? k.cc_block_filter : k.cc_block;
   const uint32_t E = FILTER ? k.cc_ept_filter : k.cc_ept;
@@ -150,7 +160,7 @@ static void launch_cc_union(const ull *edges, uint64_t m, uint32_t n,
 #define PHEM_CC_UNION(BB, EE)                                                  \
   if (B == BB && E == EE) {                                                    \
     hipLaunchKernelGGL((k_cc_union<FILTER, BB, EE>), dim3(grid), dim3(BB), 0, \
-                       s, edges, m, n, par, forest, giant, gsum);              \
+                       s, edges, m, n, par, forest, giant, gsum, gcount);      \
     return;                                                                    \
   }
   if constexpr (FILTER) {
@@ -165,7 +175,7 @@ static void launch_cc_union(const ull *edges, uint64_t m, uint32_t n,
   // phem_set_tuning admits no other shape; 256 threads as a backstop
   hipLaunchKernelGGL((k_cc_union<FILTER, 256, FILTER ? 8 : 16>),
                      dim3(grid_for(m, 256ull * (FILTER ? 8 : 16))), dim3(256),
-                     0, s, edges, m, n, par, forest, giant, gsum);
+                     0, s, edges, m, n, par, forest, giant, gsum, gcount);
 }
 
 }  // namespace phem
@@ -196,7 +206,7 @@ extern "C" int phem_cc_union(uint32_t n, const uint64_t *d_edges, uint64_t m,
   {
     PHEM_PROF("k_cc_union", (hipStream_t)stream);
     launch_cc_union<false>((const ull *)d_edges, m, n, d_parent, (ull *)d_forest,
-                           nullptr, nullptr, (hipStream_t)stream);
+                           nullptr, nullptr, nullptr, (hipStream_t)stream);
   }
   PHEM_LAUNCHED("k_cc_union");
   return PHEM_OK;
@@ -236,8 +246,9 @@ extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
     return d_labels ? phem_cc_labels(n, d_parent, d_labels, stream) : PHEM_OK;
   }
   hipStream_t s = (hipStream_t)stream;
-  uint32_t *giant_root = d_labels;  // 1 word
-  uint32_t *bits = d_labels + 2;    // ceil(n / 32) words
+  uint32_t *giant_root = d_labels;       // 1 word
+  uint32_t *giant_count = d_labels + 1;  // 1 word
+  uint32_t *bits = d_labels + 2;         // ceil(n / 32) words
   // summary: gsum_words <= ceil(n / 1024) words; all fit in n for n >= 64
   uint32_t *gsum = bits + (n + 31) / 32;
   rc = phem_cc_union(n, d_edges, m1, d_parent, d_forest, stream);
@@ -253,7 +264,7 @@ extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
     {
       PHEM_PROF("k_pick_giant", s);
       hipLaunchKernelGGL(k_pick_giant, dim3(1), dim3(SBLK), 0, s, n,
-                         (const uint32_t *)d_parent, giant_root);
+                         (const uint32_t *)d_parent, giant_root, giant_count);
     }
     PHEM_LAUNCHED("k_pick_giant");
     {
@@ -278,7 +289,7 @@ extern "C" int phem_cc_solve(uint32_t n, const uint64_t *d_edges, uint64_t m,
     {
       PHEM_PROF("k_cc_union_filtered", s);
       launch_cc_union<true>((const ull *)d_edges + a0, a1 - a0, n, d_parent,
-                            (ull *)d_forest, bits, gsum, s);
+                            (ull *)d_forest, bits, gsum, giant_count, s);
     }
     PHEM_LAUNCHED("k_cc_union_filtered");
     return PHEM_OK;

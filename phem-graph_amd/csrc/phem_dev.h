@@ -330,15 +330,17 @@ __device__ __forceinline__ uint32_t uf_root(const uint32_t *par, uint32_t x) {
 // on ten entries, and 2.1-6.1 ms instead of 0.84-0.91 ms with the check.
 // Where the large component's root is fixed from the start (it has the
 // smallest id) or the unions are few, the check costs about as much as it
-// saves, so only the unions over a fresh parent array use it.
+// saves, so it is used where most records are unions: over a fresh parent
+// array, and in CC passes that skip few records.  With CHECK, `check` can
+// still turn it off at run time.
 template <bool CHECK = false>
 __device__ __forceinline__ uint32_t uf_hook_roots(uint32_t *par, uint32_t a,
-                                                  uint32_t b) {
+                                                  uint32_t b, bool check = true) {
   while (a != b) {
     uint32_t hi = a > b ? a : b;
     uint32_t lo = a ^ b ^ hi;
     uint32_t old = hi;
-    if (CHECK)
+    if (CHECK && check)
       old = __hip_atomic_load(&par[hi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (old == hi) {
       old = atomicCAS(&par[hi], hi, lo);
@@ -354,8 +356,8 @@ __device__ __forceinline__ uint32_t uf_hook_roots(uint32_t *par, uint32_t a,
 // Union of the trees of a and b (uf_hook_roots of their roots).
 template <bool CHECK = false>
 __device__ __forceinline__ uint32_t uf_union(uint32_t *par, uint32_t a,
-                                             uint32_t b) {
-  return uf_hook_roots<CHECK>(par, uf_find(par, a), uf_find(par, b));
+                                             uint32_t b, bool check = true) {
+  return uf_hook_roots<CHECK>(par, uf_find(par, a), uf_find(par, b), check);
 }
 
 // uf_union of a and b whose parents pa = par[a], pb = par[b] are already

@@ -1777,7 +1777,7 @@ static int mst_solve_impl(uint32_t n, uint64_t m, Src src, int wmode,
       {
         PHEM_PROF("k_pick_giant", s);
         hipLaunchKernelGGL(k_pick_giant, dim3(1), dim3(SBLK), 0, s, n, par,
-                           (uint32_t *)(ctr + C_GIANT));
+                           (uint32_t *)(ctr + C_GIANT), (uint32_t *)nullptr);
       }
       PHEM_LAUNCHED("k_pick_giant");
       {

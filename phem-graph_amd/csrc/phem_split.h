@@ -47,9 +47,9 @@ __global__ __launch_bounds__(SPLIT_BLK) void k_flatten(uint32_t n, uint32_t *par
 // The most frequent value of the sorted s[0..N) (the smallest on ties),
 // ignoring the padding value ~0 (~0 when s holds padding only); every run
 // start finds its run's end by binary search.  All NT threads call; thread 0
-// receives the result.
+// receives the result, and its count in *len when len is not NULL.
 template <int N, int NT>
-__device__ __forceinline__ ull longest_run(const ull *s) {
+__device__ __forceinline__ ull longest_run(const ull *s, ull *len = nullptr) {
   ull best_len = 0, best_val = ~0ull;
   for (int j = threadIdx.x; j < N; j += NT) {
     if (s[j] == ~0ull || (j > 0 && s[j] == s[j - 1])) continue;
@@ -84,22 +84,29 @@ __device__ __forceinline__ ull longest_run(const ull *s) {
         bl = s_len[w];
         bv = s_val[w];
       }
+    if (len) *len = bl;
   }
   __syncthreads();
   return bv;
 }
 
-// The most frequent root among GSAMP strided vertices (after k_flatten).
+// The most frequent root among GSAMP strided vertices (after k_flatten);
+// with count_out, also how many of the GSAMP vertices have it.
 __global__ __launch_bounds__(SBLK) void k_pick_giant(uint32_t n,
                                                      const uint32_t *par,
-                                                     uint32_t *giant_out) {
+                                                     uint32_t *giant_out,
+                                                     uint32_t *count_out) {
   __shared__ ull s[GSAMP];
   for (int j = threadIdx.x; j < GSAMP; j += SBLK)
     s[j] = par[(uint64_t)j * n / GSAMP];
   __syncthreads();
   bitonic_sort_lds<GSAMP>(s);
-  const ull r = longest_run<GSAMP, SBLK>(s);
-  if (threadIdx.x == 0) *giant_out = (uint32_t)r;
+  ull len = 0;
+  const ull r = longest_run<GSAMP, SBLK>(s, &len);
+  if (threadIdx.x == 0) {
+    *giant_out = (uint32_t)r;
+    if (count_out) *count_out = (uint32_t)len;
+  }
 }
 
 // bit x of `bits` = (par[x] == giant root); one ballot per 64 vertices.
