@@ -213,9 +213,10 @@ __device__ __forceinline__ void min_store(ull *best, uint32_t c, uint64_t key) {
 // read issued only in waves where some lane has such a tie.  Otherwise every
 // offer of an entry's own weight reaches it: with tied weights (a grid's
 // integer weights 1..16) a component's many offers of its lightest weight
-// queue on its one entry at the memory side.  The heavy pass over the
-// 4096^2 grid took 1.84 ms that way and takes 0.48 ms with the check; with
-// float weights the check is not measurable (RMAT n = 2^24, m = 2^28).
+// queue on its one entry at the memory side, worst on hubs.  The heavy pass
+// over the 4096^2 grid took 1.84 ms that way and takes 0.48 ms with the
+// check; the light pass over RMAT n = 2^24, m = 2^28 with integer weights
+// 1..16, 5.26 and 1.11 ms; with float weights the check is not measurable.
 template <int N, bool TIES = false>
 __device__ __forceinline__ void min_store_n(ull *best, const uint32_t (&c)[N],
                                             const ull (&key)[N],

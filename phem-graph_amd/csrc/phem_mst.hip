@@ -623,9 +623,10 @@ __global__ __launch_bounds__(B_) void k_pass(
     if (MODE == P_LIGHT || MODE == P_HEAVY) {
       // input edges spread their offers over many components: the
       // pre-check reads of the tile are issued together
-      // (tie-checked in the heavy pass, whose survivors join few large
-      // components; the light pass measured no gain from it)
-      min_store_ends<NE, MODE == P_HEAVY>(best, a, b, key, keep);
+      // (tie-checked: RMAT 2^24 / 2^28 with integer weights 1..16 took 5.26
+      // ms in the light pass without, 1.11 ms with; float weights 1.08 ms
+      // and 1.06 ms)
+      min_store_ends<NE, true>(best, a, b, key, keep);
     } else {
       // relabelled edges concentrate on few components in late rounds: each
       // pre-check follows the thread's previous atomic, so a component's
