@@ -221,6 +221,34 @@ def test_mst_cc_rmat_permuted_ids_full_size(P):
     assert np.array_equal(c_lab, want)
 
 
+@pytest.mark.parametrize("wkind", ["int16_f32", "int16_u32", "all_equal"])
+def test_mst_rmat_tied_weights(P, wkind):
+    """RMAT n = 2^18, m = 2^22 whose weights tie: integers 1..16 (as float32
+    and as uint32) and one weight for every edge.  Most offers to a hub's
+    entry then carry its own weight, so the split schedule's tie-checked
+    offers decide by edge id; forest, labels and exact weight equal the
+    oracle's in both schedules."""
+    n = 1 << 18
+    du, dv, dw = P.gen_rmat(18, 1 << 22, 11, positive=True)
+    u, v = host(du).copy(), host(dv).copy()
+    r = np.random.default_rng(11).integers(1, 17, len(u))
+    if wkind == "int16_f32":
+        w, wmode = r.astype(np.float32).view(np.uint32), 1
+    elif wkind == "int16_u32":
+        w, wmode = r.astype(np.uint32), 0
+    else:
+        w, wmode = np.full(len(u), np.float32(0.5)).view(np.uint32), 1
+    k_ids, k_lab = O.kruskal(n, u, v, w, wmode)
+    ws, exact = O.weight_sum(k_ids, w, wmode)
+    for lf in (None, 0.0):
+        ids, lab, st = run_mst(P, n, u, v, w, wmode, light_factor=lf)
+        assert np.array_equal(ids, k_ids), lf
+        assert np.array_equal(lab, k_lab), lf
+        assert exact and st["weight_fx"] == ws, lf
+        if lf is None:
+            assert st["tau"] != 0xFFFFFFFFFFFFFFFF  # the split schedule ran
+
+
 @pytest.mark.parametrize("name", ["rmat_small", "grid_small", "out_of_range", "u32_weights"])
 def test_mst_aos_layout_matches_soa(P, name):
     n, u, v, w, wmode = cases.make(name)
