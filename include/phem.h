@@ -72,8 +72,8 @@ typedef struct {
                               (default 1.0; <= 0: one phase; NaN: unchanged) */
   /* kernel shapes; 0 (NaN for mst_split_ratio) in phem_set_tuning: unchanged */
   uint32_t mst_block;       /* threads per block of the MST input passes
-                               (light and heavy filter): 256 (default), 512 */
-  uint32_t mst_ept;         /* their edges per thread: 4, 8 (default), 16;
+                               (light and heavy filter): 256, 512 (default) */
+  uint32_t mst_ept;         /* their edges per thread: 4 (default), 8, 16;
                                mst_block x mst_ept <= 4096; graphs of fewer
                                than 2^22 edges use at most 2048 edges per
                                tile (per thread: 2048 / mst_block) */

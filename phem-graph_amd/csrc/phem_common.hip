@@ -26,8 +26,11 @@ std::once_flag g_tune_once;
 // kernel shapes (built-in defaults: the shapes measured fastest at RMAT
 // n = 2^24, m = 2^28; the filtered CC unions with 1024-thread blocks and a
 // 64 KB LDS summary took 1.79 ms against 1.97 ms with 512 and 2.09 ms with
-// 256 threads)
-std::atomic<uint32_t> g_mst_block{256}, g_mst_ept{8}, g_mst_ept_compact{16},
+// 256 threads; the MST input passes in 512-thread blocks with 4 edges per
+// thread, 7 waves per SIMD, took the step from 8.71 to 8.67 ms against 256 x
+// 8 at 4 waves, mean of 3, and the served mix from 15.3-15.6 to 15.9-17.6 G
+// edges/s)
+std::atomic<uint32_t> g_mst_block{512}, g_mst_ept{4}, g_mst_ept_compact{16},
     g_mst_compact_log2{22}, g_cc_block{256}, g_cc_ept{16}, g_cc_ept_filter{8},
     g_cc_block_filter{1024};
 std::atomic<double> g_mst_split_ratio{1.5};

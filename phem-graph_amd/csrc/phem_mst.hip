@@ -69,7 +69,7 @@ constexpr int BLK = 256;       // threads per block (4 waves)
 constexpr int EPT = 8;         // edges per thread per tile (input passes)
 constexpr int TILE = BLK * EPT;
 // The input passes (light and heavy filter) run with mst_block threads and
-// mst_ept edges per thread (256 x 8 built in); the compaction rounds of
+// mst_ept edges per thread (512 x 4 built in); the compaction rounds of
 // graphs with m >= 2^mst_compact_log2 with mst_ept_compact edges per thread
 // (16 built in), smaller graphs with EPT.  Segment capacities are sized for
 // the largest tile any shape can use on the graph: 4096 edges for graphs of

@@ -75,9 +75,9 @@ def test_mst_input_pass_shapes(P, block, ept):
         for name in CASES + ["big"]:
             for lf in (0.25, 1.0, 0.0):
                 check_mst(P, name, lf)
-    P.set_tuning(mst_block=256, mst_ept=8, mst_ept_compact=16)
+    P.set_tuning(mst_block=512, mst_ept=4, mst_ept_compact=16)
     with pytest.raises(P.PhemError):  # 512 x 16 is beyond the largest compiled tile
-        P.set_tuning(mst_block=512, mst_ept=16)
+        P.set_tuning(mst_ept=16)
 
 
 @pytest.mark.parametrize("ratio", [1.25, 3.0, 40.0])
@@ -113,7 +113,7 @@ def test_batch_with_shapes(P, block, ept):
             assert np.array_equal(r["labels"][g].numpy().view(np.uint32), k_lab), g
             assert r["weight_fx"][g] == O.weight_sum(k_ids, w, 1)[0], g
     finally:
-        P.set_tuning(mst_block=256, mst_ept=8)
+        P.set_tuning(mst_block=512, mst_ept=4)
 
 
 @pytest.mark.parametrize("block,fblock", [(256, 256), (512, 512), (256, 1024)])

@@ -18,14 +18,14 @@ def test_set_get_tuning_round_trip():
         assert {k: t[k] for k in ("mst_block", "mst_ept", "mst_ept_compact",
                                   "mst_compact_log2", "mst_split_ratio", "cc_block",
                                   "cc_ept", "cc_ept_filter", "cc_block_filter")} == {
-            "mst_block": 256, "mst_ept": 8, "mst_ept_compact": 16, "mst_compact_log2": 22,
+            "mst_block": 512, "mst_ept": 4, "mst_ept_compact": 16, "mst_compact_log2": 22,
             "mst_split_ratio": 1.5, "cc_block": 256, "cc_ept": 16, "cc_ept_filter": 8,
             "cc_block_filter": 1024}
-        P.set_tuning(mst_light_factor=2.0, mst_block=512, cc_ept=32,
+        P.set_tuning(mst_light_factor=2.0, mst_block=256, cc_ept=32,
                      cpu_share=0.25)  # the others unchanged
         t = P.get_tuning()
         assert t["blocks_per_cu"] == 8 and t["mst_light_factor"] == 2.0
-        assert (t["mst_block"], t["mst_ept"], t["cc_ept"]) == (512, 8, 32)
+        assert (t["mst_block"], t["mst_ept"], t["cc_ept"]) == (256, 4, 32)
         assert t["cpu_share"] == 0.25
         assert old["mst_light_factor_skewed"] == 0.625  # built-in
         P.set_tuning(mst_light_factor_skewed=0.0)
@@ -33,7 +33,7 @@ def test_set_get_tuning_round_trip():
         t = P.get_tuning()
         for bad in ({"blocks_per_cu": 65}, {"mst_block": 384}, {"mst_ept": 32},
                     {"mst_ept_compact": 4}, {"mst_compact_log2": 41},
-                    {"mst_ept": 16},  # 512 x 16 edges per tile: beyond 4096
+                    {"mst_block": 512, "mst_ept": 16},  # 8192 edges per tile: beyond 4096
                     {"mst_split_ratio": 1.0}, {"cc_block": 1024}, {"cc_ept": 4},
                     {"cc_ept_filter": 32}, {"cc_block_filter": 2048},
                     {"cc_block_filter": 1024, "cc_ept_filter": 16}, {"cpu_share": -0.5},
